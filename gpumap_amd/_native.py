@@ -1,0 +1,26 @@
+"""Loader of the compiled engine.
+
+The product path has no CPU fallback: importing gpumap_amd without the built
+extension raises ImportError naming the build command.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+from .build import LIB, core_path
+
+if not (os.path.exists(LIB) and os.path.exists(core_path())):
+    raise ImportError(
+        "gpumap_amd's gfx950 extension is not built "
+        f"(missing {os.path.basename(LIB)} or _core); run "
+        "`python -m gpumap_amd.build`")
+
+# the binding resolves libgpumap_amd.so through its rpath; loading it here
+# first makes the failure message precise if the HIP runtime is missing
+_lib = ctypes.CDLL(LIB, mode=ctypes.RTLD_GLOBAL)
+
+from . import _core  # noqa: E402
+
+__all__ = ["_core", "LIB"]

@@ -1,0 +1,172 @@
+// Shared device definitions of the gfx950 progressive UMAP engine.
+//
+// Neighbour pools: every row owns W = 32 slots (the reference's MAX_NEIGHBORS,
+// cpp/src/kernels/buffers.cuh:17) of packed 64-bit keys
+//     key = float_bits(dist) << 32 | new_flag << 31 | id
+// Distances are never negative, so unsigned key order is (dist, flag, id)
+// order -- the reference's (dist, id) ResultItem order
+// (cpp/src/kernels/update_kernel.cuh:65-69) with the new/old flag
+// (InternalID_t, buffers.cuh:139-164) between them.  An empty slot holds
+// ~0ull, above every real key.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gm {
+
+constexpr int W = 32;        // pool width
+constexpr int CAND = 256;    // insertion candidate cap (reference MAX_CANDIDATES)
+constexpr uint64_t EMPTY = ~0ull;
+constexpr uint64_t NEW_BIT = 0x80000000ull;
+constexpr uint32_t ID_MASK = 0x7fffffffu;
+constexpr uint32_t NO_ID = 0xffffffffu;
+constexpr uint32_t TAG_INIT_NBR = 1, TAG_EMB_INIT = 2, TAG_NOISE = 3,
+                   TAG_NEG = 4;
+
+__host__ __device__ inline uint64_t make_key(float d, uint32_t id,
+                                             bool is_new) {
+  return ((uint64_t)__float_as_uint(d) << 32) | (is_new ? NEW_BIT : 0ull) |
+         (uint64_t)id;
+}
+__host__ __device__ inline uint32_t key_id(uint64_t k) {
+  return (uint32_t)k & ID_MASK;
+}
+__host__ __device__ inline float key_dist(uint64_t k) {
+  return __uint_as_float((uint32_t)(k >> 32));
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), constants as published.
+__host__ __device__ inline void philox(const uint32_t in[4],
+                                       const uint32_t key[2],
+                                       uint32_t out[4]) {
+  uint32_t c0 = in[0], c1 = in[1], c2 = in[2], c3 = in[3];
+  uint32_t k0 = key[0], k1 = key[1];
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+// Counter-mode stream: counter = (a, b, block, tag), key = seed.
+struct Rng {
+  uint32_t key[2];
+  uint32_t ctr[4];
+  uint32_t buf[4];
+  int pos;
+  __device__ Rng(uint64_t seed, uint32_t a, uint32_t b, uint32_t tag) {
+    key[0] = (uint32_t)seed;
+    key[1] = (uint32_t)(seed >> 32);
+    ctr[0] = a;
+    ctr[1] = b;
+    ctr[2] = 0;
+    ctr[3] = tag;
+    pos = 4;
+  }
+  __device__ uint32_t next() {
+    if (pos == 4) {
+      philox(ctr, key, buf);
+      ctr[2]++;
+      pos = 0;
+    }
+    return buf[pos++];
+  }
+  __device__ float normal() { // Box-Muller
+    const float u1 = ((float)next() + 1.0f) * 2.3283064365386963e-10f;
+    const float u2 = (float)next() * 2.3283064365386963e-10f;
+    return sqrtf(-2.0f * logf(u1)) * cosf(6.283185307179586f * u2);
+  }
+};
+
+// Canonical squared distance: one FMA chain over the features in index
+// order.  Every kernel that produces a distance destined for a pool uses this
+// exact operation order, so a pair's distance is bit-identical wherever it is
+// computed (and identical to the CPU restatement).
+__device__ inline float canon_dist_g(const float *__restrict__ a,
+                                     const float *__restrict__ b, int dp) {
+  float acc = 0.0f;
+  for (int i = 0; i < dp; i += 4) {
+    const float4 x = *reinterpret_cast<const float4 *>(a + i);
+    const float4 y = *reinterpret_cast<const float4 *>(b + i);
+    float t = x.x - y.x;
+    acc = __builtin_fmaf(t, t, acc);
+    t = x.y - y.y;
+    acc = __builtin_fmaf(t, t, acc);
+    t = x.z - y.z;
+    acc = __builtin_fmaf(t, t, acc);
+    t = x.w - y.w;
+    acc = __builtin_fmaf(t, t, acc);
+  }
+  return acc;
+}
+
+__device__ inline uint64_t ld_relaxed(const uint64_t *p) {
+  return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline bool cas64(uint64_t *p, uint64_t expect, uint64_t val) {
+  return __hip_atomic_compare_exchange_strong(p, &expect, val,
+                                              __ATOMIC_RELAXED,
+                                              __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Lock-free bounded top-W insert, executed by one 32-lane half of a wave
+// (lane = slot).  Contract (identical to the CPU restatement): reject a
+// duplicate id or a key not below the row maximum; otherwise replace the
+// maximum (lowest slot among equal maxima) by a 64-bit CAS and retry on
+// failure.  Every successful CAS replaces the row's current maximum, so a
+// successful CAS saw the row exactly as it was, the final row is the W
+// smallest distinct keys offered regardless of order, and `rank` (keys below
+// the new one) is exact.
+// Returns 0 rejected, 1 inserted at rank >= k, 2 inserted at rank < k.
+__device__ inline int pool_offer_half(uint64_t *__restrict__ row,
+                                      uint64_t key, int k, int hl, int half) {
+  const uint32_t id = key_id(key);
+  const int base = half * 32;
+  for (;;) {
+    const uint64_t v = ld_relaxed(row + hl);
+    const bool dup = (v != EMPTY) && key_id(v) == id;
+    const uint64_t dmask = (__ballot(dup) >> base) & 0xffffffffull;
+    if (dmask)
+      return 0;
+    uint64_t mx = v;
+    int ml = hl;
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) {
+      const uint64_t ov = __shfl_xor(mx, off, 32);
+      const int ol = __shfl_xor(ml, off, 32);
+      if (ov > mx || (ov == mx && ol < ml)) {
+        mx = ov;
+        ml = ol;
+      }
+    }
+    if (key >= mx)
+      return 0;
+    const uint64_t lmask = (__ballot(v < key) >> base) & 0xffffffffull;
+    const int rank = __popcll(lmask);
+    int ok = 0;
+    if (hl == ml)
+      ok = cas64(row + ml, mx, key) ? 1 : 0;
+    ok = __shfl(ok, ml, 32);
+    if (ok) {
+      return rank < k ? 2 : 1;
+    }
+  }
+}
+
+} // namespace gm

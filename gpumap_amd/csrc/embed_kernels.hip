@@ -1,0 +1,517 @@
+// Fuzzy simplicial set and SGD layout optimiser on gfx950 (north-star
+// subsystems (b) and (c)).
+//
+// Reference: cpp/src/kernels/embedding_kernel.cuh.  The embed-time kNN (self
+// at column 0, the first k-1 stored neighbours after it, Euclidean distances
+// square-rooted; reference prepend_self_knn_kernel :599-631) is not
+// materialised: every kernel reads it through embed_entry() from the public
+// (n x k) arrays.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace gm {
+
+namespace {
+constexpr float FLT_MAX_F = 3.402823466e38f;
+constexpr float FLT_MIN_F = 1.175494351e-38f;
+
+__device__ inline bool valid_d(double d) {
+  return isfinite(d) && d >= 0.0 && d < (double)FLT_MAX_F / 2.0;
+}
+
+__device__ inline void embed_entry(const uint32_t *pidx, const float *pdist,
+                                   uint64_t i, int c, int k, bool euclid,
+                                   uint32_t *id, double *d) {
+  if (c == 0) {
+    *id = (uint32_t)i;
+    *d = 0.0;
+    return;
+  }
+  *id = pidx[i * k + c - 1];
+  const float v = pdist[i * k + c - 1];
+  if (!valid_d((double)v))
+    *d = (double)v;
+  else
+    *d = euclid ? (double)__fsqrt_rn(fmaxf(v, 0.0f)) : (double)v;
+}
+
+__device__ inline double membership(double d, double rho, double sigma) {
+  if (d - rho <= 0.0 || sigma == 0.0)
+    return 1.0;
+  const double v = exp(-((d - rho) / sigma));
+  return v < (double)FLT_MIN_F ? (double)FLT_MIN_F : v;
+}
+
+// order-preserving map of floats to unsigned ints (for atomic min/max)
+__device__ inline uint32_t f2o(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float o2f(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+} // namespace
+
+// ------------------------------------------------------------ mean dist
+// Global mean of the valid embed-time distances (reference
+// compute_mean_dist_kernel :82-128), in FP64 with a fixed reduction order.
+__global__ void k_mean_partial(const uint32_t *__restrict__ pidx,
+                               const float *__restrict__ pdist, uint64_t n,
+                               int k, int euclid, double *__restrict__ psum,
+                               unsigned long long *__restrict__ pcnt) {
+  __shared__ double s_s[256];
+  __shared__ unsigned long long s_c[256];
+  const uint64_t total = n * (uint64_t)k;
+  double s = 0.0;
+  unsigned long long c = 0;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t id;
+    double d;
+    embed_entry(pidx, pdist, t / k, (int)(t % k), k, euclid != 0, &id, &d);
+    if (valid_d(d)) {
+      s += d;
+      ++c;
+    }
+  }
+  s_s[threadIdx.x] = s;
+  s_c[threadIdx.x] = c;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < (unsigned)off) {
+      s_s[threadIdx.x] += s_s[threadIdx.x + off];
+      s_c[threadIdx.x] += s_c[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    psum[blockIdx.x] = s_s[0];
+    pcnt[blockIdx.x] = s_c[0];
+  }
+}
+
+__global__ void k_mean_final(const double *__restrict__ psum,
+                             const unsigned long long *__restrict__ pcnt,
+                             int nparts, double *__restrict__ mean) {
+  if (threadIdx.x != 0 || blockIdx.x != 0)
+    return;
+  double s = 0.0;
+  unsigned long long c = 0;
+  for (int i = 0; i < nparts; ++i) {
+    s += psum[i];
+    c += pcnt[i];
+  }
+  *mean = (c > 0 && s > 0.0) ? s / (double)c : 1.0;
+}
+
+// -------------------------------------------------------- rho / sigma
+// smooth_knn_dist per row in FP64 (reference smooth_knn_dist_kernel
+// :135-207): rho = first non-zero distance, sigma by 64-step bisection of
+// sum_{j=2..k-1} exp(-max(0, d_j - rho)/sigma) = log2(k-1), floor
+// 1e-3 * (row mean, or global mean when rho = 0).
+__global__ void k_sigma(const uint32_t *__restrict__ pidx,
+                        const float *__restrict__ pdist, uint64_t n, int k,
+                        int euclid, const double *__restrict__ mean_p,
+                        double *__restrict__ sigma, double *__restrict__ rho) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const double mean_dist = *mean_p;
+  double d[32];
+  for (int c = 0; c < k; ++c) {
+    uint32_t id;
+    embed_entry(pidx, pdist, i, c, k, euclid != 0, &id, &d[c]);
+  }
+  const double target = log2((double)(k - 1));
+  int start_nz = -1;
+  double sum = 0.0;
+  for (int c = 0; c < k; ++c) {
+    const double di = valid_d(d[c]) ? d[c] : 0.0;
+    sum += di;
+    if (di > 0.0 && start_nz == -1)
+      start_nz = c;
+  }
+  if (start_nz == -1) {
+    rho[i] = 0.0;
+    sigma[i] = 1e-3 * mean_dist;
+    return;
+  }
+  const double row_mean = sum / (double)k;
+  const double r = d[start_nz];
+  double lo = 0.0, hi = INFINITY, mid = 1.0;
+  for (int it = 0; it < 64; ++it) {
+    double psum = 0.0;
+    for (int j = 2; j < k; ++j) {
+      const double dj = valid_d(d[j]) ? d[j] : 0.0;
+      const double x = dj - r;
+      psum += (x > 0.0) ? exp(-x / mid) : 1.0;
+    }
+    if (fabs(psum - target) < 1e-5)
+      break;
+    if (psum > target) {
+      hi = mid;
+      mid = (lo + hi) * 0.5;
+    } else {
+      lo = mid;
+      mid = isinf(hi) ? mid * 2.0 : (lo + hi) * 0.5;
+    }
+  }
+  if (!isfinite(mid) || mid <= 0.0)
+    mid = 1.0;
+  const double min_scale = 1e-3 * (r > 0.0 ? row_mean : mean_dist);
+  rho[i] = r;
+  sigma[i] = mid < min_scale ? min_scale : mid;
+}
+
+// ------------------------------------------------------------ membership
+// Directed weights of the batch rows and the fuzzy union w = p + q - p q
+// (reference compute_membership_strength_kernel :214-250 and
+// symmetrize_and_expand_kernel :266-320, without the self-loop).  flags bit 0
+// = live edge, bit 1 = i is absent from j's list (the reverse edge must be
+// emitted).  Also folds the maximum weight into *maxw_bits.
+__global__ void k_membership(const uint32_t *__restrict__ pidx,
+                             const float *__restrict__ pdist, uint64_t n,
+                             int k, int euclid, const double *__restrict__ sigma,
+                             const double *__restrict__ rho,
+                             const uint32_t *__restrict__ batch, uint64_t nb,
+                             float *__restrict__ w, uint8_t *__restrict__ flags,
+                             uint32_t *__restrict__ maxw_bits) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float wf = 0.0f;
+  if (t < nb * (uint64_t)k) {
+    const uint64_t i = batch[t / k];
+    const int c = (int)(t % k);
+    uint32_t j;
+    double d;
+    embed_entry(pidx, pdist, i, c, k, euclid != 0, &j, &d);
+    uint8_t f = 0;
+    if (j < n && j != i && valid_d(d)) {
+      const double p = membership(d, rho[i], sigma[i]);
+      double q = 0.0;
+      bool found = false;
+      for (int cj = 0; cj < k; ++cj) {
+        uint32_t jj;
+        double dj;
+        embed_entry(pidx, pdist, j, cj, k, euclid != 0, &jj, &dj);
+        if (jj == i) {
+          if (valid_d(dj)) {
+            q = membership(dj, rho[j], sigma[j]);
+            found = true;
+          }
+          break;
+        }
+      }
+      wf = (float)(p + q - p * q);
+      f = 1 | (found ? 0 : 2);
+    }
+    w[t] = wf;
+    flags[t] = f;
+  }
+  // wave max -> one atomic per wave
+  for (int off = 32; off > 0; off >>= 1)
+    wf = fmaxf(wf, __shfl_xor(wf, off));
+  if ((threadIdx.x & 63) == 0 && wf > 0.0f)
+    atomicMax(maxw_bits, __float_as_uint(wf));
+}
+
+// ------------------------------------------------------ COO compaction
+// Keep the live edges with w >= max_w / n_epochs (reference trim_graph
+// :348-400), directed entries first then the reverse entries, each in
+// (batch, slot) order: a deterministic COO.
+__device__ inline bool coo_keep(const float *w, const uint8_t *flags,
+                                uint64_t nd, uint64_t t, float thr) {
+  const uint64_t s = t < nd ? t : t - nd;
+  const uint8_t f = flags[s];
+  if (!(f & 1))
+    return false;
+  if (t >= nd && !(f & 2))
+    return false;
+  return w[s] >= thr;
+}
+
+__device__ inline float coo_thr(const uint32_t *maxw_bits, int n_epochs) {
+  const float mw = __uint_as_float(*maxw_bits);
+  return mw > 0.0f ? __fdiv_rn(mw, (float)n_epochs) : 3.402823466e38f;
+}
+
+__global__ void k_coo_count(const float *__restrict__ w,
+                            const uint8_t *__restrict__ flags, uint64_t nd,
+                            const uint32_t *__restrict__ maxw_bits,
+                            int n_epochs, uint32_t *__restrict__ bcount) {
+  __shared__ uint32_t s_c;
+  if (threadIdx.x == 0)
+    s_c = 0;
+  __syncthreads();
+  const float thr = coo_thr(maxw_bits, n_epochs);
+  const uint64_t t0 = (uint64_t)blockIdx.x * 1024;
+  uint32_t c = 0;
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x) {
+    const uint64_t t = t0 + i;
+    c += (t < 2 * nd && coo_keep(w, flags, nd, t, thr)) ? 1u : 0u;
+  }
+  atomicAdd(&s_c, c);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    bcount[blockIdx.x] = s_c;
+}
+
+__global__ void k_coo_write(const uint32_t *__restrict__ pidx, int k,
+                            const uint32_t *__restrict__ batch,
+                            const float *__restrict__ w,
+                            const uint8_t *__restrict__ flags, uint64_t nd,
+                            const uint32_t *__restrict__ maxw_bits,
+                            int n_epochs, const uint32_t *__restrict__ boff,
+                            int32_t *__restrict__ head,
+                            int32_t *__restrict__ tail,
+                            float *__restrict__ wout) {
+  if (threadIdx.x >= 64)
+    return;
+  const float thr = coo_thr(maxw_bits, n_epochs);
+  const uint64_t t0 = (uint64_t)blockIdx.x * 1024;
+  uint32_t at = boff[blockIdx.x];
+  for (int i = 0; i < 1024; i += 64) {
+    const uint64_t t = t0 + i + threadIdx.x;
+    const bool f = t < 2 * nd && coo_keep(w, flags, nd, t, thr);
+    const uint64_t m = __ballot(f);
+    if (f) {
+      const uint64_t s = t < nd ? t : t - nd;
+      const int32_t hi = (int32_t)batch[s / k];
+      const int c = (int)(s % k);
+      const int32_t hj = c == 0 ? hi : (int32_t)pidx[(uint64_t)hi * k + c - 1];
+      const uint32_t pos = at + __popcll(m & ((1ull << threadIdx.x) - 1ull));
+      head[pos] = t < nd ? hi : hj;
+      tail[pos] = t < nd ? hj : hi;
+      wout[pos] = w[s];
+    }
+    at += __popcll(m);
+  }
+}
+
+// ------------------------------------------------------------ SGD setup
+// epochs_per_sample and epoch state (reference make_epochs_per_sample_kernel
+// :469-477, init_epoch_state_kernel :485-495), and head degrees for the
+// gradient rounding factor (:405-460).
+__global__ void k_sgd_init(const float *__restrict__ w,
+                           const int32_t *__restrict__ head,
+                           const uint32_t *__restrict__ nnz_p,
+                           const uint32_t *__restrict__ maxw_bits,
+                           int n_epochs, float nsr_inv,
+                           float *__restrict__ eps, float *__restrict__ nxt,
+                           float *__restrict__ nneg, int *__restrict__ deg) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= *nnz_p)
+    return;
+  const float mw = __uint_as_float(*maxw_bits);
+  const float ns = (float)n_epochs * __fdiv_rn(w[e], mw);
+  const float ep = ns > 0.0f ? __fdiv_rn((float)n_epochs, ns) : -1.0f;
+  eps[e] = ep;
+  nxt[e] = ep;
+  nneg[e] = ep * nsr_inv;
+  atomicAdd(deg + head[e], 1);
+}
+
+__global__ void k_max_int(const int *__restrict__ v, uint64_t n,
+                          int *__restrict__ out) {
+  int m = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    m = max(m, v[i]);
+  for (int off = 32; off > 0; off >>= 1)
+    m = max(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0 && m > 0)
+    atomicMax(out, m);
+}
+
+__global__ void k_rounding(const int *__restrict__ max_deg, float alpha,
+                           float *__restrict__ rounding) {
+  const int md = *max_deg;
+  if (md == 0) {
+    *rounding = 1.0f;
+    return;
+  }
+  const float max_abs = (float)md * 4.0f * fabsf(alpha);
+  const float delta =
+      __fdiv_rn(max_abs, 1.0f - 2.0f * (float)md * 1.1920928955078125e-7f);
+  int e;
+  frexpf(delta, &e);
+  *rounding = ldexpf(1.0f, e);
+}
+
+// -------------------------------------------------------------- noise
+__global__ void k_minmax_col0(const float *__restrict__ emb, uint64_t n,
+                              int c, uint32_t *__restrict__ mn,
+                              uint32_t *__restrict__ mx) {
+  uint32_t lo = 0xffffffffu, hi = 0u;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t o = f2o(emb[i * c]);
+    lo = min(lo, o);
+    hi = max(hi, o);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, __shfl_xor(lo, off));
+    hi = max(hi, __shfl_xor(hi, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(mn, lo);
+    atomicMax(mx, hi);
+  }
+}
+
+// N(0, 0.001 * x_range) on the selected rows (reference add_embedding_noise
+// :953-1007).
+__global__ __launch_bounds__(256) void k_noise(float *__restrict__ emb, int c,
+                        const uint32_t *__restrict__ batch, uint64_t nb,
+                        const uint32_t *__restrict__ mn,
+                        const uint32_t *__restrict__ mx, float noise_scale,
+                        uint64_t seed) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb * (uint64_t)c)
+    return;
+  const float scale = noise_scale * (o2f(*mx) - o2f(*mn));
+  Rng rng(seed, (uint32_t)t, (uint32_t)(t >> 32), TAG_NOISE);
+  emb[(uint64_t)batch[t / c] * c + t % c] += rng.normal() * scale;
+}
+
+// --------------------------------------------------------------- SGD
+__device__ inline float fclip(float v) {
+  if (!isfinite(v))
+    return 0.0f;
+  return fmaxf(fminf(v, 4.0f), -4.0f);
+}
+
+// One epoch of the Hogwild edge sampler (reference optimize_batch_kernel
+// :509-589) for any n_components <= 4.
+template <int NC>
+__global__ __launch_bounds__(256) void k_sgd_epoch(
+    float *__restrict__ emb, const int32_t *__restrict__ head,
+    const int32_t *__restrict__ tail, const uint32_t *__restrict__ nnz_p,
+    uint32_t n, const float *__restrict__ eps, float *__restrict__ nxt,
+    float *__restrict__ nneg, float alpha, int epoch, float gamma, float a,
+    float b, float nsr_inv, uint64_t seed, const float *__restrict__ rounding_p) {
+  const uint64_t nnz = *nnz_p;
+  const float rounding = *rounding_p;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const float ns = nxt[e];
+    if (ns > (float)epoch)
+      continue;
+    const float ep = eps[e];
+    const float ep_neg = ep * nsr_inv;
+    const int32_t j = head[e], kk = tail[e];
+    float *cur_p = emb + (uint64_t)j * NC;
+    float *oth_p = emb + (uint64_t)kk * NC;
+    float cur[NC], oth[NC], g[NC];
+#pragma unroll
+    for (int d = 0; d < NC; ++d) {
+      cur[d] = cur_p[d];
+      oth[d] = oth_p[d];
+    }
+    float d2 = 0.0f;
+#pragma unroll
+    for (int d = 0; d < NC; ++d) {
+      const float t = cur[d] - oth[d];
+      d2 += t * t;
+    }
+    float attr = 0.0f;
+    if (d2 > 0.0f)
+      attr = (-2.0f * a * b * powf(d2, b - 1.0f)) / (a * powf(d2, b) + 1.0f);
+#pragma unroll
+    for (int d = 0; d < NC; ++d) {
+      g[d] = fclip(attr * (cur[d] - oth[d])) * alpha;
+      cur[d] += g[d];
+    }
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      unsafeAtomicAdd(oth_p + d, (rounding + -g[d]) - rounding);
+    nxt[e] = ns + ep;
+    const float nn = nneg[e];
+    const int n_neg = (int)(((float)epoch - nn) / ep_neg);
+    if (n_neg > 0) {
+      Rng rng(seed, (uint32_t)e, (uint32_t)(e >> 32), TAG_NEG);
+      for (int p = 0; p < n_neg; ++p) {
+        const uint32_t r = rng.next() % n;
+        if ((int32_t)r == j)
+          continue;
+        const float *neg_p = emb + (uint64_t)r * NC;
+        float neg[NC];
+        float d2n = 0.0f;
+#pragma unroll
+        for (int d = 0; d < NC; ++d) {
+          neg[d] = neg_p[d];
+          const float t = cur[d] - neg[d];
+          d2n += t * t;
+        }
+        float rep = 0.0f;
+        if (d2n > 0.0f)
+          rep = (2.0f * gamma * b) / ((0.001f + d2n) * (a * powf(d2n, b) + 1.0f));
+#pragma unroll
+        for (int d = 0; d < NC; ++d) {
+          const float gg =
+              rep > 0.0f ? fclip(rep * (cur[d] - neg[d])) * alpha : 4.0f * alpha;
+          cur[d] += gg;
+          g[d] += gg;
+        }
+      }
+      nneg[e] = nn + (float)n_neg * ep_neg;
+    }
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      unsafeAtomicAdd(cur_p + d, (rounding + g[d]) - rounding);
+  }
+}
+
+template __global__ void k_sgd_epoch<2>(float *, const int32_t *,
+                                        const int32_t *, const uint32_t *,
+                                        uint32_t, const float *, float *,
+                                        float *, float, int, float, float,
+                                        float, float, uint64_t, const float *);
+template __global__ void k_sgd_epoch<3>(float *, const int32_t *,
+                                        const int32_t *, const uint32_t *,
+                                        uint32_t, const float *, float *,
+                                        float *, float, int, float, float,
+                                        float, float, uint64_t, const float *);
+template __global__ void k_sgd_epoch<1>(float *, const int32_t *,
+                                        const int32_t *, const uint32_t *,
+                                        uint32_t, const float *, float *,
+                                        float *, float, int, float, float,
+                                        float, float, uint64_t, const float *);
+template __global__ void k_sgd_epoch<4>(float *, const int32_t *,
+                                        const int32_t *, const uint32_t *,
+                                        uint32_t, const float *, float *,
+                                        float *, float, int, float, float,
+                                        float, float, uint64_t, const float *);
+
+// ----------------------------------------------------------- reposition
+// Unselected rows move to the membership-weighted mean of their embed-time
+// neighbours, read from a snapshot (reference heuristic_neighbor_avg_kernel
+// :811-894), any n_components <= 4.
+__global__ void k_reposition(const uint32_t *__restrict__ pidx,
+                             const float *__restrict__ pdist, uint64_t n,
+                             int k, int euclid, const double *__restrict__ sigma,
+                             const double *__restrict__ rho,
+                             const uint32_t *__restrict__ rows, uint64_t nr,
+                             const float *__restrict__ snap,
+                             float *__restrict__ emb, int c) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nr)
+    return;
+  const uint64_t i = rows[t];
+  double wsum = 0.0, acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int q = 0; q < k; ++q) {
+    uint32_t j;
+    double d;
+    embed_entry(pidx, pdist, i, q, k, euclid != 0, &j, &d);
+    if (j == i || j >= n || !valid_d(d))
+      continue;
+    const double wq = membership(d, rho[i], sigma[i]);
+    wsum += wq;
+    for (int dd = 0; dd < c; ++dd)
+      acc[dd] += wq * snap[(uint64_t)j * c + dd];
+  }
+  if (wsum > 0.0)
+    for (int dd = 0; dd < c; ++dd)
+      emb[i * c + dd] = (float)(acc[dd] / wsum);
+}
+
+} // namespace gm

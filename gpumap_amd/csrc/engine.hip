@@ -1,0 +1,1064 @@
+// Host engine of the gfx950 progressive UMAP path and its C ABI
+// (include/gpumap_c.h).
+//
+// Engine restates the reference's GPUMAP::Impl (cpp/src/impl.cuh:29-534):
+// the same queues (queue.cuh), batch selection, seeds, convergence and
+// completion rules, run() timing and result fields.  Unlike the reference,
+// all graph state lives in HBM for the whole session: the host keeps only the
+// queues and receives, per insertion/update pass, the ascending list of rows
+// whose neighbour lists changed.  One HIP stream per session, used in order
+// (the reference's three streams are used strictly serially,
+// impl.cuh:64-68,161,181,201).
+#include "../../include/gpumap_c.h"
+#include "common.hpp"
+#include "kernels.hpp"
+#include "dlpack_abi.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <numeric>
+#include <queue>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+namespace gm {
+namespace {
+
+thread_local std::string g_err;
+
+struct Fail : std::runtime_error {
+  int code;
+  Fail(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char *what, int line) {
+  if (e == hipSuccess)
+    return;
+  const int code = (e == hipErrorOutOfMemory) ? GM_ERR_OOM : GM_ERR_HIP;
+  throw Fail(code, std::string(what) + " failed at engine.hip:" +
+                       std::to_string(line) + ": " + hipGetErrorString(e));
+}
+#define HIPC(x) ::gm::hip_check((x), #x, __LINE__)
+#define LAUNCH_CHECK() HIPC(hipGetLastError())
+
+template <typename T> T *dalloc(size_t n) {
+  void *p = nullptr;
+  if (n == 0)
+    n = 1;
+  HIPC(hipMalloc(&p, n * sizeof(T)));
+  return static_cast<T *>(p);
+}
+
+inline unsigned blocks_for(uint64_t n, unsigned tpb) {
+  return (unsigned)((n + tpb - 1) / tpb);
+}
+
+struct DedupQueue { // reference UpdateQueue (queue.cuh:46-80)
+  std::queue<uint32_t> q;
+  std::unordered_set<uint32_t> in;
+  size_t size() const { return q.size(); }
+  void push(uint32_t v) {
+    if (in.insert(v).second)
+      q.push(v);
+  }
+  std::vector<uint32_t> pop_n(size_t n) {
+    std::vector<uint32_t> out;
+    n = std::min(n, q.size());
+    out.reserve(n);
+    for (size_t i = 0; i < n; ++i) {
+      out.push_back(q.front());
+      in.erase(q.front());
+      q.pop();
+    }
+    return out;
+  }
+};
+struct Fifo { // reference Queue (queue.cuh:10-40)
+  std::deque<uint32_t> q;
+  size_t size() const { return q.size(); }
+  std::vector<uint32_t> pop_n(size_t n) {
+    n = std::min(n, q.size());
+    std::vector<uint32_t> out(q.begin(), q.begin() + n);
+    q.erase(q.begin(), q.begin() + n);
+    return out;
+  }
+};
+
+constexpr int NPART = 256;
+constexpr int WINDOW_N_BATCHES = 3;
+
+// device scalars, one array
+enum Scalar {
+  S_TOTAL = 0,
+  S_OWNERS,
+  S_MAXW,
+  S_MIN,
+  S_MAX,
+  S_NNZ,
+  S_COUNT
+};
+
+} // namespace
+
+struct Engine {
+  gm_params p;
+  uint64_t N, D, Dp;
+  int k, c;
+  bool euclid;
+  int device = 0;
+  hipStream_t s = nullptr;
+  std::atomic<int> refs{1};
+
+  // device state
+  float *X = nullptr;
+  uint64_t *pool = nullptr;
+  uint32_t *pidx = nullptr;
+  float *pdist = nullptr;
+  float *emb = nullptr;
+  uint8_t *flag = nullptr;
+  uint32_t *list = nullptr;
+  uint32_t *bcount = nullptr;
+  uint32_t *scal = nullptr;
+  unsigned long long *fresh = nullptr;
+  // insertion scratch
+  uint64_t ins_cap = 0;
+  uint32_t *ins_ids = nullptr, *cand = nullptr;
+  int *ins_cnt = nullptr, *ccount = nullptr;
+  float *cdist = nullptr;
+  // update
+  uint32_t *fn = nullptr, *fo = nullptr, *newmask = nullptr,
+           *owner_mark = nullptr, *owners = nullptr, *rows = nullptr;
+  int *fnc = nullptr, *foc = nullptr;
+  uint64_t *rev_new = nullptr, *rev_old = nullptr;
+  // embed
+  double *sigma = nullptr, *rho = nullptr, *psum = nullptr, *mean = nullptr;
+  unsigned long long *pcnt = nullptr;
+  float *w = nullptr, *wout = nullptr, *eps = nullptr, *nxt = nullptr,
+        *nneg = nullptr, *snap = nullptr, *rounding = nullptr;
+  uint8_t *wflags = nullptr;
+  int32_t *head = nullptr, *tail = nullptr;
+  int *deg = nullptr, *maxdeg = nullptr;
+  uint32_t *dbatch = nullptr;
+  // pinned host staging
+  float *h_stage = nullptr;
+  uint64_t h_stage_rows = 0;
+  uint32_t *h_list = nullptr;
+  uint32_t *h_scal = nullptr;
+
+  // host state (reference Impl members, impl.cuh:33-55)
+  uint64_t n = 0;
+  Fifo insertion_queue, embedding_new_queue;
+  DedupQueue update_queue;
+  std::deque<uint32_t> batch_start_history;
+  size_t embedding_cursor = SIZE_MAX;
+  size_t embedding_sweeps_done = 0;
+  bool insertion_completed = false, update_completed = false,
+       embedding_completed = false;
+  bool closed = false;
+  std::mt19937 rng{42};
+
+  explicit Engine(const gm_params &pp) : p(pp) {
+    N = p.n_instances;
+    D = p.n_features;
+    Dp = (D + 3) / 4 * 4;
+    k = (int)p.n_neighbors;
+    c = (int)p.n_components;
+    euclid = p.metric == GM_EUCLIDEAN;
+    HIPC(hipGetDevice(&device));
+    try {
+      allocate();
+    } catch (...) {
+      release();
+      throw;
+    }
+  }
+  ~Engine() { release(); }
+
+  void allocate() {
+    HIPC(hipStreamCreate(&s));
+    const uint64_t nk = N * (uint64_t)k;
+    X = dalloc<float>(N * Dp);
+    HIPC(hipMemsetAsync(X, 0, N * Dp * sizeof(float), s));
+    pool = dalloc<uint64_t>(N * W);
+    HIPC(hipMemsetAsync(pool, 0xff, N * W * sizeof(uint64_t), s));
+    pidx = dalloc<uint32_t>(nk);
+    HIPC(hipMemsetAsync(pidx, 0xff, nk * sizeof(uint32_t), s));
+    pdist = dalloc<float>(nk);
+    HIPC(hipMemsetD32Async((hipDeviceptr_t)pdist, 0x7f7fffff, nk, s));
+    emb = dalloc<float>(N * c);
+    HIPC(hipMemsetAsync(emb, 0, N * c * sizeof(float), s));
+    flag = dalloc<uint8_t>(N);
+    HIPC(hipMemsetAsync(flag, 0, N, s));
+    list = dalloc<uint32_t>(N);
+    bcount = dalloc<uint32_t>(std::max<uint64_t>(2 * nk, N) / 1024 + 2);
+    scal = dalloc<uint32_t>(S_COUNT);
+    fresh = dalloc<unsigned long long>(1);
+    fn = dalloc<uint32_t>(N * W);
+    fo = dalloc<uint32_t>(N * W);
+    newmask = dalloc<uint32_t>(N);
+    owner_mark = dalloc<uint32_t>(N);
+    HIPC(hipMemsetAsync(owner_mark, 0, N * sizeof(uint32_t), s));
+    owners = dalloc<uint32_t>(N);
+    rows = dalloc<uint32_t>(N);
+    fnc = dalloc<int>(N);
+    foc = dalloc<int>(N);
+    HIPC(hipMemsetAsync(fnc, 0, N * sizeof(int), s));
+    HIPC(hipMemsetAsync(foc, 0, N * sizeof(int), s));
+    rev_new = dalloc<uint64_t>(N * W);
+    rev_old = dalloc<uint64_t>(N * W);
+    HIPC(hipMemsetAsync(rev_new, 0xff, N * W * sizeof(uint64_t), s));
+    HIPC(hipMemsetAsync(rev_old, 0xff, N * W * sizeof(uint64_t), s));
+    sigma = dalloc<double>(N);
+    rho = dalloc<double>(N);
+    psum = dalloc<double>(NPART);
+    pcnt = dalloc<unsigned long long>(NPART);
+    mean = dalloc<double>(1);
+    w = dalloc<float>(nk);
+    wflags = dalloc<uint8_t>(nk);
+    head = dalloc<int32_t>(2 * nk);
+    tail = dalloc<int32_t>(2 * nk);
+    wout = dalloc<float>(2 * nk);
+    eps = dalloc<float>(2 * nk);
+    nxt = dalloc<float>(2 * nk);
+    nneg = dalloc<float>(2 * nk);
+    snap = dalloc<float>(N * c);
+    rounding = dalloc<float>(1);
+    deg = dalloc<int>(N);
+    maxdeg = dalloc<int>(1);
+    dbatch = dalloc<uint32_t>(N);
+    HIPC(hipHostMalloc((void **)&h_list, N * sizeof(uint32_t)));
+    HIPC(hipHostMalloc((void **)&h_scal, S_COUNT * sizeof(uint32_t) + 64));
+    HIPC(hipStreamSynchronize(s));
+  }
+
+  void ensure_ins(uint64_t B) {
+    if (B <= ins_cap)
+      return;
+    (void)hipFree(ins_ids);
+    (void)hipFree(cand);
+    (void)hipFree(ins_cnt);
+    (void)hipFree(ccount);
+    (void)hipFree(cdist);
+    ins_ids = cand = nullptr;
+    ins_cnt = ccount = nullptr;
+    cdist = nullptr;
+    ins_cap = 0;
+    ins_ids = dalloc<uint32_t>(B * W);
+    ins_cnt = dalloc<int>(B);
+    cand = dalloc<uint32_t>(B * CAND);
+    cdist = dalloc<float>(B * CAND);
+    ccount = dalloc<int>(B);
+    ins_cap = B;
+  }
+
+  void ensure_stage(uint64_t rows_) {
+    if (rows_ <= h_stage_rows)
+      return;
+    if (h_stage)
+      (void)hipHostFree(h_stage);
+    h_stage = nullptr;
+    h_stage_rows = 0;
+    HIPC(hipHostMalloc((void **)&h_stage, rows_ * D * sizeof(float)));
+    h_stage_rows = rows_;
+  }
+
+  void release() {
+    if (s)
+      (void)hipStreamSynchronize(s);
+    void *ptrs[] = {X,     pool,    pidx,    pdist,   emb,        flag,
+                    list,  bcount,  scal,    fresh,   ins_ids,    cand,
+                    ins_cnt, ccount, cdist,  fn,      fo,         newmask,
+                    owner_mark, owners, rows, fnc,    foc,        rev_new,
+                    rev_old, sigma,  rho,    psum,    pcnt,       mean,
+                    w,     wflags,  head,    tail,    wout,       eps,
+                    nxt,   nneg,    snap,    rounding, deg,       maxdeg,
+                    dbatch};
+    for (void *q : ptrs)
+      if (q)
+        (void)(void)hipFree(q);
+    if (h_stage)
+      (void)hipHostFree(h_stage);
+    if (h_list)
+      (void)hipHostFree(h_list);
+    if (h_scal)
+      (void)hipHostFree(h_scal);
+    h_stage = nullptr;
+    h_list = nullptr;
+    h_scal = nullptr;
+    if (s)
+      (void)hipStreamDestroy(s);
+    s = nullptr;
+  }
+
+  void log(const char *msg) const {
+    if (p.verbose) {
+      fprintf(stderr, "[gpumap_amd] %s\n", msg);
+      fflush(stderr);
+    }
+  }
+
+  // ------------------------------------------------------------- append
+  void append(const float *x, uint64_t nr) {
+    if (nr == 0)
+      return;
+    if (n + nr > N)
+      throw Fail(GM_ERR_CAPACITY, "Exceeding maximum number of rows in storage");
+    ensure_stage(nr);
+    std::memcpy(h_stage, x, nr * D * sizeof(float));
+    HIPC(hipMemcpy2DAsync(X + n * Dp, Dp * sizeof(float), h_stage,
+                          D * sizeof(float), D * sizeof(float), nr,
+                          hipMemcpyHostToDevice, s));
+    HIPC(hipStreamSynchronize(s));
+    for (uint64_t r = 0; r < nr; ++r) {
+      insertion_queue.q.push_back((uint32_t)(n + r));
+      embedding_new_queue.q.push_back((uint32_t)(n + r));
+    }
+    n += nr;
+  }
+
+  // ascending rows with flag set in [0, hi); clears the flags
+  uint64_t collect_flags(uint64_t hi, std::vector<uint32_t> &out) {
+    out.clear();
+    if (hi == 0)
+      return 0;
+    const unsigned nblk = blocks_for(hi, 1024);
+    k_flag_count<<<nblk, 256, 0, s>>>(flag, 0, hi, bcount);
+    LAUNCH_CHECK();
+    k_scan_small<<<1, 1024, 0, s>>>(bcount, nblk, scal + S_TOTAL);
+    LAUNCH_CHECK();
+    k_flag_write<<<nblk, 64, 0, s>>>(flag, 0, hi, bcount, list);
+    LAUNCH_CHECK();
+    HIPC(hipMemcpyAsync(h_scal, scal + S_TOTAL, sizeof(uint32_t),
+                        hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    const uint64_t cnt = h_scal[0];
+    if (cnt) {
+      HIPC(hipMemcpyAsync(h_list, list, cnt * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, s));
+      HIPC(hipStreamSynchronize(s));
+      out.assign(h_list, h_list + cnt);
+    }
+    return cnt;
+  }
+
+  // public top-k rows for `rows_` (host list); returns fresh edge count
+  uint64_t sync_rows(const std::vector<uint32_t> &rows_) {
+    if (rows_.empty())
+      return 0;
+    std::memcpy(h_list, rows_.data(), rows_.size() * sizeof(uint32_t));
+    HIPC(hipMemcpyAsync(rows, h_list, rows_.size() * sizeof(uint32_t),
+                        hipMemcpyHostToDevice, s));
+    HIPC(hipMemsetAsync(fresh, 0, sizeof(unsigned long long), s));
+    k_sync_rows<<<blocks_for(rows_.size() * 32, 256), 256, 0, s>>>(
+        pool, rows, rows_.size(), pidx, pdist, k, fresh);
+    LAUNCH_CHECK();
+    unsigned long long f = 0;
+    HIPC(hipMemcpyAsync(h_scal, fresh, sizeof(f), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    std::memcpy(&f, h_scal, sizeof(f));
+    return f;
+  }
+
+  // ---------------------------------------------------------- insertion
+  // reference Impl::insert (impl.cuh:220-253) + run_insertion
+  // (insertion_kernel.cuh:926-1095), graph kept on the device.
+  uint32_t insert(uint32_t n_ops) {
+    if (insertion_queue.size() == 0 || n_ops == 0)
+      return 0;
+    std::vector<uint32_t> batch = insertion_queue.pop_n(n_ops);
+    batch_start_history.push_back(batch.front());
+    if ((int)batch_start_history.size() > WINDOW_N_BATCHES)
+      batch_start_history.pop_front();
+    const uint32_t window_start = batch_start_history.front();
+    const uint32_t seed = n_ops * 9973u + (uint32_t)n * 7919u;
+    const uint64_t b0 = batch.front(), B = batch.size();
+    ensure_ins(B);
+    if (!euclid) {
+      k_prep_rows<<<(unsigned)B, 64, 0, s>>>(X, Dp, D, b0, B);
+      LAUNCH_CHECK();
+    }
+    k_random_ids<<<blocks_for(B, 64), 64, 0, s>>>(ins_ids, ins_cnt, b0, B, n,
+                                                  window_start, seed);
+    LAUNCH_CHECK();
+    k_init_dist<<<blocks_for(B * W, 256), 256, 0, s>>>(X, Dp, pool, ins_ids,
+                                                       ins_cnt, b0, B);
+    LAUNCH_CHECK();
+    for (int it = 0; it < 10; ++it) {
+      k_ins_candidates<<<(unsigned)B, 256, 0, s>>>(X, Dp, pool, b0, B, k, cand,
+                                                   cdist, ccount);
+      LAUNCH_CHECK();
+      k_ins_offers<<<(unsigned)B, 256, 0, s>>>(pool, flag, cand, cdist, ccount,
+                                               b0, B, k);
+      LAUNCH_CHECK();
+    }
+    std::vector<uint32_t> flagged;
+    collect_flags(n, flagged);
+    std::vector<uint32_t> affected(batch.begin(), batch.end());
+    for (uint32_t r : flagged)
+      if (r < b0)
+        affected.push_back(r);
+    sync_rows(affected);
+    k_init_embedding<<<blocks_for(B, 64), 64, 0, s>>>(emb, c, pidx, pdist, k,
+                                                      b0, B);
+    LAUNCH_CHECK();
+    for (uint32_t r : affected)
+      update_queue.push(r);
+    return (uint32_t)B;
+  }
+
+  // ------------------------------------------------------------- update
+  // One NN-Descent pass over `b` (reference UpdateAlgo::run,
+  // update_kernel.cuh:913-1054); returns the fresh top-k edge count.
+  uint64_t nnd_pass(std::vector<uint32_t> b, std::vector<uint32_t> &affected) {
+    std::sort(b.begin(), b.end());
+    b.erase(std::unique(b.begin(), b.end()), b.end());
+    const uint64_t B = b.size();
+    std::memcpy(h_list, b.data(), B * sizeof(uint32_t));
+    HIPC(hipMemcpyAsync(rows, h_list, B * sizeof(uint32_t),
+                        hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(owners, rows, B * sizeof(uint32_t),
+                        hipMemcpyDeviceToDevice, s));
+    h_scal[8] = (uint32_t)B;
+    HIPC(hipMemcpyAsync(scal + S_OWNERS, h_scal + 8, sizeof(uint32_t),
+                        hipMemcpyHostToDevice, s));
+    k_sample<<<blocks_for(B * 32, 256), 256, 0, s>>>(pool, rows, B, fn, fo, fnc,
+                                                     foc, newmask, owner_mark);
+    LAUNCH_CHECK();
+    k_reverse<<<blocks_for(B * 32, 256), 256, 0, s>>>(
+        pool, rows, B, newmask, rev_new, rev_old, owner_mark, owners,
+        scal + S_OWNERS);
+    LAUNCH_CHECK();
+    const unsigned jgrid = (unsigned)std::min<uint64_t>(n, 2048);
+    k_join<<<jgrid, 256, 0, s>>>(X, Dp, pool, fn, fo, fnc, foc, rev_new,
+                                 rev_old, owners, scal + S_OWNERS, flag, k);
+    LAUNCH_CHECK();
+    k_join_reset<<<blocks_for(n * W, 256), 256, 0, s>>>(
+        owners, scal + S_OWNERS, rev_new, rev_old, owner_mark, fnc, foc);
+    LAUNCH_CHECK();
+    collect_flags(n, affected);
+    return sync_rows(affected);
+  }
+
+  // reference Impl::update (impl.cuh:255-333)
+  uint32_t update(uint32_t n_ops, uint64_t &changed_out) {
+    changed_out = 0;
+    if (n_ops == 0 || update_completed || n == 0)
+      return 0;
+    // converged once the queue holds fewer than 1e-4 n rows; at least the
+    // empty queue counts (reference impl.cuh:261-263 uses a threshold of 0
+    // below n = 10,000, which no queue size satisfies)
+    const size_t thr = std::max<size_t>(1, (size_t)(0.0001f * (float)n));
+    uint32_t left = n_ops;
+    while (left > 0) {
+      std::vector<uint32_t> b;
+      if ((size_t)left >= n) {
+        update_queue.pop_n(update_queue.size());
+        b.resize(n);
+        std::iota(b.begin(), b.end(), 0u);
+      } else if ((size_t)left >= update_queue.size()) {
+        b = update_queue.pop_n(update_queue.size());
+        std::unordered_set<uint32_t> seen(b.begin(), b.end());
+        std::uniform_int_distribution<uint32_t> dd(0, (uint32_t)(n - 1));
+        uint32_t off = dd(rng);
+        while (b.size() < (size_t)left) {
+          const uint32_t cnd = (off++) % (uint32_t)n;
+          if (seen.insert(cnd).second)
+            b.push_back(cnd);
+        }
+      } else {
+        b = update_queue.pop_n(left);
+      }
+      if (b.empty())
+        break;
+      std::vector<uint32_t> affected;
+      const uint32_t bsz = (uint32_t)b.size();
+      changed_out += nnd_pass(std::move(b), affected);
+      left -= bsz;
+      for (uint32_t r : affected)
+        update_queue.push(r);
+      if (update_queue.size() < thr) {
+        update_completed = (n == N);
+        break;
+      }
+    }
+    return n_ops - left;
+  }
+
+  // --------------------------------------------------------- fuzzy set
+  // Builds the trimmed COO of the batch rows in dbatch[0, nb); leaves nnz in
+  // scal[S_NNZ] and the maximum weight in scal[S_MAXW].
+  void build_fuzzy(uint64_t nb) {
+    k_mean_partial<<<NPART, 256, 0, s>>>(pidx, pdist, n, k, euclid ? 1 : 0,
+                                         psum, pcnt);
+    LAUNCH_CHECK();
+    k_mean_final<<<1, 1, 0, s>>>(psum, pcnt, NPART, mean);
+    LAUNCH_CHECK();
+    k_sigma<<<blocks_for(n, 128), 128, 0, s>>>(pidx, pdist, n, k,
+                                               euclid ? 1 : 0, mean, sigma, rho);
+    LAUNCH_CHECK();
+    HIPC(hipMemsetAsync(scal + S_MAXW, 0, sizeof(uint32_t), s));
+    const uint64_t nd = nb * (uint64_t)k;
+    k_membership<<<blocks_for(nd, 256), 256, 0, s>>>(
+        pidx, pdist, n, k, euclid ? 1 : 0, sigma, rho, dbatch, nb, w, wflags,
+        scal + S_MAXW);
+    LAUNCH_CHECK();
+    const unsigned nblk = blocks_for(2 * nd, 1024);
+    k_coo_count<<<nblk, 256, 0, s>>>(w, wflags, nd, scal + S_MAXW,
+                                     (int)p.max_epoch, bcount);
+    LAUNCH_CHECK();
+    k_scan_small<<<1, 1024, 0, s>>>(bcount, nblk, scal + S_NNZ);
+    LAUNCH_CHECK();
+    k_coo_write<<<nblk, 64, 0, s>>>(pidx, k, dbatch, w, wflags, nd,
+                                    scal + S_MAXW, (int)p.max_epoch, bcount,
+                                    head, tail, wout);
+    LAUNCH_CHECK();
+  }
+
+  // SGD on the COO in head/tail/wout with nnz in scal[S_NNZ] and the
+  // maximum weight in scal[S_MAXW] (reference run_optimize_layout,
+  // embedding_kernel.cuh:726-798).  `cap` bounds nnz for the launch grids.
+  void run_sgd(float *e, uint64_t nrows, uint64_t cap, int n_epochs, int nsr,
+               float a, float b, float alpha0, float gamma, uint64_t seed) {
+    if (cap == 0 || n_epochs <= 0)
+      return;
+    HIPC(hipMemsetAsync(deg, 0, nrows * sizeof(int), s));
+    HIPC(hipMemsetAsync(maxdeg, 0, sizeof(int), s));
+    const float nsr_inv = 1.0f / (float)nsr;
+    k_sgd_init<<<blocks_for(cap, 256), 256, 0, s>>>(
+        wout, head, scal + S_NNZ, scal + S_MAXW, n_epochs, nsr_inv, eps, nxt,
+        nneg, deg);
+    LAUNCH_CHECK();
+    k_max_int<<<std::min<unsigned>(blocks_for(nrows, 256), 1024), 256, 0, s>>>(
+        deg, nrows, maxdeg);
+    LAUNCH_CHECK();
+    k_rounding<<<1, 1, 0, s>>>(maxdeg, alpha0, rounding);
+    LAUNCH_CHECK();
+    const unsigned grid = std::min<unsigned>(blocks_for(cap, 256), 16384);
+    float alpha = alpha0;
+    for (int epoch = 0; epoch < n_epochs; ++epoch) {
+      const uint64_t es = seed + (uint64_t)epoch;
+      switch (c) {
+      case 1:
+        k_sgd_epoch<1><<<grid, 256, 0, s>>>(e, head, tail, scal + S_NNZ,
+                                            (uint32_t)nrows, eps, nxt, nneg,
+                                            alpha, epoch, gamma, a, b, nsr_inv,
+                                            es, rounding);
+        break;
+      case 2:
+        k_sgd_epoch<2><<<grid, 256, 0, s>>>(e, head, tail, scal + S_NNZ,
+                                            (uint32_t)nrows, eps, nxt, nneg,
+                                            alpha, epoch, gamma, a, b, nsr_inv,
+                                            es, rounding);
+        break;
+      case 3:
+        k_sgd_epoch<3><<<grid, 256, 0, s>>>(e, head, tail, scal + S_NNZ,
+                                            (uint32_t)nrows, eps, nxt, nneg,
+                                            alpha, epoch, gamma, a, b, nsr_inv,
+                                            es, rounding);
+        break;
+      default:
+        k_sgd_epoch<4><<<grid, 256, 0, s>>>(e, head, tail, scal + S_NNZ,
+                                            (uint32_t)nrows, eps, nxt, nneg,
+                                            alpha, epoch, gamma, a, b, nsr_inv,
+                                            es, rounding);
+      }
+      LAUNCH_CHECK();
+      alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
+    }
+  }
+
+  // -------------------------------------------------------------- embed
+  // reference Impl::embed (impl.cuh:335-500)
+  uint32_t embed(uint32_t n_ops) {
+    if (n_ops == 0 || n == 0)
+      return 0;
+    std::vector<uint32_t> batch;
+    if ((size_t)n_ops >= n) {
+      embedding_new_queue.pop_n(embedding_new_queue.size());
+      batch.resize(n);
+      std::iota(batch.begin(), batch.end(), 0u);
+      embedding_cursor = n - 1;
+      if (embedding_cursor == N - 1 && update_completed)
+        ++embedding_sweeps_done;
+    } else {
+      if (embedding_new_queue.size() > 0) {
+        auto it = embedding_new_queue.pop_n(embedding_new_queue.size());
+        batch.insert(batch.end(), it.begin(), it.end());
+      }
+      size_t left = batch.size() >= n_ops ? 0 : n_ops - batch.size();
+      if (left > 0) {
+        std::unordered_set<uint32_t> seen(batch.begin(), batch.end());
+        size_t cur =
+            embedding_cursor == SIZE_MAX ? 0 : (embedding_cursor + 1) % n;
+        for (size_t chk = 0; left > 0 && chk < n; ++chk) {
+          if (seen.insert((uint32_t)cur).second) {
+            batch.push_back((uint32_t)cur);
+            embedding_cursor = cur;
+            if (embedding_cursor == N - 1 && update_completed)
+              ++embedding_sweeps_done;
+            --left;
+          }
+          cur = (cur + 1) % n;
+        }
+      }
+    }
+    const uint64_t nb = batch.size();
+    std::vector<uint32_t> unsel;
+    if (nb < n) {
+      std::vector<uint8_t> in_batch(n, 0);
+      for (uint32_t id : batch)
+        in_batch[id] = 1;
+      for (uint64_t i = 0; i < n; ++i)
+        if (!in_batch[i])
+          unsel.push_back((uint32_t)i);
+    }
+    std::memcpy(h_list, batch.data(), nb * sizeof(uint32_t));
+    HIPC(hipMemcpyAsync(dbatch, h_list, nb * sizeof(uint32_t),
+                        hipMemcpyHostToDevice, s));
+    build_fuzzy(nb);
+    if (embedding_sweeps_done > 2)
+      embedding_completed = update_completed;
+
+    // noise (reference add_embedding_noise, embedding_kernel.cuh:953-1007)
+    h_scal[8] = 0xffffffffu;
+    h_scal[9] = 0u;
+    HIPC(hipMemcpyAsync(scal + S_MIN, h_scal + 8, 2 * sizeof(uint32_t),
+                        hipMemcpyHostToDevice, s));
+    k_minmax_col0<<<std::min<unsigned>(blocks_for(n, 256), 1024), 256, 0, s>>>(
+        emb, n, c, scal + S_MIN, scal + S_MAX);
+    LAUNCH_CHECK();
+    k_noise<<<blocks_for(nb * c, 256), 256, 0, s>>>(
+        emb, c, dbatch, nb, scal + S_MIN, scal + S_MAX, 0.001f,
+        42ull ^ (uint64_t)embedding_sweeps_done);
+    LAUNCH_CHECK();
+    run_sgd(emb, n, 2 * nb * (uint64_t)k, (int)p.max_epoch,
+            (int)p.negative_sample_rate, p.a, p.b, p.alpha, 1.0f, 42ull);
+
+    // reposition of unselected rows (heuristic_neighbor_avg_kernel :811-894)
+    if (!unsel.empty()) {
+      HIPC(hipStreamSynchronize(s)); // h_list still feeds the batch upload
+      std::memcpy(h_list, unsel.data(), unsel.size() * sizeof(uint32_t));
+      HIPC(hipMemcpyAsync(rows, h_list, unsel.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
+      HIPC(hipMemcpyAsync(snap, emb, n * c * sizeof(float),
+                          hipMemcpyDeviceToDevice, s));
+      k_reposition<<<blocks_for(unsel.size(), 128), 128, 0, s>>>(
+          pidx, pdist, n, k, euclid ? 1 : 0, sigma, rho, rows, unsel.size(),
+          snap, emb, c);
+      LAUNCH_CHECK();
+    }
+    return (uint32_t)nb;
+  }
+
+  // reference Impl::run (impl.cuh:146-218)
+  void run(uint32_t ni, uint32_t nu, uint32_t ne, gm_result *r) {
+    using clk = std::chrono::steady_clock;
+    std::memset(r, 0, sizeof(*r));
+    if (!insertion_completed) {
+      auto t0 = clk::now();
+      r->n_insertion_ops = insert(ni);
+      HIPC(hipStreamSynchronize(s));
+      r->insertion_time = std::chrono::duration<float>(clk::now() - t0).count();
+      insertion_completed = insertion_queue.size() == 0 && n == N;
+    }
+    if (!(nu == 0 || update_completed)) {
+      uint64_t ch = 0;
+      auto t0 = clk::now();
+      r->n_update_ops = update(nu, ch);
+      HIPC(hipStreamSynchronize(s));
+      r->update_time = std::chrono::duration<float>(clk::now() - t0).count();
+      r->n_update_changed_edges = ch;
+    }
+    r->embedding_pending_edges = n;
+    if (!(ne == 0 || n == 0 || embedding_completed)) {
+      auto t0 = clk::now();
+      r->n_embedding_ops = embed(ne);
+      HIPC(hipStreamSynchronize(s));
+      r->embedding_time = std::chrono::duration<float>(clk::now() - t0).count();
+    }
+    r->n_inserted_instances = n;
+    r->insertion_queue_size = insertion_queue.size();
+    r->update_queue_size = update_queue.size();
+    r->embedding_queue_size = r->embedding_pending_edges;
+    r->insertion_completed = insertion_completed;
+    r->update_completed = update_completed;
+    r->embedding_completed = embedding_completed;
+  }
+};
+
+// ---------------------------------------------------------------- DLPack
+struct Export {
+  DLManagedTensor mt;
+  int64_t shape[2];
+  int64_t strides[2];
+};
+
+void engine_unref(Engine *e) {
+  if (e->refs.fetch_sub(1) == 1)
+    delete e;
+}
+
+void dl_deleter(DLManagedTensor *self) {
+  Export *x = reinterpret_cast<Export *>(self);
+  engine_unref(static_cast<Engine *>(self->manager_ctx));
+  delete x;
+}
+
+__global__ void k_philox_probe(const uint32_t *in, uint32_t *out) {
+  uint32_t c[4] = {in[0], in[1], in[2], in[3]};
+  uint32_t kk[2] = {in[4], in[5]};
+  uint32_t o[4];
+  philox(c, kk, o);
+  for (int i = 0; i < 4; ++i)
+    out[i] = o[i];
+}
+
+} // namespace gm
+
+struct gm_handle {
+  gm::Engine *e;
+};
+
+using gm::Fail;
+
+namespace {
+template <typename F> int guarded(F &&f) {
+  try {
+    f();
+    gm::g_err.clear();
+    return GM_OK;
+  } catch (const Fail &x) {
+    gm::g_err = x.what();
+    return x.code;
+  } catch (const std::bad_alloc &) {
+    gm::g_err = "host allocation failed";
+    return GM_ERR_OOM;
+  } catch (const std::exception &x) {
+    gm::g_err = x.what();
+    return GM_ERR_HIP;
+  }
+}
+
+void check_handle(gm_handle *h) {
+  if (!h || !h->e)
+    throw Fail(GM_ERR_INVALID, "null or destroyed gm_handle");
+  HIPC(hipSetDevice(h->e->device));
+}
+
+void view_of(gm::Engine *e, int32_t which, gm_view *v) {
+  std::memset(v, 0, sizeof(*v));
+  v->rows = e->n;
+  v->device_id = e->device;
+  switch (which) {
+  case GM_EMBEDDING:
+    v->data = e->emb;
+    v->cols = (uint64_t)e->c;
+    v->row_stride = (uint64_t)e->c;
+    v->dtype_code = 2;
+    v->dtype_bits = 32;
+    break;
+  case GM_NEIGHBOR_INDICES:
+    v->data = e->pidx;
+    v->cols = (uint64_t)e->k;
+    v->row_stride = (uint64_t)e->k;
+    v->dtype_code = 1;
+    v->dtype_bits = 32;
+    break;
+  case GM_NEIGHBOR_DISTS:
+    v->data = e->pdist;
+    v->cols = (uint64_t)e->k;
+    v->row_stride = (uint64_t)e->k;
+    v->dtype_code = 2;
+    v->dtype_bits = 32;
+    break;
+  default:
+    throw Fail(GM_ERR_INVALID, "unknown array id " + std::to_string(which));
+  }
+}
+} // namespace
+
+extern "C" {
+
+void gm_default_params(gm_params *o) {
+  if (!o)
+    return;
+  std::memset(o, 0, sizeof(*o));
+  o->n_neighbors = 15;
+  o->n_components = 2;
+  o->min_dist = 0.1f;
+  o->negative_sample_rate = 5;
+  o->max_epoch = 200;
+  o->a = 1.0f;
+  o->b = 1.0f;
+  o->alpha = 1.0f;
+  o->verbose = 0;
+  o->metric = GM_EUCLIDEAN;
+}
+
+int gm_create(const gm_params *params, gm_handle **out) {
+  return guarded([&] {
+    if (!params || !out)
+      throw Fail(GM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const gm_params &q = *params;
+    if (q.n_neighbors < 2 || q.n_neighbors > (uint64_t)gm::W)
+      throw Fail(GM_ERR_INVALID, "n_neighbors must be in [2, 32]");
+    if (q.n_components < 1 || q.n_components > 4)
+      throw Fail(GM_ERR_INVALID, "n_components must be in [1, 4]");
+    if (q.n_instances == 0 || q.n_instances >= 0x7fffffffull)
+      throw Fail(GM_ERR_INVALID, "n_instances must be in [1, 2^31-1)");
+    if (q.n_features == 0)
+      throw Fail(GM_ERR_INVALID, "n_features must be positive");
+    if (q.metric != GM_EUCLIDEAN && q.metric != GM_COSINE)
+      throw Fail(GM_ERR_INVALID, "metric must be 0 (euclidean) or 1 (cosine)");
+    if (q.negative_sample_rate < 1)
+      throw Fail(GM_ERR_INVALID, "negative_sample_rate must be >= 1");
+    if (q.max_epoch > 1000000)
+      throw Fail(GM_ERR_INVALID, "max_epoch too large");
+    auto *h = new gm_handle{nullptr};
+    try {
+      h->e = new gm::Engine(q);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int gm_initialize(gm_handle *h, uint64_t n_instances, uint64_t n_features) {
+  return guarded([&] {
+    check_handle(h);
+    if (n_instances != h->e->N || n_features != h->e->D)
+      throw Fail(GM_ERR_INVALID,
+                 "initialize(): sizes differ from the params the handle was "
+                 "created with");
+  });
+}
+
+int gm_append_data(gm_handle *h, const float *x, uint64_t rows,
+                   uint64_t cols) {
+  return guarded([&] {
+    check_handle(h);
+    if (cols != h->e->D)
+      throw Fail(GM_ERR_INVALID,
+                 "n_features mismatch with numpy array second dimension.");
+    if (rows == 0)
+      return;
+    if (!x)
+      throw Fail(GM_ERR_INVALID, "null data pointer");
+    h->e->append(x, rows);
+  });
+}
+
+int gm_run(gm_handle *h, uint32_t ni, uint32_t nu, uint32_t ne,
+           gm_result *out) {
+  return guarded([&] {
+    check_handle(h);
+    if (!out)
+      throw Fail(GM_ERR_INVALID, "null result pointer");
+    h->e->run(ni, nu, ne, out);
+  });
+}
+
+int gm_close(gm_handle *h) {
+  return guarded([&] {
+    check_handle(h);
+    HIPC(hipStreamSynchronize(h->e->s));
+    h->e->closed = true;
+  });
+}
+
+void gm_destroy(gm_handle *h) {
+  if (!h)
+    return;
+  if (h->e) {
+    (void)hipSetDevice(h->e->device);
+    gm::engine_unref(h->e);
+  }
+  h->e = nullptr;
+  delete h;
+}
+
+int gm_get_view(gm_handle *h, int32_t which, gm_view *out) {
+  return guarded([&] {
+    check_handle(h);
+    if (!out)
+      throw Fail(GM_ERR_INVALID, "null view pointer");
+    view_of(h->e, which, out);
+  });
+}
+
+int gm_export_dlpack(gm_handle *h, int32_t which, void **dl) {
+  return guarded([&] {
+    check_handle(h);
+    if (!dl)
+      throw Fail(GM_ERR_INVALID, "null output pointer");
+    gm_view v;
+    view_of(h->e, which, &v);
+    HIPC(hipStreamSynchronize(h->e->s));
+    auto *x = new gm::Export();
+    x->shape[0] = (int64_t)v.rows;
+    x->shape[1] = (int64_t)v.cols;
+    x->strides[0] = (int64_t)v.row_stride;
+    x->strides[1] = 1;
+    x->mt.dl_tensor.data = v.data;
+    x->mt.dl_tensor.device = {10 /* kDLROCM */, v.device_id};
+    x->mt.dl_tensor.ndim = 2;
+    x->mt.dl_tensor.dtype = {(uint8_t)v.dtype_code, (uint8_t)v.dtype_bits, 1};
+    x->mt.dl_tensor.shape = x->shape;
+    x->mt.dl_tensor.strides = x->strides;
+    x->mt.dl_tensor.byte_offset = 0;
+    x->mt.manager_ctx = h->e;
+    x->mt.deleter = gm::dl_deleter;
+    h->e->refs.fetch_add(1);
+    *dl = &x->mt;
+  });
+}
+
+int gm_copy_knn(gm_handle *h, uint32_t *idx, float *dist) {
+  return guarded([&] {
+    check_handle(h);
+    gm::Engine *e = h->e;
+    const size_t nk = e->n * (size_t)e->k;
+    if (idx)
+      HIPC(hipMemcpyAsync(idx, e->pidx, nk * 4, hipMemcpyDeviceToHost, e->s));
+    if (dist)
+      HIPC(hipMemcpyAsync(dist, e->pdist, nk * 4, hipMemcpyDeviceToHost, e->s));
+    HIPC(hipStreamSynchronize(e->s));
+  });
+}
+
+int gm_copy_embedding(gm_handle *h, float *emb) {
+  return guarded([&] {
+    check_handle(h);
+    gm::Engine *e = h->e;
+    HIPC(hipMemcpyAsync(emb, e->emb, e->n * e->c * 4, hipMemcpyDeviceToHost,
+                        e->s));
+    HIPC(hipStreamSynchronize(e->s));
+  });
+}
+
+int gm_set_embedding(gm_handle *h, const float *emb) {
+  return guarded([&] {
+    check_handle(h);
+    gm::Engine *e = h->e;
+    HIPC(hipMemcpyAsync(e->emb, emb, e->n * e->c * 4, hipMemcpyHostToDevice,
+                        e->s));
+    HIPC(hipStreamSynchronize(e->s));
+  });
+}
+
+const char *gm_last_error(void) { return gm::g_err.c_str(); }
+
+int gm_debug_pool(gm_handle *h, uint64_t *keys) {
+  return guarded([&] {
+    check_handle(h);
+    gm::Engine *e = h->e;
+    HIPC(hipMemcpyAsync(keys, e->pool, e->n * gm::W * 8, hipMemcpyDeviceToHost,
+                        e->s));
+    HIPC(hipStreamSynchronize(e->s));
+  });
+}
+
+int gm_debug_fuzzy(gm_handle *h, const uint32_t *batch, uint64_t nb,
+                   int32_t *head, int32_t *tail, float *w, double *sigma,
+                   double *rho, int64_t *nnz) {
+  return guarded([&] {
+    check_handle(h);
+    gm::Engine *e = h->e;
+    if (nb > e->n || nb == 0)
+      throw Fail(GM_ERR_INVALID, "batch size out of range");
+    for (uint64_t i = 0; i < nb; ++i)
+      if (batch[i] >= e->n)
+        throw Fail(GM_ERR_INVALID, "batch row out of range");
+    HIPC(hipMemcpyAsync(e->dbatch, batch, nb * 4, hipMemcpyHostToDevice, e->s));
+    e->build_fuzzy(nb);
+    uint32_t cnt = 0;
+    HIPC(hipMemcpyAsync(&cnt, e->scal + gm::S_NNZ, 4, hipMemcpyDeviceToHost,
+                        e->s));
+    HIPC(hipStreamSynchronize(e->s));
+    if (head)
+      HIPC(hipMemcpyAsync(head, e->head, cnt * 4ull, hipMemcpyDeviceToHost, e->s));
+    if (tail)
+      HIPC(hipMemcpyAsync(tail, e->tail, cnt * 4ull, hipMemcpyDeviceToHost, e->s));
+    if (w)
+      HIPC(hipMemcpyAsync(w, e->wout, cnt * 4ull, hipMemcpyDeviceToHost, e->s));
+    if (sigma)
+      HIPC(hipMemcpyAsync(sigma, e->sigma, e->n * 8, hipMemcpyDeviceToHost, e->s));
+    if (rho)
+      HIPC(hipMemcpyAsync(rho, e->rho, e->n * 8, hipMemcpyDeviceToHost, e->s));
+    HIPC(hipStreamSynchronize(e->s));
+    *nnz = cnt;
+  });
+}
+
+int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
+                 const int32_t *tail, const float *w, int64_t nnz,
+                 int32_t n_epochs, int32_t nsr, float a, float b, float alpha0,
+                 float gamma, uint64_t seed) {
+  gm_handle *h = nullptr;
+  int rc = guarded([&] {
+    if (n == 0 || c < 1 || c > 4 || nnz < 0 || nsr < 1)
+      throw Fail(GM_ERR_INVALID, "invalid SGD arguments");
+    for (int64_t i = 0; i < nnz; ++i)
+      if (head[i] < 0 || (uint64_t)head[i] >= n || tail[i] < 0 ||
+          (uint64_t)tail[i] >= n)
+        throw Fail(GM_ERR_INVALID, "COO index out of range");
+    gm_params q;
+    gm_default_params(&q);
+    q.n_instances = std::max<uint64_t>(n, (uint64_t)(nnz + 1) / 2 + 1);
+    q.n_features = 1;
+    q.n_neighbors = 2;
+    q.n_components = (uint64_t)c;
+    h = new gm_handle{new gm::Engine(q)};
+    gm::Engine *e = h->e;
+    e->n = n;
+    float mw = 0.0f;
+    for (int64_t i = 0; i < nnz; ++i)
+      mw = std::max(mw, w[i]);
+    uint32_t sc[2] = {(uint32_t)nnz, 0};
+    std::memcpy(&sc[1], &mw, 4);
+    HIPC(hipMemcpyAsync(e->emb, emb, n * c * 4, hipMemcpyHostToDevice, e->s));
+    HIPC(hipMemcpyAsync(e->head, head, nnz * 4, hipMemcpyHostToDevice, e->s));
+    HIPC(hipMemcpyAsync(e->tail, tail, nnz * 4, hipMemcpyHostToDevice, e->s));
+    HIPC(hipMemcpyAsync(e->wout, w, nnz * 4, hipMemcpyHostToDevice, e->s));
+    HIPC(hipMemcpyAsync(e->scal + gm::S_NNZ, &sc[0], 4, hipMemcpyHostToDevice,
+                        e->s));
+    HIPC(hipMemcpyAsync(e->scal + gm::S_MAXW, &sc[1], 4, hipMemcpyHostToDevice,
+                        e->s));
+    if (mw > 0.0f && nnz > 0)
+      e->run_sgd(e->emb, n, (uint64_t)nnz, n_epochs, nsr, a, b, alpha0, gamma,
+                 seed);
+    HIPC(hipMemcpyAsync(emb, e->emb, n * c * 4, hipMemcpyDeviceToHost, e->s));
+    HIPC(hipStreamSynchronize(e->s));
+  });
+  gm_destroy(h);
+  return rc;
+}
+
+int gm_debug_philox(const uint32_t ctr[4], const uint32_t key[2],
+                    uint32_t out[4]) {
+  uint32_t *d = nullptr;
+  int rc = guarded([&] {
+    HIPC(hipMalloc(&d, 16 * sizeof(uint32_t)));
+    uint32_t in[6] = {ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]};
+    HIPC(hipMemcpy(d, in, sizeof(in), hipMemcpyHostToDevice));
+    gm::k_philox_probe<<<1, 1>>>(d, d + 8);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpy(out, d + 8, 16, hipMemcpyDeviceToHost));
+  });
+  if (d)
+    (void)hipFree(d);
+  return rc;
+}
+
+} // extern "C"

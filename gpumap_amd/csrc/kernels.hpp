@@ -1,0 +1,99 @@
+// Kernel declarations shared by the engine and the kernel translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gm {
+
+// nnd_kernels.hip
+__global__ void k_prep_rows(float *X, uint64_t dp, uint64_t d, uint64_t b0,
+                            uint64_t nb);
+__global__ void k_random_ids(uint32_t *ids, int *cnt, uint64_t b0,
+                             uint64_t nb, uint64_t n, uint32_t window_start,
+                             uint32_t seed);
+__global__ void k_init_dist(const float *X, uint64_t dp, uint64_t *pool,
+                            const uint32_t *ids, const int *cnt, uint64_t b0,
+                            uint64_t nb);
+__global__ void k_ins_candidates(const float *X, uint64_t dp,
+                                 const uint64_t *pool, uint64_t b0,
+                                 uint64_t nb, int k, uint32_t *cand,
+                                 float *cdist, int *ccount);
+__global__ void k_ins_offers(uint64_t *pool, uint8_t *flag,
+                             const uint32_t *cand, const float *cdist,
+                             const int *ccount, uint64_t b0, uint64_t nb,
+                             int k);
+__global__ void k_sample(uint64_t *pool, const uint32_t *rows, uint64_t nrows,
+                         uint32_t *fn, uint32_t *fo, int *fnc, int *foc,
+                         uint32_t *newmask, uint32_t *owner_mark);
+__global__ void k_reverse(const uint64_t *pool, const uint32_t *rows,
+                          uint64_t nrows, const uint32_t *newmask,
+                          uint64_t *rev_new, uint64_t *rev_old,
+                          uint32_t *owner_mark, uint32_t *owners,
+                          uint32_t *owner_count);
+__global__ void k_join(const float *X, uint64_t dp, uint64_t *pool,
+                       const uint32_t *fn, const uint32_t *fo, const int *fnc,
+                       const int *foc, const uint64_t *rev_new,
+                       const uint64_t *rev_old, const uint32_t *owners,
+                       const uint32_t *owner_count, uint8_t *flag, int k);
+__global__ void k_join_reset(const uint32_t *owners,
+                             const uint32_t *owner_count, uint64_t *rev_new,
+                             uint64_t *rev_old, uint32_t *owner_mark, int *fnc,
+                             int *foc);
+__global__ void k_sync_rows(const uint64_t *pool, const uint32_t *rows,
+                            uint64_t nrows, uint32_t *pidx, float *pdist,
+                            int k, unsigned long long *fresh);
+__global__ void k_flag_count(const uint8_t *flag, uint64_t lo, uint64_t hi,
+                             uint32_t *bcount);
+__global__ void k_scan_small(uint32_t *v, uint64_t n, uint32_t *total);
+__global__ void k_flag_write(uint8_t *flag, uint64_t lo, uint64_t hi,
+                             const uint32_t *boff, uint32_t *out);
+__global__ void k_init_embedding(float *emb, int c, const uint32_t *pidx,
+                                 const float *pdist, int k, uint64_t b0,
+                                 uint64_t nb);
+
+// embed_kernels.hip
+__global__ void k_mean_partial(const uint32_t *pidx, const float *pdist,
+                               uint64_t n, int k, int euclid, double *psum,
+                               unsigned long long *pcnt);
+__global__ void k_mean_final(const double *psum, const unsigned long long *pcnt,
+                             int nparts, double *mean);
+__global__ void k_sigma(const uint32_t *pidx, const float *pdist, uint64_t n,
+                        int k, int euclid, const double *mean_p, double *sigma,
+                        double *rho);
+__global__ void k_membership(const uint32_t *pidx, const float *pdist,
+                             uint64_t n, int k, int euclid,
+                             const double *sigma, const double *rho,
+                             const uint32_t *batch, uint64_t nb, float *w,
+                             uint8_t *flags, uint32_t *maxw_bits);
+__global__ void k_coo_count(const float *w, const uint8_t *flags, uint64_t nd,
+                            const uint32_t *maxw_bits, int n_epochs,
+                            uint32_t *bcount);
+__global__ void k_coo_write(const uint32_t *pidx, int k, const uint32_t *batch,
+                            const float *w, const uint8_t *flags, uint64_t nd,
+                            const uint32_t *maxw_bits, int n_epochs,
+                            const uint32_t *boff, int32_t *head, int32_t *tail,
+                            float *wout);
+__global__ void k_sgd_init(const float *w, const int32_t *head,
+                           const uint32_t *nnz_p, const uint32_t *maxw_bits,
+                           int n_epochs, float nsr_inv, float *eps, float *nxt,
+                           float *nneg, int *deg);
+__global__ void k_max_int(const int *v, uint64_t n, int *out);
+__global__ void k_rounding(const int *max_deg, float alpha, float *rounding);
+__global__ void k_minmax_col0(const float *emb, uint64_t n, int c,
+                              uint32_t *mn, uint32_t *mx);
+__global__ void k_noise(float *emb, int c, const uint32_t *batch, uint64_t nb,
+                        const uint32_t *mn, const uint32_t *mx,
+                        float noise_scale, uint64_t seed);
+template <int NC>
+__global__ void k_sgd_epoch(float *emb, const int32_t *head,
+                            const int32_t *tail, const uint32_t *nnz_p,
+                            uint32_t n, const float *eps, float *nxt,
+                            float *nneg, float alpha, int epoch, float gamma,
+                            float a, float b, float nsr_inv, uint64_t seed,
+                            const float *rounding_p);
+__global__ void k_reposition(const uint32_t *pidx, const float *pdist,
+                             uint64_t n, int k, int euclid, const double *sigma,
+                             const double *rho, const uint32_t *rows,
+                             uint64_t nr, const float *snap, float *emb, int c);
+
+} // namespace gm

@@ -1,0 +1,164 @@
+/*
+ * gpumap_c.h -- C ABI of the MI355X (gfx950) progressive UMAP engine.
+ *
+ * This is the drop-in boundary: the reference's public C++ class
+ * gpumap::GPUMAP (cpp/include/gpumap/gpumap.cuh:60-96, PImpl over
+ * cpp/src/impl.cuh) and its pybind11 module _core (cpp/bindings/binding.cu:
+ * 45-116) become plain C entry points with POD structs, status codes instead
+ * of C++ exceptions, and device arrays exported through DLPack (kDLROCM).
+ *
+ * Conventions: every function returns GM_OK (0) or a negative gm_status; the
+ * message of the last failure on the calling thread is gm_last_error().  No
+ * C++ exception crosses this boundary.  One handle is not thread-safe; calls
+ * on one handle are serialised by the caller (the reference's contract,
+ * binding.cu:109-111).
+ */
+#ifndef GPUMAP_C_H
+#define GPUMAP_C_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gm_status {
+  GM_OK = 0,
+  GM_ERR_INVALID = -1,  /* bad argument: Python ValueError / RuntimeError */
+  GM_ERR_CAPACITY = -2, /* append beyond n_instances (impl.cuh:81-83) */
+  GM_ERR_OOM = -3,      /* device allocation failed: Python MemoryError */
+  GM_ERR_HIP = -4,      /* any other HIP runtime error */
+  GM_ERR_STATE = -5     /* call not valid in the handle's state */
+} gm_status;
+
+typedef enum gm_metric { GM_EUCLIDEAN = 0, GM_COSINE = 1 } gm_metric;
+
+/* Mirrors gpumap::GPUMAPParams (cpp/include/gpumap/gpumap.cuh:15-36) with
+ * fixed-width fields; defaults from gm_default_params(). */
+typedef struct gm_params {
+  uint64_t n_neighbors;          /* k, 2..32 (default 15) */
+  uint64_t n_components;         /* 1..4 (default 2) */
+  float min_dist;                /* default 0.1 */
+  uint64_t negative_sample_rate; /* default 5 */
+  uint64_t max_epoch;            /* default 200 */
+  float a, b;                    /* curve parameters (default 1, 1) */
+  float alpha;                   /* initial learning rate (default 1) */
+  int32_t verbose;
+  int32_t metric;                /* gm_metric */
+  uint64_t n_instances;          /* capacity N */
+  uint64_t n_features;           /* D */
+} gm_params;
+
+/* Mirrors gpumap::GPUMAPResult (cpp/include/gpumap/gpumap.cuh:38-56). */
+typedef struct gm_result {
+  uint32_t n_insertion_ops;
+  uint32_t n_update_ops;
+  uint32_t n_embedding_ops;
+  float insertion_time;
+  float update_time;
+  float embedding_time;
+  int32_t insertion_completed;
+  int32_t update_completed;
+  int32_t embedding_completed;
+  uint64_t n_inserted_instances;
+  uint64_t n_update_changed_edges;
+  uint64_t embedding_pending_edges;
+  uint64_t insertion_queue_size;
+  uint64_t update_queue_size;
+  uint64_t embedding_queue_size;
+} gm_result;
+
+typedef enum gm_array {
+  GM_EMBEDDING = 0,        /* float32 (n_inserted, n_components) */
+  GM_NEIGHBOR_INDICES = 1, /* uint32  (n_inserted, n_neighbors), ascending */
+  GM_NEIGHBOR_DISTS = 2    /* float32 (n_inserted, n_neighbors), squared L2
+                              (Euclidean) or 2 - 2 cos (cosine) */
+} gm_array;
+
+/* A non-owning description of a device array (row-major). */
+typedef struct gm_view {
+  void *data;
+  uint64_t rows, cols, row_stride; /* strides in elements */
+  int32_t dtype_code;              /* DLPack code: 1 uint, 2 float */
+  int32_t dtype_bits;
+  int32_t device_id;
+} gm_view;
+
+typedef struct gm_handle gm_handle;
+
+void gm_default_params(gm_params *out);
+
+/* GPUMAP(const GPUMAPParams&) -- gpumap.cuh:73, impl.cu:5, impl.cuh:57-62.
+ * Allocates every device array for n_instances rows up front. */
+int gm_create(const gm_params *params, gm_handle **out);
+
+/* GPUMAP::initialize -- gpumap.cuh:83 (a no-op in the reference,
+ * impl.cu:12-15); here it checks that the sizes match the params. */
+int gm_initialize(gm_handle *h, uint64_t n_instances, uint64_t n_features);
+
+/* GPUMAP::append_data -- gpumap.cuh:84, impl.cuh:77-101,
+ * gpumap_wrapper.cuh:29-40.  Copies `rows` x `cols` row-major float32 host
+ * rows before returning; 0 rows is a no-op; cols must equal n_features. */
+int gm_append_data(gm_handle *h, const float *host_rows, uint64_t rows,
+                   uint64_t cols);
+
+/* GPUMAP::run -- gpumap.cuh:85-86, impl.cuh:146-218: insertion, update and
+ * embedding with the given op counts, timed each, blocking. */
+int gm_run(gm_handle *h, uint32_t n_insertion_ops, uint32_t n_update_ops,
+           uint32_t n_embedding_ops, gm_result *out);
+
+/* GPUMAP::close -- gpumap.cuh:87, impl.cuh:502-507: synchronises the device.
+ * Arrays stay readable until gm_destroy. */
+int gm_close(gm_handle *h);
+
+/* ~GPUMAP.  Device memory is released when the last exported DLPack tensor
+ * of the handle is deleted. */
+void gm_destroy(gm_handle *h);
+
+/* get_embedding / get_neighbor_indices / get_neighbor_dists --
+ * gpumap.cuh:89-91, impl.cuh:509-533.  The reference returns the full
+ * capacity; these describe the n_inserted rows (the slicing the Python facade
+ * does at gpumap.py:148-162). */
+int gm_get_view(gm_handle *h, int32_t which, gm_view *out);
+
+/* Same array as a DLManagedTensor* (DLPack, device type kDLROCM = 10) whose
+ * deleter keeps the handle's memory alive (cai_wrapper.cuh:39-62 exports a
+ * non-owning __cuda_array_interface__ instead). */
+int gm_export_dlpack(gm_handle *h, int32_t which, void **dl_managed_tensor);
+
+/* Host copies of the public arrays (n_inserted rows). */
+int gm_copy_knn(gm_handle *h, uint32_t *idx, float *dist);
+int gm_copy_embedding(gm_handle *h, float *emb);
+int gm_set_embedding(gm_handle *h, const float *emb);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *gm_last_error(void);
+
+/* ---- stage entry points (no counterpart in the reference; they expose the
+ * engine's stages on caller-given inputs for parity tests and profiling) */
+
+/* Raw 32-slot neighbour pools of the inserted rows, packed keys. */
+int gm_debug_pool(gm_handle *h, uint64_t *keys);
+
+/* Fuzzy set + trim for the given batch rows on the handle's current graph
+ * (what one embed call builds before the SGD).  head/tail/w need room for
+ * 2 * n_batch * n_neighbors entries; sigma/rho (may be NULL) n_inserted. */
+int gm_debug_fuzzy(gm_handle *h, const uint32_t *batch, uint64_t n_batch,
+                   int32_t *head, int32_t *tail, float *w, double *sigma,
+                   double *rho, int64_t *nnz);
+
+/* SGD epochs of the engine on a caller-given embedding (n x c, host) and COO
+ * (host), with the engine's kernels. */
+int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
+                 const int32_t *tail, const float *w, int64_t nnz,
+                 int32_t n_epochs, int32_t nsr, float a, float b, float alpha0,
+                 float gamma, uint64_t seed);
+
+/* Philox4x32-10 of the device code, for known-answer tests. */
+int gm_debug_philox(const uint32_t ctr[4], const uint32_t key[2],
+                    uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

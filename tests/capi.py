@@ -1,0 +1,180 @@
+"""ctypes view of include/gpumap_c.h for the tests: the GPU parity tests call
+the engine through its C ABI, the same boundary a reference-side binding
+would use."""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "gpumap_amd", "libgpumap_amd.so")
+HEADER = os.path.join(ROOT, "include", "gpumap_c.h")
+
+u32, u64, i32, i64, f32 = (ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32,
+                           ctypes.c_int64, ctypes.c_float)
+P = ctypes.POINTER
+
+
+class GmParams(ctypes.Structure):
+    _fields_ = [("n_neighbors", u64), ("n_components", u64), ("min_dist", f32),
+                ("negative_sample_rate", u64), ("max_epoch", u64), ("a", f32),
+                ("b", f32), ("alpha", f32), ("verbose", i32), ("metric", i32),
+                ("n_instances", u64), ("n_features", u64)]
+
+
+class GmResult(ctypes.Structure):
+    _fields_ = [("n_insertion_ops", u32), ("n_update_ops", u32),
+                ("n_embedding_ops", u32), ("insertion_time", f32),
+                ("update_time", f32), ("embedding_time", f32),
+                ("insertion_completed", i32), ("update_completed", i32),
+                ("embedding_completed", i32), ("n_inserted_instances", u64),
+                ("n_update_changed_edges", u64), ("embedding_pending_edges", u64),
+                ("insertion_queue_size", u64), ("update_queue_size", u64),
+                ("embedding_queue_size", u64)]
+
+
+class GmView(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("rows", u64), ("cols", u64),
+                ("row_stride", u64), ("dtype_code", i32), ("dtype_bits", i32),
+                ("device_id", i32)]
+
+
+def header_functions() -> list[str]:
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gm_[a-z_]+)\s*\(", text)))
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.gm_default_params.argtypes = [P(GmParams)]
+        L.gm_create.argtypes = [P(GmParams), P(vp)]
+        L.gm_initialize.argtypes = [vp, u64, u64]
+        L.gm_append_data.argtypes = [vp, P(f32), u64, u64]
+        L.gm_run.argtypes = [vp, u32, u32, u32, P(GmResult)]
+        L.gm_close.argtypes = [vp]
+        L.gm_destroy.argtypes = [vp]
+        L.gm_destroy.restype = None
+        L.gm_get_view.argtypes = [vp, i32, P(GmView)]
+        L.gm_export_dlpack.argtypes = [vp, i32, P(vp)]
+        L.gm_copy_knn.argtypes = [vp, P(u32), P(f32)]
+        L.gm_copy_embedding.argtypes = [vp, P(f32)]
+        L.gm_set_embedding.argtypes = [vp, P(f32)]
+        L.gm_last_error.restype = ctypes.c_char_p
+        L.gm_debug_pool.argtypes = [vp, P(u64)]
+        L.gm_debug_fuzzy.argtypes = [vp, P(u32), u64, P(i32), P(i32), P(f32),
+                                     P(ctypes.c_double), P(ctypes.c_double),
+                                     P(i64)]
+        L.gm_debug_sgd.argtypes = [P(f32), u64, i32, P(i32), P(i32), P(f32),
+                                   i64, i32, i32, f32, f32, f32, f32, u64]
+        L.gm_debug_philox.argtypes = [P(u32), P(u32), P(u32)]
+        _lib = L
+    return _lib
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(P(ct))
+
+
+def check(rc):
+    if rc != 0:
+        raise RuntimeError(f"gm status {rc}: {lib().gm_last_error().decode()}")
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    check(lib().gm_debug_philox(_p(c, u32), _p(k, u32), _p(out, u32)))
+    return out
+
+
+def sgd(emb, head, tail, w, n_epochs, nsr, a, b, alpha0=1.0, gamma=1.0, seed=42):
+    emb = np.ascontiguousarray(emb, np.float32).copy()
+    head = np.ascontiguousarray(head, np.int32)
+    tail = np.ascontiguousarray(tail, np.int32)
+    w = np.ascontiguousarray(w, np.float32)
+    check(lib().gm_debug_sgd(_p(emb, f32), emb.shape[0], emb.shape[1],
+                             _p(head, i32), _p(tail, i32), _p(w, f32), len(w),
+                             n_epochs, nsr, a, b, alpha0, gamma, seed))
+    return emb
+
+
+class Engine:
+    def __init__(self, n_instances, n_features, n_neighbors=15, n_components=2,
+                 metric=0, a=1.577, b=0.895, alpha=1.0, max_epoch=200,
+                 negative_sample_rate=5):
+        p = GmParams()
+        lib().gm_default_params(ctypes.byref(p))
+        p.n_instances, p.n_features = n_instances, n_features
+        p.n_neighbors, p.n_components, p.metric = n_neighbors, n_components, metric
+        p.a, p.b, p.alpha, p.max_epoch = a, b, alpha, max_epoch
+        p.negative_sample_rate = negative_sample_rate
+        h = ctypes.c_void_p()
+        check(lib().gm_create(ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        self.k, self.c, self.n = n_neighbors, n_components, 0
+
+    def close(self):
+        if self.h:
+            lib().gm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def append(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        check(lib().gm_append_data(self.h, _p(x, f32), x.shape[0], x.shape[1]))
+
+    def run(self, i, u, e):
+        r = GmResult()
+        check(lib().gm_run(self.h, i, u, e, ctypes.byref(r)))
+        self.n = r.n_inserted_instances
+        return r
+
+    def knn(self):
+        idx = np.zeros((self.n, self.k), np.uint32)
+        dist = np.zeros((self.n, self.k), np.float32)
+        check(lib().gm_copy_knn(self.h, _p(idx, u32), _p(dist, f32)))
+        return idx, dist
+
+    def embedding(self):
+        e = np.zeros((self.n, self.c), np.float32)
+        check(lib().gm_copy_embedding(self.h, _p(e, f32)))
+        return e
+
+    def set_embedding(self, e):
+        e = np.ascontiguousarray(e, np.float32)
+        check(lib().gm_set_embedding(self.h, _p(e, f32)))
+
+    def pool(self):
+        p = np.zeros((self.n, 32), np.uint64)
+        check(lib().gm_debug_pool(self.h, _p(p, u64)))
+        return p
+
+    def fuzzy(self, batch):
+        batch = np.ascontiguousarray(batch, np.uint32)
+        cap = 2 * len(batch) * self.k + 1
+        head = np.zeros(cap, np.int32)
+        tail = np.zeros(cap, np.int32)
+        w = np.zeros(cap, np.float32)
+        sig = np.zeros(self.n, np.float64)
+        rho = np.zeros(self.n, np.float64)
+        nnz = i64(0)
+        check(lib().gm_debug_fuzzy(self.h, _p(batch, u32), len(batch),
+                                   _p(head, i32), _p(tail, i32), _p(w, f32),
+                                   _p(sig, ctypes.c_double),
+                                   _p(rho, ctypes.c_double), ctypes.byref(nnz)))
+        m = nnz.value
+        return head[:m], tail[:m], w[:m], sig, rho
