@@ -1,6 +1,9 @@
-"""ctypes view of include/gpumap_c.h for the tests: the GPU parity tests call
-the engine through its C ABI, the same boundary a reference-side binding
-would use."""
+"""ctypes view of the C ABI (include/gpumap_c.h).
+
+The binding a non-pybind11 caller would write (see the reference-side stub in
+the integration notes); the GPU parity tests and the bench drive the engine
+through it.  ``Engine`` owns one session handle.
+"""
 
 from __future__ import annotations
 
@@ -12,6 +15,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "gpumap_amd", "libgpumap_amd.so")
+if not os.path.exists(LIB_PATH):
+    raise ImportError("libgpumap_amd.so is not built; run "
+                      "`python -m gpumap_amd.build`")
 HEADER = os.path.join(ROOT, "include", "gpumap_c.h")
 
 u32, u64, i32, i64, f32 = (ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32,
@@ -23,7 +29,7 @@ class GmParams(ctypes.Structure):
     _fields_ = [("n_neighbors", u64), ("n_components", u64), ("min_dist", f32),
                 ("negative_sample_rate", u64), ("max_epoch", u64), ("a", f32),
                 ("b", f32), ("alpha", f32), ("verbose", i32), ("metric", i32),
-                ("n_instances", u64), ("n_features", u64)]
+                ("n_instances", u64), ("n_features", u64), ("knn_mode", i32)]
 
 
 class GmResult(ctypes.Structure):
@@ -41,6 +47,15 @@ class GmView(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("rows", u64), ("cols", u64),
                 ("row_stride", u64), ("dtype_code", i32), ("dtype_bits", i32),
                 ("device_id", i32)]
+
+
+class GmKernelStats(ctypes.Structure):
+    _fields_ = [("launches", u64), ("device_ms", ctypes.c_double),
+                ("work", u64 * 4)]
+
+
+KSTAT_JOIN, KSTAT_SGD, KSTAT_INSERT = 0, 1, 2
+KNN_NNDESCENT, KNN_EXACT_FP32, KNN_EXACT_BF16 = 0, 1, 2
 
 
 def header_functions() -> list[str]:
@@ -78,6 +93,10 @@ def lib():
         L.gm_debug_sgd.argtypes = [P(f32), u64, i32, P(i32), P(i32), P(f32),
                                    i64, i32, i32, f32, f32, f32, f32, u64]
         L.gm_debug_philox.argtypes = [P(u32), P(u32), P(u32)]
+        L.gm_append_data_device.argtypes = [vp, vp, u64, u64]
+        L.gm_set_profiling.argtypes = [vp, i32]
+        L.gm_get_kernel_stats.argtypes = [vp, i32, P(GmKernelStats)]
+        L.gm_reset_kernel_stats.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -113,13 +132,15 @@ def sgd(emb, head, tail, w, n_epochs, nsr, a, b, alpha0=1.0, gamma=1.0, seed=42)
 class Engine:
     def __init__(self, n_instances, n_features, n_neighbors=15, n_components=2,
                  metric=0, a=1.577, b=0.895, alpha=1.0, max_epoch=200,
-                 negative_sample_rate=5):
+                 negative_sample_rate=5, knn_mode=None):
         p = GmParams()
         lib().gm_default_params(ctypes.byref(p))
         p.n_instances, p.n_features = n_instances, n_features
         p.n_neighbors, p.n_components, p.metric = n_neighbors, n_components, metric
         p.a, p.b, p.alpha, p.max_epoch = a, b, alpha, max_epoch
         p.negative_sample_rate = negative_sample_rate
+        if knn_mode is not None:
+            p.knn_mode = knn_mode
         h = ctypes.c_void_p()
         check(lib().gm_create(ctypes.byref(p), ctypes.byref(h)))
         self.h = h
@@ -136,6 +157,22 @@ class Engine:
     def append(self, x):
         x = np.ascontiguousarray(x, np.float32)
         check(lib().gm_append_data(self.h, _p(x, f32), x.shape[0], x.shape[1]))
+
+    def append_device(self, ptr: int, rows: int, cols: int):
+        """Append rows already in device memory (a raw device pointer)."""
+        check(lib().gm_append_data_device(self.h, ctypes.c_void_p(ptr), rows,
+                                          cols))
+
+    def set_profiling(self, on: bool = True):
+        check(lib().gm_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_stats(self, which: int) -> GmKernelStats:
+        st = GmKernelStats()
+        check(lib().gm_get_kernel_stats(self.h, which, ctypes.byref(st)))
+        return st
+
+    def reset_kernel_stats(self):
+        check(lib().gm_reset_kernel_stats(self.h))
 
     def run(self, i, u, e):
         r = GmResult()

@@ -92,6 +92,41 @@ struct Rng {
   }
 };
 
+// IEEE round-to-nearest square root and quotient of floats.  The hardware
+// estimate is within one ulp; the candidate is corrected with exact double
+// arithmetic (products of 24/25-bit significands fit in 53 bits, and a float
+// square root or quotient never lands exactly on a rounding midpoint).
+__device__ inline float sqrt_rn(float x) {
+  if (!(x > 0.0f) || isinf(x))
+    return sqrtf(x);
+  float y = sqrtf(x);
+  const double xd = (double)x;
+  const float up = nextafterf(y, INFINITY), dn = nextafterf(y, 0.0f);
+  const double mu = 0.5 * ((double)y + (double)up);
+  const double md = 0.5 * ((double)y + (double)dn);
+  if (xd >= mu * mu)
+    y = up;
+  else if (xd < md * md)
+    y = dn;
+  return y;
+}
+__device__ inline float div_rn(float x, float d) {
+  float q = x / d;
+  if (!isfinite(q) || q == 0.0f || !isfinite(d) || d == 0.0f)
+    return q;
+  const double xd = (double)x, dd = (double)d;
+  const float up = nextafterf(q, INFINITY), dn = nextafterf(q, -INFINITY);
+  // pick the neighbour whose product is nearest x (products are exact)
+  const double e0 = fabs(xd - (double)q * dd);
+  const double eu = fabs(xd - (double)up * dd);
+  const double ed = fabs(xd - (double)dn * dd);
+  if (eu < e0 && eu <= ed)
+    return up;
+  if (ed < e0)
+    return dn;
+  return q;
+}
+
 // Canonical squared distance: one FMA chain over the features in index
 // order.  Every kernel that produces a distance destined for a pool uses this
 // exact operation order, so a pair's distance is bit-identical wherever it is

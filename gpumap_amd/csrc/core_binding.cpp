@@ -67,6 +67,38 @@ int32_t metric_from_object(const py::object &v) {
   return m;
 }
 
+std::string knn_mode_to_string(int32_t m) {
+  switch (m) {
+  case GM_KNN_NNDESCENT:
+    return "nndescent";
+  case GM_KNN_EXACT_BF16:
+    return "exact_bf16";
+  default:
+    return "exact_fp32";
+  }
+}
+
+int32_t knn_mode_from_object(const py::object &v) {
+  if (py::isinstance<py::str>(v)) {
+    std::string s = v.cast<std::string>();
+    for (char &ch : s)
+      ch = (char)std::tolower((unsigned char)ch);
+    if (s == "nndescent" || s == "nn_descent")
+      return GM_KNN_NNDESCENT;
+    if (s == "exact" || s == "exact_fp32")
+      return GM_KNN_EXACT_FP32;
+    if (s == "exact_bf16")
+      return GM_KNN_EXACT_BF16;
+    throw py::value_error("Unknown knn_mode '" + s +
+                          "' -- expected 'nndescent', 'exact_fp32' or "
+                          "'exact_bf16'");
+  }
+  const int m = v.cast<int>();
+  if (m < GM_KNN_NNDESCENT || m > GM_KNN_EXACT_BF16)
+    throw py::value_error("Unknown knn_mode " + std::to_string(m));
+  return m;
+}
+
 struct Session {
   gm_handle *h = nullptr;
   explicit Session(const Params &p) { check(gm_create(&p.p, &h)); }
@@ -201,6 +233,12 @@ PYBIND11_MODULE(_core, m) {
           [](const Params &p) { return metric_to_string(p.p.metric); },
           [](Params &p, const py::object &v) {
             p.p.metric = metric_from_object(v);
+          })
+      .def_property(
+          "knn_mode",
+          [](const Params &p) { return knn_mode_to_string(p.p.knn_mode); },
+          [](Params &p, const py::object &v) {
+            p.p.knn_mode = knn_mode_from_object(v);
           })
       .def_property(
           "n_instances", [](const Params &p) { return p.p.n_instances; },

@@ -32,7 +32,7 @@ __device__ inline void embed_entry(const uint32_t *pidx, const float *pdist,
   if (!valid_d((double)v))
     *d = (double)v;
   else
-    *d = euclid ? (double)__fsqrt_rn(fmaxf(v, 0.0f)) : (double)v;
+    *d = euclid ? (double)sqrt_rn(fmaxf(v, 0.0f)) : (double)v;
 }
 
 __device__ inline double membership(double d, double rho, double sigma) {
@@ -231,7 +231,7 @@ __device__ inline bool coo_keep(const float *w, const uint8_t *flags,
 
 __device__ inline float coo_thr(const uint32_t *maxw_bits, int n_epochs) {
   const float mw = __uint_as_float(*maxw_bits);
-  return mw > 0.0f ? __fdiv_rn(mw, (float)n_epochs) : 3.402823466e38f;
+  return mw > 0.0f ? div_rn(mw, (float)n_epochs) : 3.402823466e38f;
 }
 
 __global__ void k_coo_count(const float *__restrict__ w,
@@ -302,8 +302,8 @@ __global__ void k_sgd_init(const float *__restrict__ w,
   if (e >= *nnz_p)
     return;
   const float mw = __uint_as_float(*maxw_bits);
-  const float ns = (float)n_epochs * __fdiv_rn(w[e], mw);
-  const float ep = ns > 0.0f ? __fdiv_rn((float)n_epochs, ns) : -1.0f;
+  const float ns = (float)n_epochs * div_rn(w[e], mw);
+  const float ep = ns > 0.0f ? div_rn((float)n_epochs, ns) : -1.0f;
   eps[e] = ep;
   nxt[e] = ep;
   nneg[e] = ep * nsr_inv;
@@ -331,7 +331,7 @@ __global__ void k_rounding(const int *__restrict__ max_deg, float alpha,
   }
   const float max_abs = (float)md * 4.0f * fabsf(alpha);
   const float delta =
-      __fdiv_rn(max_abs, 1.0f - 2.0f * (float)md * 1.1920928955078125e-7f);
+      div_rn(max_abs, 1.0f - 2.0f * (float)md * 1.1920928955078125e-7f);
   int e;
   frexpf(delta, &e);
   *rounding = ldexpf(1.0f, e);
@@ -388,9 +388,11 @@ __global__ __launch_bounds__(256) void k_sgd_epoch(
     const int32_t *__restrict__ tail, const uint32_t *__restrict__ nnz_p,
     uint32_t n, const float *__restrict__ eps, float *__restrict__ nxt,
     float *__restrict__ nneg, float alpha, int epoch, float gamma, float a,
-    float b, float nsr_inv, uint64_t seed, const float *__restrict__ rounding_p) {
+    float b, float nsr_inv, uint64_t seed, const float *__restrict__ rounding_p,
+    unsigned long long *__restrict__ stats) {
   const uint64_t nnz = *nnz_p;
   const float rounding = *rounding_p;
+  unsigned long long c_pos = 0, c_neg = 0;
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
        e += (uint64_t)gridDim.x * blockDim.x) {
     const float ns = nxt[e];
@@ -399,6 +401,7 @@ __global__ __launch_bounds__(256) void k_sgd_epoch(
     const float ep = eps[e];
     const float ep_neg = ep * nsr_inv;
     const int32_t j = head[e], kk = tail[e];
+    ++c_pos;
     float *cur_p = emb + (uint64_t)j * NC;
     float *oth_p = emb + (uint64_t)kk * NC;
     float cur[NC], oth[NC], g[NC];
@@ -433,6 +436,7 @@ __global__ __launch_bounds__(256) void k_sgd_epoch(
         const uint32_t r = rng.next() % n;
         if ((int32_t)r == j)
           continue;
+        ++c_neg;
         const float *neg_p = emb + (uint64_t)r * NC;
         float neg[NC];
         float d2n = 0.0f;
@@ -459,28 +463,42 @@ __global__ __launch_bounds__(256) void k_sgd_epoch(
     for (int d = 0; d < NC; ++d)
       unsafeAtomicAdd(cur_p + d, (rounding + g[d]) - rounding);
   }
+  if (stats) { // one atomic per wave and counter
+    for (int off = 32; off > 0; off >>= 1) {
+      c_pos += __shfl_xor(c_pos, off);
+      c_neg += __shfl_xor(c_neg, off);
+    }
+    if ((threadIdx.x & 63) == 0 && (c_pos | c_neg)) {
+      atomicAdd(stats + 3, c_pos);
+      atomicAdd(stats + 4, c_neg);
+    }
+  }
 }
 
 template __global__ void k_sgd_epoch<2>(float *, const int32_t *,
                                         const int32_t *, const uint32_t *,
                                         uint32_t, const float *, float *,
                                         float *, float, int, float, float,
-                                        float, float, uint64_t, const float *);
+                                        float, float, uint64_t, const float *,
+                                        unsigned long long *);
 template __global__ void k_sgd_epoch<3>(float *, const int32_t *,
                                         const int32_t *, const uint32_t *,
                                         uint32_t, const float *, float *,
                                         float *, float, int, float, float,
-                                        float, float, uint64_t, const float *);
+                                        float, float, uint64_t, const float *,
+                                        unsigned long long *);
 template __global__ void k_sgd_epoch<1>(float *, const int32_t *,
                                         const int32_t *, const uint32_t *,
                                         uint32_t, const float *, float *,
                                         float *, float, int, float, float,
-                                        float, float, uint64_t, const float *);
+                                        float, float, uint64_t, const float *,
+                                        unsigned long long *);
 template __global__ void k_sgd_epoch<4>(float *, const int32_t *,
                                         const int32_t *, const uint32_t *,
                                         uint32_t, const float *, float *,
                                         float *, float, int, float, float,
-                                        float, float, uint64_t, const float *);
+                                        float, float, uint64_t, const float *,
+                                        unsigned long long *);
 
 // ----------------------------------------------------------- reposition
 // Unselected rows move to the membership-weighted mean of their embed-time

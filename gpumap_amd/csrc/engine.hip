@@ -125,9 +125,23 @@ struct Engine {
   float *emb = nullptr;
   uint8_t *flag = nullptr;
   uint32_t *list = nullptr;
+  float *pmax = nullptr;      // per-row pool maximum distance (inf if not full)
+  uint64_t *part = nullptr;   // exact insertion: per (row, split) top W
+  uint64_t part_cap = 0;
   uint32_t *bcount = nullptr;
   uint32_t *scal = nullptr;
   unsigned long long *fresh = nullptr;
+  unsigned long long *kstats = nullptr; // device work counters (8)
+  // profiling: HIP events around the timed kernels on this stream
+  bool prof = false;
+  struct Timed {
+    int which;
+    hipEvent_t a, b;
+    uint64_t launches;
+  };
+  std::vector<Timed> pending;
+  gm_kernel_stats totals[GM_KSTAT_COUNT] = {};
+  uint64_t sgd_edge_epochs = 0;
   // insertion scratch
   uint64_t ins_cap = 0;
   uint32_t *ins_ids = nullptr, *cand = nullptr;
@@ -198,9 +212,13 @@ struct Engine {
     flag = dalloc<uint8_t>(N);
     HIPC(hipMemsetAsync(flag, 0, N, s));
     list = dalloc<uint32_t>(N);
+    pmax = dalloc<float>(N);
+    HIPC(hipMemsetD32Async((hipDeviceptr_t)pmax, 0x7f800000, N, s));
     bcount = dalloc<uint32_t>(std::max<uint64_t>(2 * nk, N) / 1024 + 2);
     scal = dalloc<uint32_t>(S_COUNT);
     fresh = dalloc<unsigned long long>(1);
+    kstats = dalloc<unsigned long long>(8);
+    HIPC(hipMemsetAsync(kstats, 0, 8 * sizeof(unsigned long long), s));
     fn = dalloc<uint32_t>(N * W);
     fo = dalloc<uint32_t>(N * W);
     newmask = dalloc<uint32_t>(N);
@@ -273,6 +291,20 @@ struct Engine {
   void release() {
     if (s)
       (void)hipStreamSynchronize(s);
+    for (auto &t : pending) {
+      (void)hipEventDestroy(t.a);
+      (void)hipEventDestroy(t.b);
+    }
+    pending.clear();
+    if (kstats)
+      (void)hipFree(kstats);
+    kstats = nullptr;
+    if (pmax)
+      (void)hipFree(pmax);
+    if (part)
+      (void)hipFree(part);
+    pmax = nullptr;
+    part = nullptr;
     void *ptrs[] = {X,     pool,    pidx,    pdist,   emb,        flag,
                     list,  bcount,  scal,    fresh,   ins_ids,    cand,
                     ins_cnt, ccount, cdist,  fn,      fo,         newmask,
@@ -298,6 +330,47 @@ struct Engine {
     s = nullptr;
   }
 
+  hipEvent_t ev_start(int which) {
+    if (!prof)
+      return nullptr;
+    Timed t{which, nullptr, nullptr, 0};
+    HIPC(hipEventCreate(&t.a));
+    HIPC(hipEventCreate(&t.b));
+    HIPC(hipEventRecord(t.a, s));
+    pending.push_back(t);
+    return t.a;
+  }
+  void ev_stop(uint64_t launches) {
+    if (!prof)
+      return;
+    Timed &t = pending.back();
+    t.launches = launches;
+    HIPC(hipEventRecord(t.b, s));
+  }
+  // fold finished events and the device counters into the totals
+  void collect_stats() {
+    if (!prof)
+      return;
+    HIPC(hipStreamSynchronize(s));
+    for (auto &t : pending) {
+      float ms = 0.0f;
+      HIPC(hipEventElapsedTime(&ms, t.a, t.b));
+      totals[t.which].launches += t.launches;
+      totals[t.which].device_ms += ms;
+      (void)hipEventDestroy(t.a);
+      (void)hipEventDestroy(t.b);
+    }
+    pending.clear();
+    unsigned long long c[8];
+    HIPC(hipMemcpy(c, kstats, sizeof(c), hipMemcpyDeviceToHost));
+    totals[GM_KSTAT_JOIN].work[0] = c[0];
+    totals[GM_KSTAT_JOIN].work[1] = c[1];
+    totals[GM_KSTAT_JOIN].work[2] = c[2];
+    totals[GM_KSTAT_SGD].work[0] = c[3];
+    totals[GM_KSTAT_SGD].work[1] = c[4];
+    totals[GM_KSTAT_SGD].work[2] = sgd_edge_epochs;
+  }
+
   void log(const char *msg) const {
     if (p.verbose) {
       fprintf(stderr, "[gpumap_amd] %s\n", msg);
@@ -316,6 +389,21 @@ struct Engine {
     HIPC(hipMemcpy2DAsync(X + n * Dp, Dp * sizeof(float), h_stage,
                           D * sizeof(float), D * sizeof(float), nr,
                           hipMemcpyHostToDevice, s));
+    HIPC(hipStreamSynchronize(s));
+    for (uint64_t r = 0; r < nr; ++r) {
+      insertion_queue.q.push_back((uint32_t)(n + r));
+      embedding_new_queue.q.push_back((uint32_t)(n + r));
+    }
+    n += nr;
+  }
+
+  void append_device(const float *x, uint64_t nr) {
+    if (nr == 0)
+      return;
+    if (n + nr > N)
+      throw Fail(GM_ERR_CAPACITY, "Exceeding maximum number of rows in storage");
+    HIPC(hipMemcpy2DAsync(X + n * Dp, Dp * sizeof(float), x, D * sizeof(float),
+                          D * sizeof(float), nr, hipMemcpyDeviceToDevice, s));
     HIPC(hipStreamSynchronize(s));
     for (uint64_t r = 0; r < nr; ++r) {
       insertion_queue.q.push_back((uint32_t)(n + r));
@@ -358,7 +446,7 @@ struct Engine {
                         hipMemcpyHostToDevice, s));
     HIPC(hipMemsetAsync(fresh, 0, sizeof(unsigned long long), s));
     k_sync_rows<<<blocks_for(rows_.size() * 32, 256), 256, 0, s>>>(
-        pool, rows, rows_.size(), pidx, pdist, k, fresh);
+        pool, rows, rows_.size(), pidx, pdist, k, fresh, pmax);
     LAUNCH_CHECK();
     unsigned long long f = 0;
     HIPC(hipMemcpyAsync(h_scal, fresh, sizeof(f), hipMemcpyDeviceToHost, s));
@@ -380,25 +468,14 @@ struct Engine {
     const uint32_t window_start = batch_start_history.front();
     const uint32_t seed = n_ops * 9973u + (uint32_t)n * 7919u;
     const uint64_t b0 = batch.front(), B = batch.size();
-    ensure_ins(B);
     if (!euclid) {
       k_prep_rows<<<(unsigned)B, 64, 0, s>>>(X, Dp, D, b0, B);
       LAUNCH_CHECK();
     }
-    k_random_ids<<<blocks_for(B, 64), 64, 0, s>>>(ins_ids, ins_cnt, b0, B, n,
-                                                  window_start, seed);
-    LAUNCH_CHECK();
-    k_init_dist<<<blocks_for(B * W, 256), 256, 0, s>>>(X, Dp, pool, ins_ids,
-                                                       ins_cnt, b0, B);
-    LAUNCH_CHECK();
-    for (int it = 0; it < 10; ++it) {
-      k_ins_candidates<<<(unsigned)B, 256, 0, s>>>(X, Dp, pool, b0, B, k, cand,
-                                                   cdist, ccount);
-      LAUNCH_CHECK();
-      k_ins_offers<<<(unsigned)B, 256, 0, s>>>(pool, flag, cand, cdist, ccount,
-                                               b0, B, k);
-      LAUNCH_CHECK();
-    }
+    if (p.knn_mode == GM_KNN_NNDESCENT)
+      insert_nndescent(b0, B, window_start, seed);
+    else
+      insert_exact(b0, B);
     std::vector<uint32_t> flagged;
     collect_flags(n, flagged);
     std::vector<uint32_t> affected(batch.begin(), batch.end());
@@ -414,10 +491,61 @@ struct Engine {
     return (uint32_t)B;
   }
 
+  // NN-Descent insertion (the reference's method)
+  void insert_nndescent(uint64_t b0, uint64_t B, uint32_t window_start,
+                        uint32_t seed) {
+    ensure_ins(B);
+    k_random_ids<<<blocks_for(B, 64), 64, 0, s>>>(ins_ids, ins_cnt, b0, B, n,
+                                                  window_start, seed);
+    LAUNCH_CHECK();
+    k_init_dist<<<blocks_for(B * W, 256), 256, 0, s>>>(X, Dp, pool, ins_ids,
+                                                       ins_cnt, b0, B);
+    LAUNCH_CHECK();
+    ev_start(GM_KSTAT_INSERT);
+    for (int it = 0; it < 10; ++it) {
+      k_ins_candidates<<<(unsigned)B, 256, 0, s>>>(X, Dp, pool, b0, B, k, cand,
+                                                   cdist, ccount);
+      LAUNCH_CHECK();
+      k_ins_offers<<<(unsigned)B, 256, 0, s>>>(pool, flag, cand, cdist, ccount,
+                                               b0, B, k);
+      LAUNCH_CHECK();
+    }
+    ev_stop(20);
+  }
+
+  // exact insertion by distance tiles (exact_kernels.hip)
+  void insert_exact(uint64_t b0, uint64_t B) {
+    const uint64_t row_tiles = (B + 63) / 64;
+    const uint64_t col_tiles = (n + 63) / 64;
+    int nsplit = 1;
+    while (nsplit < 128 && row_tiles * (uint64_t)nsplit < 1024 &&
+           (uint64_t)nsplit * 2 <= col_tiles)
+      nsplit *= 2;
+    const uint64_t need = B * (uint64_t)nsplit * W;
+    if (need > part_cap) {
+      if (part)
+        (void)hipFree(part);
+      part = nullptr;
+      part_cap = 0;
+      part = dalloc<uint64_t>(need);
+      part_cap = need;
+    }
+    ev_start(GM_KSTAT_INSERT);
+    k_exact_tiles<<<dim3((unsigned)row_tiles, (unsigned)nsplit), 256, 0, s>>>(
+        X, Dp, pool, pmax, b0, B, n, k, nsplit, part, flag, 0);
+    LAUNCH_CHECK();
+    k_exact_merge<<<(unsigned)B, 256, 0, s>>>(part, nsplit, pool, b0, B, flag);
+    LAUNCH_CHECK();
+    ev_stop(1);
+  }
+
   // ------------------------------------------------------------- update
   // One NN-Descent pass over `b` (reference UpdateAlgo::run,
   // update_kernel.cuh:913-1054); returns the fresh top-k edge count.
   uint64_t nnd_pass(std::vector<uint32_t> b, std::vector<uint32_t> &affected) {
+    affected.clear();
+    if (p.knn_mode != GM_KNN_NNDESCENT)
+      return 0; // exact pools hold only old entries: nothing to join
     std::sort(b.begin(), b.end());
     b.erase(std::unique(b.begin(), b.end()), b.end());
     const uint64_t B = b.size();
@@ -437,9 +565,12 @@ struct Engine {
         scal + S_OWNERS);
     LAUNCH_CHECK();
     const unsigned jgrid = (unsigned)std::min<uint64_t>(n, 2048);
+    ev_start(GM_KSTAT_JOIN);
     k_join<<<jgrid, 256, 0, s>>>(X, Dp, pool, fn, fo, fnc, foc, rev_new,
-                                 rev_old, owners, scal + S_OWNERS, flag, k);
+                                 rev_old, owners, scal + S_OWNERS, flag, k,
+                                 kstats);
     LAUNCH_CHECK();
+    ev_stop(1);
     k_join_reset<<<blocks_for(n * W, 256), 256, 0, s>>>(
         owners, scal + S_OWNERS, rev_new, rev_old, owner_mark, fnc, foc);
     LAUNCH_CHECK();
@@ -542,6 +673,8 @@ struct Engine {
     k_rounding<<<1, 1, 0, s>>>(maxdeg, alpha0, rounding);
     LAUNCH_CHECK();
     const unsigned grid = std::min<unsigned>(blocks_for(cap, 256), 16384);
+    unsigned long long *st = prof ? kstats : nullptr;
+    ev_start(GM_KSTAT_SGD);
     float alpha = alpha0;
     for (int epoch = 0; epoch < n_epochs; ++epoch) {
       const uint64_t es = seed + (uint64_t)epoch;
@@ -550,28 +683,36 @@ struct Engine {
         k_sgd_epoch<1><<<grid, 256, 0, s>>>(e, head, tail, scal + S_NNZ,
                                             (uint32_t)nrows, eps, nxt, nneg,
                                             alpha, epoch, gamma, a, b, nsr_inv,
-                                            es, rounding);
+                                            es, rounding, st);
         break;
       case 2:
         k_sgd_epoch<2><<<grid, 256, 0, s>>>(e, head, tail, scal + S_NNZ,
                                             (uint32_t)nrows, eps, nxt, nneg,
                                             alpha, epoch, gamma, a, b, nsr_inv,
-                                            es, rounding);
+                                            es, rounding, st);
         break;
       case 3:
         k_sgd_epoch<3><<<grid, 256, 0, s>>>(e, head, tail, scal + S_NNZ,
                                             (uint32_t)nrows, eps, nxt, nneg,
                                             alpha, epoch, gamma, a, b, nsr_inv,
-                                            es, rounding);
+                                            es, rounding, st);
         break;
       default:
         k_sgd_epoch<4><<<grid, 256, 0, s>>>(e, head, tail, scal + S_NNZ,
                                             (uint32_t)nrows, eps, nxt, nneg,
                                             alpha, epoch, gamma, a, b, nsr_inv,
-                                            es, rounding);
+                                            es, rounding, st);
       }
       LAUNCH_CHECK();
       alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
+    }
+    ev_stop((uint64_t)n_epochs);
+    if (prof) {
+      uint32_t nnz = 0;
+      HIPC(hipMemcpyAsync(h_scal + 10, scal + S_NNZ, 4, hipMemcpyDeviceToHost, s));
+      HIPC(hipStreamSynchronize(s));
+      nnz = h_scal[10];
+      sgd_edge_epochs += (uint64_t)nnz * (uint64_t)n_epochs;
     }
   }
 
@@ -691,6 +832,7 @@ struct Engine {
     r->insertion_completed = insertion_completed;
     r->update_completed = update_completed;
     r->embedding_completed = embedding_completed;
+    collect_stats();
   }
 };
 
@@ -801,6 +943,7 @@ void gm_default_params(gm_params *o) {
   o->alpha = 1.0f;
   o->verbose = 0;
   o->metric = GM_EUCLIDEAN;
+  o->knn_mode = GM_KNN_EXACT_FP32;
 }
 
 int gm_create(const gm_params *params, gm_handle **out) {
@@ -819,6 +962,9 @@ int gm_create(const gm_params *params, gm_handle **out) {
       throw Fail(GM_ERR_INVALID, "n_features must be positive");
     if (q.metric != GM_EUCLIDEAN && q.metric != GM_COSINE)
       throw Fail(GM_ERR_INVALID, "metric must be 0 (euclidean) or 1 (cosine)");
+    if (q.knn_mode != GM_KNN_NNDESCENT && q.knn_mode != GM_KNN_EXACT_FP32)
+      throw Fail(GM_ERR_INVALID, "knn_mode must be 0 (nndescent) or 1 "
+                                 "(exact_fp32)");
     if (q.negative_sample_rate < 1)
       throw Fail(GM_ERR_INVALID, "negative_sample_rate must be >= 1");
     if (q.max_epoch > 1000000)
@@ -924,6 +1070,50 @@ int gm_export_dlpack(gm_handle *h, int32_t which, void **dl) {
   });
 }
 
+int gm_append_data_device(gm_handle *h, const float *device_rows,
+                          uint64_t rows, uint64_t cols) {
+  return guarded([&] {
+    check_handle(h);
+    if (cols != h->e->D)
+      throw Fail(GM_ERR_INVALID,
+                 "n_features mismatch with numpy array second dimension.");
+    if (rows == 0)
+      return;
+    if (!device_rows)
+      throw Fail(GM_ERR_INVALID, "null data pointer");
+    h->e->append_device(device_rows, rows);
+  });
+}
+
+int gm_set_profiling(gm_handle *h, int32_t enable) {
+  return guarded([&] {
+    check_handle(h);
+    h->e->prof = enable != 0;
+  });
+}
+
+int gm_get_kernel_stats(gm_handle *h, int32_t which, gm_kernel_stats *out) {
+  return guarded([&] {
+    check_handle(h);
+    if (which < 0 || which >= GM_KSTAT_COUNT || !out)
+      throw Fail(GM_ERR_INVALID, "bad kernel-stats request");
+    *out = h->e->totals[which];
+  });
+}
+
+int gm_reset_kernel_stats(gm_handle *h) {
+  return guarded([&] {
+    check_handle(h);
+    gm::Engine *e = h->e;
+    HIPC(hipStreamSynchronize(e->s));
+    for (auto &t : e->totals)
+      t = gm_kernel_stats{};
+    e->sgd_edge_epochs = 0;
+    HIPC(hipMemsetAsync(e->kstats, 0, 8 * sizeof(unsigned long long), e->s));
+    HIPC(hipStreamSynchronize(e->s));
+  });
+}
+
 int gm_copy_knn(gm_handle *h, uint32_t *idx, float *dist) {
   return guarded([&] {
     check_handle(h);
@@ -1019,6 +1209,7 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
     q.n_features = 1;
     q.n_neighbors = 2;
     q.n_components = (uint64_t)c;
+    q.knn_mode = GM_KNN_NNDESCENT;
     h = new gm_handle{new gm::Engine(q)};
     gm::Engine *e = h->e;
     e->n = n;

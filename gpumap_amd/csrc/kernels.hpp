@@ -34,14 +34,15 @@ __global__ void k_join(const float *X, uint64_t dp, uint64_t *pool,
                        const uint32_t *fn, const uint32_t *fo, const int *fnc,
                        const int *foc, const uint64_t *rev_new,
                        const uint64_t *rev_old, const uint32_t *owners,
-                       const uint32_t *owner_count, uint8_t *flag, int k);
+                       const uint32_t *owner_count, uint8_t *flag, int k,
+                       unsigned long long *stats);
 __global__ void k_join_reset(const uint32_t *owners,
                              const uint32_t *owner_count, uint64_t *rev_new,
                              uint64_t *rev_old, uint32_t *owner_mark, int *fnc,
                              int *foc);
 __global__ void k_sync_rows(const uint64_t *pool, const uint32_t *rows,
                             uint64_t nrows, uint32_t *pidx, float *pdist,
-                            int k, unsigned long long *fresh);
+                            int k, unsigned long long *fresh, float *pmax);
 __global__ void k_flag_count(const uint8_t *flag, uint64_t lo, uint64_t hi,
                              uint32_t *bcount);
 __global__ void k_scan_small(uint32_t *v, uint64_t n, uint32_t *total);
@@ -50,6 +51,14 @@ __global__ void k_flag_write(uint8_t *flag, uint64_t lo, uint64_t hi,
 __global__ void k_init_embedding(float *emb, int c, const uint32_t *pidx,
                                  const float *pdist, int k, uint64_t b0,
                                  uint64_t nb);
+
+// exact_kernels.hip
+__global__ void k_exact_tiles(const float *X, uint64_t dp, uint64_t *pool,
+                              const float *pmax, uint64_t b0, uint64_t nb,
+                              uint64_t n, int k, int nsplit, uint64_t *part,
+                              uint8_t *flag, int new_flag);
+__global__ void k_exact_merge(const uint64_t *part, int nsplit, uint64_t *pool,
+                              uint64_t b0, uint64_t nb, uint8_t *flag);
 
 // embed_kernels.hip
 __global__ void k_mean_partial(const uint32_t *pidx, const float *pdist,
@@ -90,7 +99,7 @@ __global__ void k_sgd_epoch(float *emb, const int32_t *head,
                             uint32_t n, const float *eps, float *nxt,
                             float *nneg, float alpha, int epoch, float gamma,
                             float a, float b, float nsr_inv, uint64_t seed,
-                            const float *rounding_p);
+                            const float *rounding_p, unsigned long long *stats);
 __global__ void k_reposition(const uint32_t *pidx, const float *pdist,
                              uint64_t n, int k, int euclid, const double *sigma,
                              const double *rho, const uint32_t *rows,

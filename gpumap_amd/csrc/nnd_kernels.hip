@@ -82,13 +82,13 @@ __global__ __launch_bounds__(64) void k_prep_rows(float *X, uint64_t dp, uint64_
     float s = 0.0f;
     for (uint64_t i = 0; i < dp; ++i)
       s = __builtin_fmaf(x[i], x[i], s);
-    s_norm = __fsqrt_rn(s);
+    s_norm = sqrt_rn(s);
   }
   __syncthreads();
   const float nrm = s_norm;
   if (nrm > 0.0f)
     for (uint64_t i = threadIdx.x; i < d; i += blockDim.x)
-      x[i] = __fdiv_rn(x[i], nrm);
+      x[i] = div_rn(x[i], nrm);
 }
 
 // Random initial neighbours (reference initialize_random_neighbors_kernel,
@@ -394,7 +394,7 @@ __global__ __launch_bounds__(256) void k_join(
     const uint64_t *__restrict__ rev_new, const uint64_t *__restrict__ rev_old,
     const uint32_t *__restrict__ owners,
     const uint32_t *__restrict__ owner_count, uint8_t *__restrict__ flag,
-    int k) {
+    int k, unsigned long long *__restrict__ stats) {
   __shared__ uint32_t s_ids[128];
   __shared__ uint32_t s_mid[128 * W];  // members' pool ids, sorted
   __shared__ uint64_t s_mmax[128];
@@ -407,6 +407,8 @@ __global__ __launch_bounds__(256) void k_join(
   const int hw = tid >> 5, hl = tid & 31, half = (tid >> 5) & 1;
   const int rt = tid >> 4, ct = tid & 15;
   const uint32_t n_owner = *owner_count;
+  // work counters of this block: list members, pairs, queued offers
+  unsigned long long c_members = 0, c_pairs = 0, c_offers = 0;
 
   for (uint32_t oi = blockIdx.x; oi < n_owner; oi += gridDim.x) {
     const uint32_t v = owners[oi];
@@ -484,6 +486,8 @@ __global__ __launch_bounds__(256) void k_join(
     __syncthreads();
     const int o = s_o;
     const int m_all = a + o;
+    c_members += (unsigned long long)m_all;
+    c_pairs += (unsigned long long)(a * (a - 1) / 2 + a * o);
     // ---- 2. members' pools
     for (int t = tid; t < 128 * W; t += blockDim.x) {
       const int m = t / W, s = t % W;
@@ -605,6 +609,7 @@ __global__ __launch_bounds__(256) void k_join(
       }
       __syncthreads();
       const int qn = s_qn;
+      c_offers += (unsigned long long)qn;
       for (int e = hw; e < qn; e += 8) {
         const uint32_t tgt = s_ids[s_qtgt[e]];
         const int res = pool_offer_half(pool + (uint64_t)tgt * W, s_qkey[e], k,
@@ -614,6 +619,11 @@ __global__ __launch_bounds__(256) void k_join(
       }
       __syncthreads();
     }
+  }
+  if (tid == 0 && stats && c_members) {
+    atomicAdd(stats + 0, c_members);
+    atomicAdd(stats + 1, c_pairs);
+    atomicAdd(stats + 2, c_offers);
   }
 }
 
@@ -642,12 +652,14 @@ __global__ void k_join_reset(const uint32_t *__restrict__ owners,
 // ------------------------------------------------------ public kNN rows
 // Sorted top-k of each listed row's pool into the public (n x k) arrays
 // (reference stage_and_scatter_knn_rows, matrix_utils.cuh:127-174, without
-// the host staging).  Counts the ids that were not in the previous row.
+// the host staging).  Counts the ids that were not in the previous row and
+// records the pool's maximum distance.
 __global__ void k_sync_rows(const uint64_t *__restrict__ pool,
                             const uint32_t *__restrict__ rows, uint64_t nrows,
                             uint32_t *__restrict__ pidx,
                             float *__restrict__ pdist, int k,
-                            unsigned long long *__restrict__ fresh) {
+                            unsigned long long *__restrict__ fresh,
+                            float *__restrict__ pmax) {
   const uint64_t g = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
   const int hl = threadIdx.x & 31;
   if (g >= nrows)
@@ -667,6 +679,10 @@ __global__ void k_sync_rows(const uint64_t *__restrict__ pool,
   }
   if (hl == 0 && fm)
     atomicAdd(fresh, (unsigned long long)__popcll(fm));
+  // pool maximum (infinite while the pool has an empty slot), the reverse
+  // filter of exact insertion
+  if (hl == 31)
+    pmax[u] = key == EMPTY ? INFINITY : key_dist(key);
 }
 
 // ------------------------------------------------------------ compaction

@@ -33,6 +33,21 @@ typedef enum gm_status {
 
 typedef enum gm_metric { GM_EUCLIDEAN = 0, GM_COSINE = 1 } gm_metric;
 
+/* How insertion builds a new row's neighbours.
+ *   GM_KNN_NNDESCENT   the reference's method: random start + 10 rounds of
+ *                      neighbours-of-neighbours candidates
+ *                      (insertion_kernel.cuh:926-1095), refined by the
+ *                      NN-Descent update
+ *   GM_KNN_EXACT_FP32  exact: FP32 distance tiles against every inserted row
+ *   GM_KNN_EXACT_BF16  exact: BF16 MFMA tiles screen with a proven error
+ *                      bound, survivors re-ranked in FP32 (same graph as
+ *                      GM_KNN_EXACT_FP32) */
+typedef enum gm_knn_mode {
+  GM_KNN_NNDESCENT = 0,
+  GM_KNN_EXACT_FP32 = 1,
+  GM_KNN_EXACT_BF16 = 2
+} gm_knn_mode;
+
 /* Mirrors gpumap::GPUMAPParams (cpp/include/gpumap/gpumap.cuh:15-36) with
  * fixed-width fields; defaults from gm_default_params(). */
 typedef struct gm_params {
@@ -47,6 +62,8 @@ typedef struct gm_params {
   int32_t metric;                /* gm_metric */
   uint64_t n_instances;          /* capacity N */
   uint64_t n_features;           /* D */
+  int32_t knn_mode;              /* gm_knn_mode (no counterpart in the
+                                    reference; default GM_KNN_EXACT_FP32) */
 } gm_params;
 
 /* Mirrors gpumap::GPUMAPResult (cpp/include/gpumap/gpumap.cuh:38-56). */
@@ -126,6 +143,11 @@ int gm_get_view(gm_handle *h, int32_t which, gm_view *out);
  * non-owning __cuda_array_interface__ instead). */
 int gm_export_dlpack(gm_handle *h, int32_t which, void **dl_managed_tensor);
 
+/* append_data from device memory (rows x cols float32, row-major): the same
+ * contract as gm_append_data for callers whose chunks are already in HBM. */
+int gm_append_data_device(gm_handle *h, const float *device_rows,
+                          uint64_t rows, uint64_t cols);
+
 /* Host copies of the public arrays (n_inserted rows). */
 int gm_copy_knn(gm_handle *h, uint32_t *idx, float *dist);
 int gm_copy_embedding(gm_handle *h, float *emb);
@@ -153,6 +175,32 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
                  const int32_t *tail, const float *w, int64_t nnz,
                  int32_t n_epochs, int32_t nsr, float a, float b, float alpha0,
                  float gamma, uint64_t seed);
+
+/* Per-kernel measurement (no counterpart in the reference, which only times
+ * whole operations, impl.cuh:159-203).  With profiling on, HIP events on the
+ * session's stream bracket the timed kernels and device counters record their
+ * algorithmic work:
+ *   GM_KSTAT_JOIN    local join; work = {list members, distance pairs,
+ *                    queued pool offers}
+ *   GM_KSTAT_SGD     SGD epochs; work = {positive samples, negative samples,
+ *                    edge-epochs scanned}
+ *   GM_KSTAT_INSERT  insertion rounds (candidates + offers); work unused */
+typedef enum gm_kstat {
+  GM_KSTAT_JOIN = 0,
+  GM_KSTAT_SGD = 1,
+  GM_KSTAT_INSERT = 2,
+  GM_KSTAT_COUNT = 3
+} gm_kstat;
+
+typedef struct gm_kernel_stats {
+  uint64_t launches;
+  double device_ms; /* summed event time over the launches */
+  uint64_t work[4];
+} gm_kernel_stats;
+
+int gm_set_profiling(gm_handle *h, int32_t enable);
+int gm_get_kernel_stats(gm_handle *h, int32_t which, gm_kernel_stats *out);
+int gm_reset_kernel_stats(gm_handle *h);
 
 /* Philox4x32-10 of the device code, for known-answer tests. */
 int gm_debug_philox(const uint32_t ctr[4], const uint32_t key[2],

@@ -41,6 +41,7 @@ class OrParams(ctypes.Structure):
         ("n_features", ctypes.c_uint64),
         ("n_threads", ctypes.c_int32),
         ("max_iter", ctypes.c_int32),
+        ("knn_mode", ctypes.c_int32),
     ]
 
 
@@ -157,13 +158,14 @@ class Oracle:
 
     def __init__(self, n_instances, n_features, n_neighbors=15, n_components=2,
                  metric=0, a=1.577, b=0.895, alpha=1.0, max_epoch=200,
-                 negative_sample_rate=5, n_threads=1, max_iter=10):
+                 negative_sample_rate=5, n_threads=1, max_iter=10,
+                 knn_mode=0):
         p = OrParams(n_neighbors=n_neighbors, n_components=n_components,
                      negative_sample_rate=negative_sample_rate,
                      max_epoch=max_epoch, min_dist=0.1, a=a, b=b, alpha=alpha,
                      metric=metric, verbose=0, n_instances=n_instances,
                      n_features=n_features, n_threads=n_threads,
-                     max_iter=max_iter)
+                     max_iter=max_iter, knn_mode=knn_mode)
         self._h = lib().or_create(ctypes.byref(p))
         if not self._h:
             raise ValueError("invalid oracle parameters")

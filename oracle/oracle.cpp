@@ -37,6 +37,13 @@
 //  D9 Update convergence threshold max(1, 1e-4 n) rows: the reference's
 //     (size_t)(1e-4 n) is 0 below n = 10,000 and `size < 0` never holds
 //     (impl.cuh:261-263,320), so small sessions would never complete.
+//
+// knn_mode 1 (exact insertion, the MI355X path's default): each inserted row's
+// pool is the exact top-32 (canonical distance, ties by id) over all rows
+// inserted so far, and every earlier row is offered every new row; entries
+// enter as old, so the NN-Descent update has nothing to join and only drains
+// its queue.  By induction every pool is the exact 32-NN set of the rows
+// inserted so far.
 // Everything else (constants, seeds, queue semantics, completion rules,
 // trim/eps/rounding/alpha schedule, noise, reposition) follows the reference.
 #include "oracle.h"
@@ -609,6 +616,52 @@ struct Oracle {
       }
     }
 
+    std::vector<uint8_t> dirty(n, 0);
+    if (p.knn_mode == 1) {
+      exact_insert(b0, B, dirty);
+    } else {
+      nnd_insert(b0, B, window_start, seed, dirty);
+    }
+
+    // affected rows (CPU diff :1034-1089): batch rows + changed existing rows
+    std::vector<uint32_t> affected(batch.begin(), batch.end());
+    for (uint64_t r = 0; r < b0; ++r)
+      if (dirty[r])
+        affected.push_back((uint32_t)r);
+    for (uint32_t r : affected)
+      sync_row(r);
+    init_embedding(b0, B);
+    for (uint32_t r : affected)
+      update_queue.push(r);
+    return (uint32_t)B;
+  }
+
+  // exact insertion: forward top-W over [0, n) and reverse offers to [0, b0)
+  void exact_insert(uint64_t b0, uint64_t B, std::vector<uint8_t> &dirty) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nt)
+    for (uint64_t bi = 0; bi < B; ++bi) {
+      const uint32_t u = (uint32_t)(b0 + bi);
+      uint64_t *row = &pool[(size_t)u * W];
+      for (int j = 0; j < W; ++j)
+        row[j] = EMPTY;
+      for (uint64_t j = 0; j < n; ++j) {
+        if (j == u)
+          continue;
+        offer(row, k, make_key(dist(u, (uint32_t)j), (uint32_t)j, false));
+      }
+    }
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
+    for (uint64_t j = 0; j < b0; ++j) {
+      for (uint64_t bi = 0; bi < B; ++bi) {
+        const uint32_t u = (uint32_t)(b0 + bi);
+        if (offer(&pool[j * W], k, make_key(dist(u, (uint32_t)j), u, false)))
+          dirty[j] = 1;
+      }
+    }
+  }
+
+  void nnd_insert(uint64_t b0, uint64_t B, uint32_t window_start,
+                  uint32_t seed, std::vector<uint8_t> &dirty) {
     // random initial neighbours (initialize_random_neighbors_kernel :170-248)
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
     for (uint64_t bi = 0; bi < B; ++bi) {
@@ -647,7 +700,6 @@ struct Oracle {
         row[q] = make_key(dist(u, ids[q]), ids[q], true);
     }
 
-    std::vector<uint8_t> dirty(n, 0);
     std::vector<uint32_t> cand((size_t)B * C);
     std::vector<float> cdist((size_t)B * C);
     std::vector<int> ccount(B);
@@ -705,15 +757,10 @@ struct Oracle {
       }
     }
 
-    // affected rows (CPU diff :1034-1089): batch rows + changed existing rows
-    std::vector<uint32_t> affected(batch.begin(), batch.end());
-    for (uint64_t r = 0; r < b0; ++r)
-      if (dirty[r])
-        affected.push_back((uint32_t)r);
-    for (uint32_t r : affected)
-      sync_row(r);
+  }
 
-    // embedding init (initialize_embeddings :790-878)
+  // embedding init (initialize_embeddings :790-878)
+  void init_embedding(uint64_t b0, uint64_t B) {
     for (uint64_t bi = 0; bi < B; ++bi) {
       const uint64_t u = b0 + bi;
       Rng rng(42, (uint32_t)u, 0, TAG_EMB_INIT);
@@ -741,9 +788,6 @@ struct Oracle {
         for (int d = 0; d < c; ++d)
           emb[u * c + d] = acc[d] / wsum;
     }
-    for (uint32_t r : affected)
-      update_queue.push(r);
-    return (uint32_t)B;
   }
 
   // ------------------------------------------------------------- update

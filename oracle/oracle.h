@@ -47,6 +47,8 @@ typedef struct or_params {
   uint64_t n_features;
   int32_t n_threads; /* OpenMP threads for the timed baseline; 1 = serial */
   int32_t max_iter;  /* insertion NN-D rounds (reference: 10) */
+  int32_t knn_mode;  /* 0 NN-Descent insertion (reference method),
+                        1 exact insertion (brute force over all rows) */
 } or_params;
 
 typedef struct or_result {

@@ -7,7 +7,7 @@ import ctypes
 
 import pytest
 
-from tests import capi
+from gpumap_amd import capi
 
 REFERENCE_ALL = [
     "GPUMAP", "GPUMAPParams", "Metric", "BaseStrategy", "WeightedStrategy",
@@ -69,6 +69,11 @@ def test_params_defaults_and_metric_property():
     assert p.min_dist == pytest.approx(0.1)
     assert (p.a, p.b, p.alpha, p.verbose) == (1.0, 1.0, 1.0, False)
     assert p.metric == "euclidean"
+    assert p.knn_mode == "exact_fp32"
+    p.knn_mode = "nndescent"
+    assert p.knn_mode == "nndescent"
+    with pytest.raises(ValueError):
+        p.knn_mode = "approximate"
     p.metric = "CoSiNe"
     assert p.metric == "cosine"
     p.metric = Metric.Euclidean

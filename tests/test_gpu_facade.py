@@ -27,7 +27,8 @@ def test_facade_getters_are_device_views_equal_to_oracle():
 
     x = ev.gaussian10(n=2000, d=24)
     m = GPUMAP(ArrayDataLoader(x), _params(2000, 24))
-    o = oracle.Oracle(2000, 24, a=m.params.a, b=m.params.b, n_threads=8)
+    o = oracle.Oracle(2000, 24, a=m.params.a, b=m.params.b, n_threads=8,
+                      knn_mode=1)
     for s in range(0, 2000, 400):
         res = m.run_with_n_ops(400, 400, 0)
         o.append(x[s:s + 400])

@@ -13,7 +13,7 @@ import pytest
 
 import oracle
 from gpumap_amd import evaluation as ev
-from tests import capi
+from gpumap_amd import capi
 
 pytestmark = pytest.mark.gpu
 
@@ -25,11 +25,12 @@ RESULT_FIELDS = ["n_insertion_ops", "n_update_ops", "n_embedding_ops",
 
 
 def _lockstep(x, chunk, k=15, metric=0, n_ops=None, embed=0, max_cycles=80,
-              check_every=1):
+              check_every=1, mode=0):
     n, d = x.shape
     n_ops = n_ops or chunk
-    e = capi.Engine(n, d, n_neighbors=k, metric=metric)
-    o = oracle.Oracle(n, d, n_neighbors=k, metric=metric, n_threads=8)
+    e = capi.Engine(n, d, n_neighbors=k, metric=metric, knn_mode=mode)
+    o = oracle.Oracle(n, d, n_neighbors=k, metric=metric, n_threads=8,
+                      knn_mode=min(mode, 1))
     pos, cyc = 0, 0
     while True:
         ins = min(chunk, n - pos)
@@ -62,31 +63,47 @@ def test_philox_device_matches_published_vectors():
         assert tuple(int(v) for v in capi.philox(ctr, key)) == want
 
 
-def test_knn_bit_exact_euclidean():
+MODES = [pytest.param(0, id="nndescent"), pytest.param(1, id="exact_fp32")]
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_knn_bit_exact_euclidean(mode):
     x = ev.gaussian10(n=3000, d=32)
-    _lockstep(x, chunk=500)
+    _lockstep(x, chunk=500, mode=mode)
 
 
-def test_knn_bit_exact_cosine():
+@pytest.mark.parametrize("mode", MODES)
+def test_knn_bit_exact_cosine(mode):
     x = ev.gaussian10(n=2000, d=40)
-    _lockstep(x, chunk=400, metric=1)
+    _lockstep(x, chunk=400, metric=1, mode=mode)
 
 
-def test_knn_bit_exact_k30_odd_dim():
+@pytest.mark.parametrize("mode", MODES)
+def test_knn_bit_exact_k30_odd_dim(mode):
     # k = 30 fills the 256-candidate cap; D = 37 exercises the padding
     g = np.random.default_rng(3)
     x = (g.standard_normal((2500, 37)) * 3).astype(np.float32)
-    _lockstep(x, chunk=500, k=30)
+    _lockstep(x, chunk=500, k=30, mode=mode)
 
 
-def test_knn_bit_exact_tiny_batches_and_duplicates():
+@pytest.mark.parametrize("mode", MODES)
+def test_knn_bit_exact_tiny_batches_and_duplicates(mode):
     # first batches smaller than the pool, exact duplicate rows (distance-0
     # ties broken by id), partial update ops drawn from the queue
     g = np.random.default_rng(4)
     base = g.standard_normal((150, 8)).astype(np.float32)
     x = np.concatenate([base, base[:50]], 0)
     x = x[g.permutation(len(x))]
-    _lockstep(x, chunk=7, n_ops=40, max_cycles=200)
+    _lockstep(x, chunk=7, n_ops=40, max_cycles=200, mode=mode)
+
+
+def test_exact_mode_is_exact_knn():
+    # the exact pools give the FP64 brute-force neighbours (tie-aware)
+    x = ev.gaussian10(n=3000, d=32)
+    e, o = _lockstep(x, chunk=700, mode=1)
+    idx, _ = e.knn()
+    ei, ed = ev.exact_knn(x, 15)
+    assert ev.recall_at_k(idx, x, ei, ed) == 1.0
 
 
 def _fp64_union(idx, dist, sig, rho, batch, k):
@@ -116,7 +133,7 @@ def _fp64_union(idx, dist, sig, rho, batch, k):
 
 def test_fuzzy_set_matches_oracle_and_fp64():
     x = ev.gaussian10(n=3000, d=32)
-    e, o = _lockstep(x, chunk=1000, check_every=10)
+    e, o = _lockstep(x, chunk=1000, check_every=10, mode=0)
     idx, dist = e.knn()
     k = 15
     for batch in (np.arange(3000, dtype=np.uint32),
@@ -163,13 +180,14 @@ def test_sgd_disjoint_edges_match_oracle():
     np.testing.assert_allclose(out_g, out_o, rtol=2e-6, atol=1e-6)
 
 
-def test_gaussian10_quality_and_exact_graph():
+@pytest.mark.parametrize("mode", MODES)
+def test_gaussian10_quality_and_exact_graph(mode):
     """10,000 x 64 with the reference's op plan: 500-row insertions with
     run(500, 500, 500), then full passes until complete."""
     x = ev.gaussian10()
     n, d = x.shape
-    e = capi.Engine(n, d)
-    o = oracle.Oracle(n, d, n_threads=16)
+    e = capi.Engine(n, d, knn_mode=mode)
+    o = oracle.Oracle(n, d, n_threads=16, knn_mode=mode)
     pos = 0
     while pos < n:
         e.append(x[pos:pos + 500])
