@@ -91,7 +91,7 @@ def lib():
                                      P(ctypes.c_double), P(ctypes.c_double),
                                      P(i64)]
         L.gm_debug_sgd.argtypes = [P(f32), u64, i32, P(i32), P(i32), P(f32),
-                                   i64, i32, i32, f32, f32, f32, f32, u64]
+                                   i64, i32, i32, f32, f32, f32, f32, u64, i32]
         L.gm_debug_philox.argtypes = [P(u32), P(u32), P(u32)]
         L.gm_append_data_device.argtypes = [vp, vp, u64, u64]
         L.gm_set_profiling.argtypes = [vp, i32]
@@ -118,14 +118,16 @@ def philox(ctr, key):
     return out
 
 
-def sgd(emb, head, tail, w, n_epochs, nsr, a, b, alpha0=1.0, gamma=1.0, seed=42):
+def sgd(emb, head, tail, w, n_epochs, nsr, a, b, alpha0=1.0, gamma=1.0, seed=42,
+        per_epoch=False):
     emb = np.ascontiguousarray(emb, np.float32).copy()
     head = np.ascontiguousarray(head, np.int32)
     tail = np.ascontiguousarray(tail, np.int32)
     w = np.ascontiguousarray(w, np.float32)
     check(lib().gm_debug_sgd(_p(emb, f32), emb.shape[0], emb.shape[1],
                              _p(head, i32), _p(tail, i32), _p(w, f32), len(w),
-                             n_epochs, nsr, a, b, alpha0, gamma, seed))
+                             n_epochs, nsr, a, b, alpha0, gamma, seed,
+                             1 if per_epoch else 0))
     return emb
 
 

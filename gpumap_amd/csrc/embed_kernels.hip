@@ -9,6 +9,9 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <cstdio>
+#include <cstdlib>
+
 namespace gm {
 
 namespace {
@@ -374,14 +377,127 @@ __global__ __launch_bounds__(256) void k_noise(float *__restrict__ emb, int c,
 }
 
 // --------------------------------------------------------------- SGD
+// x^y for x > 0 through the hardware base-2 exponential and logarithm
+// (about 2 ulp; the SGD is compared statistically)
+__device__ inline float pow_pos(float x, float y) {
+  return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+
 __device__ inline float fclip(float v) {
   if (!isfinite(v))
     return 0.0f;
   return fmaxf(fminf(v, 4.0f), -4.0f);
 }
 
-// One epoch of the Hogwild edge sampler (reference optimize_batch_kernel
-// :509-589) for any n_components <= 4.
+// One positive sample of edge e and its negative samples (reference
+// optimize_batch_kernel :509-589) for any n_components <= 4.  The negative
+// samples' positions are gathered together before their sequential updates
+// of the head (the values a sequential read would see in a Hogwild run).
+template <int NC>
+__device__ inline void sgd_edge(float *__restrict__ emb, int32_t j, int32_t kk,
+                                uint64_t e, uint32_t n, float ep, float *nxt_e,
+                                float *nneg_e, float alpha, int epoch,
+                                float gamma, float a, float b, float nsr_inv,
+                                uint64_t seed, float rounding,
+                                unsigned long long &c_pos,
+                                unsigned long long &c_neg) {
+  const float ns = *nxt_e;
+  const float ep_neg = ep * nsr_inv;
+  ++c_pos;
+  float *cur_p = emb + (uint64_t)j * NC;
+  float *oth_p = emb + (uint64_t)kk * NC;
+  float cur[NC], oth[NC], g[NC];
+#pragma unroll
+  for (int d = 0; d < NC; ++d) {
+    cur[d] = cur_p[d];
+    oth[d] = oth_p[d];
+  }
+  float d2 = 0.0f;
+#pragma unroll
+  for (int d = 0; d < NC; ++d) {
+    const float t = cur[d] - oth[d];
+    d2 += t * t;
+  }
+  float attr = 0.0f;
+  if (d2 > 0.0f) {
+    const float pb = pow_pos(d2, b);
+    attr = (-2.0f * a * b * (pb / d2)) / (a * pb + 1.0f);
+  }
+#pragma unroll
+  for (int d = 0; d < NC; ++d) {
+    g[d] = fclip(attr * (cur[d] - oth[d])) * alpha;
+    cur[d] += g[d];
+  }
+#pragma unroll
+  for (int d = 0; d < NC; ++d)
+    unsafeAtomicAdd(oth_p + d, (rounding + -g[d]) - rounding);
+  *nxt_e = ns + ep;
+  const float nn = *nneg_e;
+  const int n_neg = (int)(((float)epoch - nn) / ep_neg);
+  if (n_neg > 0) {
+    Rng rng(seed, (uint32_t)e, (uint32_t)(e >> 32), TAG_NEG);
+    for (int p0 = 0; p0 < n_neg; p0 += 8) {
+      const int cnt = min(8, n_neg - p0);
+      uint32_t r[8];
+      float neg[8][NC];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q < cnt) {
+          r[q] = rng.next() % n;
+          const float *np_ = emb + (uint64_t)r[q] * NC;
+#pragma unroll
+          for (int d = 0; d < NC; ++d)
+            neg[q][d] = np_[d];
+        }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if (q >= cnt || (int32_t)r[q] == j)
+          continue;
+        ++c_neg;
+        float d2n = 0.0f;
+#pragma unroll
+        for (int d = 0; d < NC; ++d) {
+          const float t = cur[d] - neg[q][d];
+          d2n += t * t;
+        }
+        float rep = 0.0f;
+        if (d2n > 0.0f)
+          rep = (2.0f * gamma * b) /
+                ((0.001f + d2n) * (a * pow_pos(d2n, b) + 1.0f));
+#pragma unroll
+        for (int d = 0; d < NC; ++d) {
+          const float gg =
+              rep > 0.0f ? fclip(rep * (cur[d] - neg[q][d])) * alpha
+                         : 4.0f * alpha;
+          cur[d] += gg;
+          g[d] += gg;
+        }
+      }
+    }
+    *nneg_e = nn + (float)n_neg * ep_neg;
+  }
+#pragma unroll
+  for (int d = 0; d < NC; ++d)
+    unsafeAtomicAdd(cur_p + d, (rounding + g[d]) - rounding);
+}
+
+__device__ inline void flush_sgd_stats(unsigned long long *stats,
+                                       unsigned long long c_pos,
+                                       unsigned long long c_neg) {
+  if (!stats)
+    return;
+  for (int off = 32; off > 0; off >>= 1) {
+    c_pos += __shfl_xor(c_pos, off);
+    c_neg += __shfl_xor(c_neg, off);
+  }
+  if ((threadIdx.x & 63) == 0 && (c_pos | c_neg)) {
+    atomicAdd(stats + 3, c_pos);
+    atomicAdd(stats + 4, c_neg);
+  }
+}
+
+// One epoch of the Hogwild edge sampler, one launch per epoch (used when the
+// edge set is too large for the register-resident kernel below).
 template <int NC>
 __global__ __launch_bounds__(256) void k_sgd_epoch(
     float *__restrict__ emb, const int32_t *__restrict__ head,
@@ -395,85 +511,206 @@ __global__ __launch_bounds__(256) void k_sgd_epoch(
   unsigned long long c_pos = 0, c_neg = 0;
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
        e += (uint64_t)gridDim.x * blockDim.x) {
-    const float ns = nxt[e];
-    if (ns > (float)epoch)
+    if (nxt[e] > (float)epoch)
       continue;
-    const float ep = eps[e];
-    const float ep_neg = ep * nsr_inv;
-    const int32_t j = head[e], kk = tail[e];
-    ++c_pos;
-    float *cur_p = emb + (uint64_t)j * NC;
-    float *oth_p = emb + (uint64_t)kk * NC;
-    float cur[NC], oth[NC], g[NC];
-#pragma unroll
-    for (int d = 0; d < NC; ++d) {
-      cur[d] = cur_p[d];
-      oth[d] = oth_p[d];
-    }
-    float d2 = 0.0f;
-#pragma unroll
-    for (int d = 0; d < NC; ++d) {
-      const float t = cur[d] - oth[d];
-      d2 += t * t;
-    }
-    float attr = 0.0f;
-    if (d2 > 0.0f)
-      attr = (-2.0f * a * b * powf(d2, b - 1.0f)) / (a * powf(d2, b) + 1.0f);
-#pragma unroll
-    for (int d = 0; d < NC; ++d) {
-      g[d] = fclip(attr * (cur[d] - oth[d])) * alpha;
-      cur[d] += g[d];
-    }
-#pragma unroll
-    for (int d = 0; d < NC; ++d)
-      unsafeAtomicAdd(oth_p + d, (rounding + -g[d]) - rounding);
-    nxt[e] = ns + ep;
-    const float nn = nneg[e];
-    const int n_neg = (int)(((float)epoch - nn) / ep_neg);
-    if (n_neg > 0) {
-      Rng rng(seed, (uint32_t)e, (uint32_t)(e >> 32), TAG_NEG);
-      for (int p = 0; p < n_neg; ++p) {
-        const uint32_t r = rng.next() % n;
-        if ((int32_t)r == j)
-          continue;
-        ++c_neg;
-        const float *neg_p = emb + (uint64_t)r * NC;
-        float neg[NC];
-        float d2n = 0.0f;
-#pragma unroll
-        for (int d = 0; d < NC; ++d) {
-          neg[d] = neg_p[d];
-          const float t = cur[d] - neg[d];
-          d2n += t * t;
-        }
-        float rep = 0.0f;
-        if (d2n > 0.0f)
-          rep = (2.0f * gamma * b) / ((0.001f + d2n) * (a * powf(d2n, b) + 1.0f));
-#pragma unroll
-        for (int d = 0; d < NC; ++d) {
-          const float gg =
-              rep > 0.0f ? fclip(rep * (cur[d] - neg[d])) * alpha : 4.0f * alpha;
-          cur[d] += gg;
-          g[d] += gg;
-        }
-      }
-      nneg[e] = nn + (float)n_neg * ep_neg;
-    }
-#pragma unroll
-    for (int d = 0; d < NC; ++d)
-      unsafeAtomicAdd(cur_p + d, (rounding + g[d]) - rounding);
+    sgd_edge<NC>(emb, head[e], tail[e], e, n, eps[e], nxt + e, nneg + e,
+                 alpha, epoch, gamma, a, b, nsr_inv, seed, rounding, c_pos,
+                 c_neg);
   }
-  if (stats) { // one atomic per wave and counter
-    for (int off = 32; off > 0; off >>= 1) {
-      c_pos += __shfl_xor(c_pos, off);
-      c_neg += __shfl_xor(c_neg, off);
+  flush_sgd_stats(stats, c_pos, c_neg);
+}
+
+// Device-wide barrier of the fused SGD: one monotonically increasing arrival
+// counter; barrier number i is passed once the counter reaches (i + 1) *
+// nblocks.  Same-address relaxed atomics are coherent, so no fence is needed
+// for the barrier itself, and the Hogwild updates need no ordering between
+// epochs.  The wait is bounded: a barrier that times out sets bar[2] (the
+// host reports it) and the kernel still runs to completion.
+__device__ inline void grid_barrier(unsigned *bar, unsigned nblocks,
+                                    unsigned index) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned target = (index + 1u) * nblocks;
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned spins = 0;
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+           target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > (1u << 22)) {
+        __hip_atomic_store(bar + 2, 1u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
     }
-    if ((threadIdx.x & 63) == 0 && (c_pos | c_neg)) {
-      atomicAdd(stats + 3, c_pos);
-      atomicAdd(stats + 4, c_neg);
+  }
+  __syncthreads();
+}
+
+// All epochs in one cooperative launch: each thread keeps EPT edges' ids and
+// schedule in registers for the whole run, and a device-wide barrier separates
+// the epochs (the reference launches one kernel per epoch, :785-793).
+template <int NC, int EPT>
+__global__ __launch_bounds__(256) void k_sgd_fused(
+    float *__restrict__ emb, const int32_t *__restrict__ head,
+    const int32_t *__restrict__ tail, const uint32_t *__restrict__ nnz_p,
+    uint32_t n, const float *__restrict__ eps, int n_epochs, float alpha0,
+    float gamma, float a, float b, float nsr_inv, uint64_t seed,
+    const float *__restrict__ rounding_p, unsigned *__restrict__ bar,
+    unsigned long long *__restrict__ stats) {
+  const uint64_t nnz = *nnz_p;
+  const float rounding = *rounding_p;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t hh[EPT], tt[EPT];
+  float ep[EPT], nx[EPT], nn[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const uint64_t e = t0 + (uint64_t)q * stride;
+    if (e < nnz) {
+      hh[q] = head[e];
+      tt[q] = tail[e];
+      ep[q] = eps[e];
+      nx[q] = ep[q];
+      nn[q] = ep[q] * nsr_inv;
+    } else {
+      hh[q] = tt[q] = 0;
+      ep[q] = -1.0f;
+      nx[q] = INFINITY;
+      nn[q] = 0.0f;
     }
+  }
+  unsigned long long c_pos = 0, c_neg = 0;
+  float alpha = alpha0;
+  for (int epoch = 0; epoch < n_epochs; ++epoch) {
+    const uint64_t es = seed + (uint64_t)epoch;
+#pragma unroll
+    for (int q = 0; q < EPT; ++q)
+      if (!(nx[q] > (float)epoch))
+        sgd_edge<NC>(emb, hh[q], tt[q], t0 + (uint64_t)q * stride, n, ep[q],
+                     &nx[q], &nn[q], alpha, epoch, gamma, a, b, nsr_inv, es,
+                     rounding, c_pos, c_neg);
+    alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
+    grid_barrier(bar, gridDim.x, (unsigned)epoch);
+  }
+  flush_sgd_stats(stats, c_pos, c_neg);
+}
+
+// Host launcher of k_sgd_fused, in the kernels' own translation unit: picks
+// the smallest EPT whose co-resident grid holds every edge, launches it
+// cooperatively and returns hipSuccess; hipErrorInvalidConfiguration when no
+// EPT fits (the caller then launches one kernel per epoch).
+template <int NC, int EPT>
+static hipError_t try_fused(float *emb, const int32_t *head,
+                            const int32_t *tail, const uint32_t *nnz_p,
+                            uint32_t n, const float *eps, int n_epochs,
+                            float alpha0, float gamma, float a, float b,
+                            float nsr_inv, uint64_t seed,
+                            const float *rounding_p, unsigned *bar,
+                            unsigned long long *stats, uint64_t nnz,
+                            int num_cu, hipStream_t s) {
+  static int per_cu = -1;
+  if (per_cu < 0) {
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void *>(&k_sgd_fused<NC, EPT>), 256,
+        0);
+    if (e != hipSuccess) {
+      per_cu = -1;
+      return e;
+    }
+  }
+  const uint64_t need = (nnz + 256ull * EPT - 1) / (256ull * EPT);
+  static const bool dbg = std::getenv("GPUMAP_DEBUG_SGD") != nullptr;
+  if (dbg)
+    fprintf(stderr, "[sgd] nc=%d ept=%d per_cu=%d num_cu=%d need=%llu\n", NC,
+            EPT, per_cu, num_cu, (unsigned long long)need);
+  if (need == 0 || need > (uint64_t)per_cu * (uint64_t)num_cu)
+    return hipErrorInvalidConfiguration;
+  void *args[] = {&emb,   &head, &tail,    &nnz_p, &n,          &eps,
+                  &n_epochs, &alpha0, &gamma, &a,  &b,          &nsr_inv,
+                  &seed,  &rounding_p, &bar, &stats};
+  return hipLaunchCooperativeKernel(
+      reinterpret_cast<const void *>(&k_sgd_fused<NC, EPT>),
+      dim3((unsigned)need), dim3(256), args, 0, s);
+}
+
+template <int NC>
+static hipError_t try_fused_any(float *emb, const int32_t *head,
+                                const int32_t *tail, const uint32_t *nnz_p,
+                                uint32_t n, const float *eps, int n_epochs,
+                                float alpha0, float gamma, float a, float b,
+                                float nsr_inv, uint64_t seed,
+                                const float *rounding_p, unsigned *bar,
+                                unsigned long long *stats, uint64_t nnz,
+                                int num_cu, hipStream_t s) {
+  hipError_t e = try_fused<NC, 1>(emb, head, tail, nnz_p, n, eps, n_epochs,
+                                  alpha0, gamma, a, b, nsr_inv, seed,
+                                  rounding_p, bar, stats, nnz, num_cu, s);
+  if (e != hipErrorInvalidConfiguration)
+    return e;
+  e = try_fused<NC, 2>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0, gamma,
+                       a, b, nsr_inv, seed, rounding_p, bar, stats, nnz,
+                       num_cu, s);
+  if (e != hipErrorInvalidConfiguration)
+    return e;
+  e = try_fused<NC, 4>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0, gamma,
+                       a, b, nsr_inv, seed, rounding_p, bar, stats, nnz,
+                       num_cu, s);
+  if (e != hipErrorInvalidConfiguration)
+    return e;
+  return try_fused<NC, 8>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0,
+                          gamma, a, b, nsr_inv, seed, rounding_p, bar, stats,
+                          nnz, num_cu, s);
+}
+
+hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
+                            const int32_t *tail, const uint32_t *nnz_p,
+                            uint32_t n, const float *eps, int n_epochs,
+                            float alpha0, float gamma, float a, float b,
+                            float nsr_inv, uint64_t seed,
+                            const float *rounding_p, unsigned *bar,
+                            unsigned long long *stats, uint64_t nnz,
+                            int num_cu, hipStream_t s) {
+  switch (nc) {
+  case 1:
+    return try_fused_any<1>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0,
+                            gamma, a, b, nsr_inv, seed, rounding_p, bar, stats,
+                            nnz, num_cu, s);
+  case 2:
+    return try_fused_any<2>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0,
+                            gamma, a, b, nsr_inv, seed, rounding_p, bar, stats,
+                            nnz, num_cu, s);
+  case 3:
+    return try_fused_any<3>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0,
+                            gamma, a, b, nsr_inv, seed, rounding_p, bar, stats,
+                            nnz, num_cu, s);
+  default:
+    return try_fused_any<4>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0,
+                            gamma, a, b, nsr_inv, seed, rounding_p, bar, stats,
+                            nnz, num_cu, s);
   }
 }
+
+#define GM_SGD_FUSED(NC, EPT)                                                  \
+  template __global__ void k_sgd_fused<NC, EPT>(                               \
+      float *, const int32_t *, const int32_t *, const uint32_t *, uint32_t,   \
+      const float *, int, float, float, float, float, float, uint64_t,         \
+      const float *, unsigned *, unsigned long long *);
+GM_SGD_FUSED(1, 1)
+GM_SGD_FUSED(1, 2)
+GM_SGD_FUSED(1, 4)
+GM_SGD_FUSED(1, 8)
+GM_SGD_FUSED(2, 1)
+GM_SGD_FUSED(2, 2)
+GM_SGD_FUSED(2, 4)
+GM_SGD_FUSED(2, 8)
+GM_SGD_FUSED(3, 1)
+GM_SGD_FUSED(3, 2)
+GM_SGD_FUSED(3, 4)
+GM_SGD_FUSED(3, 8)
+GM_SGD_FUSED(4, 1)
+GM_SGD_FUSED(4, 2)
+GM_SGD_FUSED(4, 4)
+GM_SGD_FUSED(4, 8)
 
 template __global__ void k_sgd_epoch<2>(float *, const int32_t *,
                                         const int32_t *, const uint32_t *,

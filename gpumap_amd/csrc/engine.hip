@@ -44,7 +44,8 @@ void hip_check(hipError_t e, const char *what, int line) {
     return;
   const int code = (e == hipErrorOutOfMemory) ? GM_ERR_OOM : GM_ERR_HIP;
   throw Fail(code, std::string(what) + " failed at engine.hip:" +
-                       std::to_string(line) + ": " + hipGetErrorString(e));
+                       std::to_string(line) + ": " + hipGetErrorString(e) +
+                       " (hipError " + std::to_string((int)e) + ")");
 }
 #define HIPC(x) ::gm::hip_check((x), #x, __LINE__)
 #define LAUNCH_CHECK() HIPC(hipGetLastError())
@@ -160,6 +161,9 @@ struct Engine {
   uint8_t *wflags = nullptr;
   int32_t *head = nullptr, *tail = nullptr;
   int *deg = nullptr, *maxdeg = nullptr;
+  unsigned *gbar = nullptr; // grid barrier of the fused SGD: count, gen, timeout
+  int num_cu = 0;
+  bool fused_used = false;
   uint32_t *dbatch = nullptr;
   // pinned host staging
   float *h_stage = nullptr;
@@ -251,6 +255,10 @@ struct Engine {
     rounding = dalloc<float>(1);
     deg = dalloc<int>(N);
     maxdeg = dalloc<int>(1);
+    gbar = dalloc<unsigned>(4);
+    HIPC(hipMemsetAsync(gbar, 0, 4 * sizeof(unsigned), s));
+    HIPC(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount,
+                               device));
     dbatch = dalloc<uint32_t>(N);
     HIPC(hipHostMalloc((void **)&h_list, N * sizeof(uint32_t)));
     HIPC(hipHostMalloc((void **)&h_scal, S_COUNT * sizeof(uint32_t) + 64));
@@ -299,6 +307,9 @@ struct Engine {
     if (kstats)
       (void)hipFree(kstats);
     kstats = nullptr;
+    if (gbar)
+      (void)hipFree(gbar);
+    gbar = nullptr;
     if (pmax)
       (void)hipFree(pmax);
     if (part)
@@ -657,7 +668,8 @@ struct Engine {
   // maximum weight in scal[S_MAXW] (reference run_optimize_layout,
   // embedding_kernel.cuh:726-798).  `cap` bounds nnz for the launch grids.
   void run_sgd(float *e, uint64_t nrows, uint64_t cap, int n_epochs, int nsr,
-               float a, float b, float alpha0, float gamma, uint64_t seed) {
+               float a, float b, float alpha0, float gamma, uint64_t seed,
+               bool force_per_epoch = false) {
     if (cap == 0 || n_epochs <= 0)
       return;
     HIPC(hipMemsetAsync(deg, 0, nrows * sizeof(int), s));
@@ -675,8 +687,26 @@ struct Engine {
     const unsigned grid = std::min<unsigned>(blocks_for(cap, 256), 16384);
     unsigned long long *st = prof ? kstats : nullptr;
     ev_start(GM_KSTAT_SGD);
+    bool fused = false;
+    if (!force_per_epoch) {
+      HIPC(hipMemsetAsync(gbar, 0, 4 * sizeof(unsigned), s));
+      const hipError_t fe = launch_sgd_fused(
+          c, e, head, tail, scal + S_NNZ, (uint32_t)nrows, eps, n_epochs,
+          alpha0, gamma, a, b, nsr_inv, seed, rounding, gbar, st, cap, num_cu,
+          s);
+      if (fe == hipSuccess) {
+        fused = true;
+        fused_used = true;
+      } else {
+        // too many edges for the resident grid, or cooperative launch not
+        // available in this context: one launch per epoch instead
+        (void)hipGetLastError();
+        if (fe != hipErrorInvalidConfiguration)
+          log("cooperative SGD launch unavailable; one launch per epoch");
+      }
+    }
     float alpha = alpha0;
-    for (int epoch = 0; epoch < n_epochs; ++epoch) {
+    for (int epoch = 0; epoch < n_epochs && !fused; ++epoch) {
       const uint64_t es = seed + (uint64_t)epoch;
       switch (c) {
       case 1:
@@ -780,7 +810,11 @@ struct Engine {
         emb, c, dbatch, nb, scal + S_MIN, scal + S_MAX, 0.001f,
         42ull ^ (uint64_t)embedding_sweeps_done);
     LAUNCH_CHECK();
-    run_sgd(emb, n, 2 * nb * (uint64_t)k, (int)p.max_epoch,
+    // the edge count sizes the launch grids (one 4-byte readback)
+    HIPC(hipMemcpyAsync(h_scal + 13, scal + S_NNZ, 4, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    const uint64_t nnz_now = h_scal[13];
+    run_sgd(emb, n, nnz_now, (int)p.max_epoch,
             (int)p.negative_sample_rate, p.a, p.b, p.alpha, 1.0f, 42ull);
 
     // reposition of unselected rows (heuristic_neighbor_avg_kernel :811-894)
@@ -832,7 +866,20 @@ struct Engine {
     r->insertion_completed = insertion_completed;
     r->update_completed = update_completed;
     r->embedding_completed = embedding_completed;
+    check_barrier();
     collect_stats();
+  }
+
+  void check_barrier() {
+    if (!fused_used)
+      return;
+    fused_used = false;
+    unsigned flagv = 0;
+    HIPC(hipMemcpyAsync(h_scal + 12, gbar + 2, 4, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    flagv = h_scal[12];
+    if (flagv)
+      throw Fail(GM_ERR_HIP, "SGD device-wide barrier timed out");
   }
 };
 
@@ -1194,7 +1241,7 @@ int gm_debug_fuzzy(gm_handle *h, const uint32_t *batch, uint64_t nb,
 int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
                  const int32_t *tail, const float *w, int64_t nnz,
                  int32_t n_epochs, int32_t nsr, float a, float b, float alpha0,
-                 float gamma, uint64_t seed) {
+                 float gamma, uint64_t seed, int32_t per_epoch) {
   gm_handle *h = nullptr;
   int rc = guarded([&] {
     if (n == 0 || c < 1 || c > 4 || nnz < 0 || nsr < 1)
@@ -1228,9 +1275,10 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
                         e->s));
     if (mw > 0.0f && nnz > 0)
       e->run_sgd(e->emb, n, (uint64_t)nnz, n_epochs, nsr, a, b, alpha0, gamma,
-                 seed);
+                 seed, per_epoch != 0);
     HIPC(hipMemcpyAsync(emb, e->emb, n * c * 4, hipMemcpyDeviceToHost, e->s));
     HIPC(hipStreamSynchronize(e->s));
+    e->check_barrier();
   });
   gm_destroy(h);
   return rc;

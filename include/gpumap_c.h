@@ -170,11 +170,13 @@ int gm_debug_fuzzy(gm_handle *h, const uint32_t *batch, uint64_t n_batch,
                    double *rho, int64_t *nnz);
 
 /* SGD epochs of the engine on a caller-given embedding (n x c, host) and COO
- * (host), with the engine's kernels. */
+ * (host), with the engine's kernels: all epochs in one cooperative launch
+ * when the edges fit the resident grid, or (per_epoch != 0, or when they do
+ * not fit) one launch per epoch. */
 int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
                  const int32_t *tail, const float *w, int64_t nnz,
                  int32_t n_epochs, int32_t nsr, float a, float b, float alpha0,
-                 float gamma, uint64_t seed);
+                 float gamma, uint64_t seed, int32_t per_epoch);
 
 /* Per-kernel measurement (no counterpart in the reference, which only times
  * whole operations, impl.cuh:159-203).  With profiling on, HIP events on the

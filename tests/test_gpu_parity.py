@@ -161,7 +161,9 @@ def test_sgd_single_edge_matches_closed_form():
     emb = np.array([[0.0, 0.0], [1.5, -2.0]], np.float32)
     out_g = capi.sgd(emb, [0], [1], np.array([1.0], np.float32), 2, 1, 1.577, 0.895)
     out_o = oracle.sgd(emb, [0], [1], np.array([1.0], np.float32), 2, 1, 1.577, 0.895)
-    np.testing.assert_allclose(out_g, out_o, rtol=1e-6, atol=1e-7)
+    # hardware exp2/log2 in the device pow (about 2 ulp) can move the
+    # gradient by one step of the 2^-20 truncation grid
+    np.testing.assert_allclose(out_g, out_o, rtol=1e-5, atol=2e-6)
     assert not np.array_equal(out_g, emb)
 
 
@@ -177,7 +179,40 @@ def test_sgd_disjoint_edges_match_oracle():
     w[0] = 1.0
     out_g = capi.sgd(emb, head, tail, w, 2, 1, 1.577, 0.895)
     out_o = oracle.sgd(emb, head, tail, w, 2, 1, 1.577, 0.895)
-    np.testing.assert_allclose(out_g, out_o, rtol=2e-6, atol=1e-6)
+    np.testing.assert_allclose(out_g, out_o, rtol=1e-5, atol=4e-6)
+
+
+def test_sgd_fused_equals_per_epoch_launches():
+    # conflict-free edges, attraction only: the single cooperative launch and
+    # the one-launch-per-epoch path compute the same epochs
+    g = np.random.default_rng(6)
+    n = 512
+    emb = g.standard_normal((n, 2)).astype(np.float32)
+    h = np.arange(0, n, 2, dtype=np.int32)
+    t = h + 1
+    w = g.uniform(0.5, 1, h.size).astype(np.float32)
+    w[0] = 1
+    fused = capi.sgd(emb, h, t, w, 2, 1, 1.577, 0.895)
+    per = capi.sgd(emb, h, t, w, 2, 1, 1.577, 0.895, per_epoch=True)
+    np.testing.assert_array_equal(fused, per)
+    assert not np.array_equal(fused, emb)
+
+
+def test_sgd_many_epochs_statistics_match_oracle():
+    # a real fuzzy graph, full schedule with negative sampling: Hogwild runs
+    # differ in detail, so compare what the layout preserves
+    x = ev.gaussian10(n=2000, d=16)
+    e, o = _lockstep(x, chunk=500, mode=1, check_every=10)
+    idx, dist = e.knn()
+    batch = np.arange(2000, dtype=np.uint32)
+    h, t, w, _, _ = e.fuzzy(batch)
+    g = np.random.default_rng(8)
+    emb0 = g.uniform(-10, 10, size=(2000, 2)).astype(np.float32)
+    out_g = capi.sgd(emb0, h, t, w, 200, 5, 1.577, 0.895)
+    out_o = oracle.sgd(emb0, h, t, w, 200, 5, 1.577, 0.895)
+    tg = ev.trustworthiness(x, out_g)
+    to = ev.trustworthiness(x, out_o)
+    assert abs(tg - to) <= 0.01, (tg, to)
 
 
 @pytest.mark.parametrize("mode", MODES)
