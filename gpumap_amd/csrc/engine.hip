@@ -129,6 +129,12 @@ struct Engine {
   float *pmax = nullptr;      // per-row pool maximum distance (inf if not full)
   uint64_t *part = nullptr;   // exact insertion: per (row, split) top W
   uint64_t part_cap = 0;
+  // BF16 screening (knn_mode exact_bf16): hi/lo split rows, FP32 norms,
+  // per-batch-row tau
+  uint64_t Db = 0;
+  __bf16 *Xh = nullptr, *Xl = nullptr;
+  float *nrm = nullptr, *tau = nullptr;
+  uint64_t tau_cap = 0;
   uint32_t *bcount = nullptr;
   uint32_t *scal = nullptr;
   unsigned long long *fresh = nullptr;
@@ -218,6 +224,12 @@ struct Engine {
     list = dalloc<uint32_t>(N);
     pmax = dalloc<float>(N);
     HIPC(hipMemsetD32Async((hipDeviceptr_t)pmax, 0x7f800000, N, s));
+    if (p.knn_mode == GM_KNN_EXACT_BF16) {
+      Db = (D + 15) / 16 * 16;
+      Xh = dalloc<__bf16>(N * Db);
+      Xl = dalloc<__bf16>(N * Db);
+      nrm = dalloc<float>(N);
+    }
     bcount = dalloc<uint32_t>(std::max<uint64_t>(2 * nk, N) / 1024 + 2);
     scal = dalloc<uint32_t>(S_COUNT);
     fresh = dalloc<unsigned long long>(1);
@@ -310,6 +322,11 @@ struct Engine {
     if (gbar)
       (void)hipFree(gbar);
     gbar = nullptr;
+    for (void *q : {(void *)Xh, (void *)Xl, (void *)nrm, (void *)tau})
+      if (q)
+        (void)hipFree(q);
+    Xh = Xl = nullptr;
+    nrm = tau = nullptr;
     if (pmax)
       (void)hipFree(pmax);
     if (part)
@@ -542,9 +559,42 @@ struct Engine {
       part_cap = need;
     }
     ev_start(GM_KSTAT_INSERT);
-    k_exact_tiles<<<dim3((unsigned)row_tiles, (unsigned)nsplit), 256, 0, s>>>(
-        X, Dp, pool, pmax, b0, B, n, k, nsplit, part, flag, 0);
-    LAUNCH_CHECK();
+    const dim3 grid((unsigned)row_tiles, (unsigned)nsplit);
+    if (p.knn_mode == GM_KNN_EXACT_BF16) {
+      if (B > tau_cap) {
+        if (tau)
+          (void)hipFree(tau);
+        tau = nullptr;
+        tau_cap = 0;
+        tau = dalloc<float>(B);
+        tau_cap = B;
+      }
+      // |d~ - d| <= coef * (|u|^2 + |j|^2): the hi/lo BF16 split (3.1 *
+      // 2^-14 covers round-to-nearest or truncating conversion), 3D products
+      // summed in FP32 in any order with up to 2 ulp per addition, the two
+      // norms, the d~ roundings and the canonical distance's own error (all
+      // within 11.2 (D + 1) 2^-24), times 1.25
+      const double u24 = std::ldexp(1.0, -24);
+      const float coef = (float)(1.25 * (3.1 * std::ldexp(1.0, -14) +
+                                         11.2 * (double)(D + 1) * u24));
+      const float sfac = (float)(1.0 + 2.2 * (double)D * u24);
+      k_bf16_prep<<<(unsigned)B, 64, 0, s>>>(X, Dp, Db, Xh, Xl, nrm, b0, B);
+      LAUNCH_CHECK();
+      k_bf16_tiles<1><<<grid, 256, 0, s>>>(X, Dp, Xh, Xl, Db, nrm, pool, pmax,
+                                           tau, b0, B, n, k, nsplit, part,
+                                           flag, coef, sfac);
+      LAUNCH_CHECK();
+      k_tau_merge<<<(unsigned)B, 256, 0, s>>>(part, nsplit, tau, B);
+      LAUNCH_CHECK();
+      k_bf16_tiles<2><<<grid, 256, 0, s>>>(X, Dp, Xh, Xl, Db, nrm, pool, pmax,
+                                           tau, b0, B, n, k, nsplit, part,
+                                           flag, coef, sfac);
+      LAUNCH_CHECK();
+    } else {
+      k_exact_tiles<<<grid, 256, 0, s>>>(X, Dp, pool, pmax, b0, B, n, k,
+                                         nsplit, part, flag, 0);
+      LAUNCH_CHECK();
+    }
     k_exact_merge<<<(unsigned)B, 256, 0, s>>>(part, nsplit, pool, b0, B, flag);
     LAUNCH_CHECK();
     ev_stop(1);
@@ -1009,9 +1059,9 @@ int gm_create(const gm_params *params, gm_handle **out) {
       throw Fail(GM_ERR_INVALID, "n_features must be positive");
     if (q.metric != GM_EUCLIDEAN && q.metric != GM_COSINE)
       throw Fail(GM_ERR_INVALID, "metric must be 0 (euclidean) or 1 (cosine)");
-    if (q.knn_mode != GM_KNN_NNDESCENT && q.knn_mode != GM_KNN_EXACT_FP32)
-      throw Fail(GM_ERR_INVALID, "knn_mode must be 0 (nndescent) or 1 "
-                                 "(exact_fp32)");
+    if (q.knn_mode < GM_KNN_NNDESCENT || q.knn_mode > GM_KNN_EXACT_BF16)
+      throw Fail(GM_ERR_INVALID, "knn_mode must be 0 (nndescent), 1 "
+                                 "(exact_fp32) or 2 (exact_bf16)");
     if (q.negative_sample_rate < 1)
       throw Fail(GM_ERR_INVALID, "negative_sample_rate must be >= 1");
     if (q.max_epoch > 1000000)

@@ -60,6 +60,20 @@ __global__ void k_exact_tiles(const float *X, uint64_t dp, uint64_t *pool,
 __global__ void k_exact_merge(const uint64_t *part, int nsplit, uint64_t *pool,
                               uint64_t b0, uint64_t nb, uint8_t *flag);
 
+// mfma_kernels.hip
+__global__ void k_bf16_prep(const float *X, uint64_t dp, uint64_t db,
+                            __bf16 *Xh, __bf16 *Xl, float *nrm, uint64_t b0,
+                            uint64_t nb);
+template <int PASS>
+__global__ void k_bf16_tiles(const float *X, uint64_t dp, const __bf16 *Xh,
+                             const __bf16 *Xl, uint64_t db, const float *nrm,
+                             uint64_t *pool, const float *pmax,
+                             const float *tau, uint64_t b0, uint64_t nb,
+                             uint64_t n, int k, int nsplit, uint64_t *part,
+                             uint8_t *flag, float coef, float sfac);
+__global__ void k_tau_merge(const uint64_t *part, int nsplit, float *tau,
+                            uint64_t nb);
+
 // embed_kernels.hip
 __global__ void k_mean_partial(const uint32_t *pidx, const float *pdist,
                                uint64_t n, int k, int euclid, double *psum,

@@ -72,6 +72,8 @@ def test_params_defaults_and_metric_property():
     assert p.knn_mode == "exact_fp32"
     p.knn_mode = "nndescent"
     assert p.knn_mode == "nndescent"
+    p.knn_mode = "exact_bf16"
+    assert p.knn_mode == "exact_bf16"
     with pytest.raises(ValueError):
         p.knn_mode = "approximate"
     p.metric = "CoSiNe"

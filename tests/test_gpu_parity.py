@@ -63,7 +63,8 @@ def test_philox_device_matches_published_vectors():
         assert tuple(int(v) for v in capi.philox(ctr, key)) == want
 
 
-MODES = [pytest.param(0, id="nndescent"), pytest.param(1, id="exact_fp32")]
+MODES = [pytest.param(0, id="nndescent"), pytest.param(1, id="exact_fp32"),
+         pytest.param(2, id="exact_bf16")]
 
 
 @pytest.mark.parametrize("mode", MODES)
@@ -97,10 +98,11 @@ def test_knn_bit_exact_tiny_batches_and_duplicates(mode):
     _lockstep(x, chunk=7, n_ops=40, max_cycles=200, mode=mode)
 
 
-def test_exact_mode_is_exact_knn():
+@pytest.mark.parametrize("mode", MODES[1:])
+def test_exact_mode_is_exact_knn(mode):
     # the exact pools give the FP64 brute-force neighbours (tie-aware)
     x = ev.gaussian10(n=3000, d=32)
-    e, o = _lockstep(x, chunk=700, mode=1)
+    e, o = _lockstep(x, chunk=700, mode=mode)
     idx, _ = e.knn()
     ei, ed = ev.exact_knn(x, 15)
     assert ev.recall_at_k(idx, x, ei, ed) == 1.0
@@ -182,6 +184,27 @@ def test_sgd_disjoint_edges_match_oracle():
     np.testing.assert_allclose(out_g, out_o, rtol=1e-5, atol=4e-6)
 
 
+def test_bf16_screening_on_image_like_rows():
+    # quantised sparse image-like rows (many exact-zero features, values
+    # k/255 not representable in BF16): the BF16-screened graph equals the
+    # FP32 tiles' and the oracle's
+    x = ev.image_like(n=3000, d=784)
+    n, d = x.shape
+    engines = [capi.Engine(n, d, knn_mode=m) for m in (1, 2)]
+    o = oracle.Oracle(n, d, n_threads=16, knn_mode=1)
+    for s in range(0, n, 500):
+        for e in engines:
+            e.append(x[s:s + 500])
+            e.run(500, 500, 0)
+        o.append(x[s:s + 500])
+        o.run(500, 500, 0)
+    (i1, d1), (i2, d2) = engines[0].knn(), engines[1].knn()
+    i3, d3 = o.knn()
+    assert np.array_equal(i1, i2) and np.array_equal(d1, d2)
+    assert np.array_equal(i1, i3) and np.array_equal(d1, d3)
+    assert np.array_equal(np.sort(engines[1].pool(), 1), np.sort(o.pool(), 1))
+
+
 def test_sgd_fused_equals_per_epoch_launches():
     # conflict-free edges, attraction only: the single cooperative launch and
     # the one-launch-per-epoch path compute the same epochs
@@ -222,7 +245,7 @@ def test_gaussian10_quality_and_exact_graph(mode):
     x = ev.gaussian10()
     n, d = x.shape
     e = capi.Engine(n, d, knn_mode=mode)
-    o = oracle.Oracle(n, d, n_threads=16, knn_mode=mode)
+    o = oracle.Oracle(n, d, n_threads=16, knn_mode=min(mode, 1))
     pos = 0
     while pos < n:
         e.append(x[pos:pos + 500])
