@@ -105,6 +105,21 @@ def roofline_numbers(e, capi, d, c):
             "algo_bytes_per_launch": bytes_ / ss.launches,
             "achieved_GBps": bytes_ / (ss.device_ms * 1e-3) / 1e9,
             "total_ms": ss.device_ms}
+    ins = e.kernel_stats(capi.KSTAT_INSERT)
+    if ins.launches:
+        pairs, survivors, redone = ins.work[0], ins.work[1], ins.work[2]
+        ms = ins.device_ms / ins.launches
+        # algorithmic work of an exact insertion: one multiply-add per
+        # feature for every (batch row, inserted row) pair
+        flops = 2.0 * d * pairs
+        out["insertion"] = {
+            "launches": int(ins.launches), "avg_ms": ms,
+            "pairs": int(pairs), "survivors_reranked": int(survivors),
+            "survivors_per_row": survivors / max(1, e.n),
+            "insertions_redone_fp32": int(redone),
+            "algo_flops_per_launch": flops / ins.launches,
+            "achieved_TFLOPs": flops / (ins.device_ms * 1e-3) / 1e12,
+            "total_ms": ins.device_ms}
     return out
 
 

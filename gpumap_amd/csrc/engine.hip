@@ -132,9 +132,14 @@ struct Engine {
   // BF16 screening (knn_mode exact_bf16): hi/lo split rows, FP32 norms,
   // per-batch-row tau
   uint64_t Db = 0;
-  __bf16 *Xh = nullptr, *Xl = nullptr;
+  __bf16 *Xs = nullptr, *Xb = nullptr; // column and batch operand stores
   float *nrm = nullptr, *tau = nullptr;
-  uint64_t tau_cap = 0;
+  uint32_t *rlist = nullptr; // survivors per batch row: column | flags
+  unsigned *rcnt = nullptr;  // survivors per batch row
+  unsigned *qctl = nullptr;  // [1]: a survivor list overflowed
+  uint64_t xb_cap = 0;
+  static constexpr unsigned RCAP = 2048; // survivors per batch row
+  uint64_t ins_pairs = 0; // (batch row, column) pairs screened or computed
   uint32_t *bcount = nullptr;
   uint32_t *scal = nullptr;
   unsigned long long *fresh = nullptr;
@@ -226,9 +231,13 @@ struct Engine {
     HIPC(hipMemsetD32Async((hipDeviceptr_t)pmax, 0x7f800000, N, s));
     if (p.knn_mode == GM_KNN_EXACT_BF16) {
       Db = (D + 15) / 16 * 16;
-      Xh = dalloc<__bf16>(N * Db);
-      Xl = dalloc<__bf16>(N * Db);
-      nrm = dalloc<float>(N);
+      const uint64_t rows_ = (N + 127) / 128 * 128;
+      Xs = dalloc<__bf16>(rows_ * Db * 2);
+      HIPC(hipMemsetAsync(Xs, 0, rows_ * Db * 2 * sizeof(__bf16), s));
+      nrm = dalloc<float>(rows_);
+      HIPC(hipMemsetAsync(nrm, 0, rows_ * sizeof(float), s));
+      qctl = dalloc<unsigned>(4);
+      HIPC(hipMemsetAsync(qctl, 0, 4 * sizeof(unsigned), s));
     }
     bcount = dalloc<uint32_t>(std::max<uint64_t>(2 * nk, N) / 1024 + 2);
     scal = dalloc<uint32_t>(S_COUNT);
@@ -322,11 +331,14 @@ struct Engine {
     if (gbar)
       (void)hipFree(gbar);
     gbar = nullptr;
-    for (void *q : {(void *)Xh, (void *)Xl, (void *)nrm, (void *)tau})
+    for (void *q : {(void *)Xs, (void *)Xb, (void *)nrm, (void *)tau,
+                    (void *)rlist, (void *)rcnt, (void *)qctl})
       if (q)
         (void)hipFree(q);
-    Xh = Xl = nullptr;
+    Xs = Xb = nullptr;
     nrm = tau = nullptr;
+    rlist = nullptr;
+    rcnt = qctl = nullptr;
     if (pmax)
       (void)hipFree(pmax);
     if (part)
@@ -397,6 +409,9 @@ struct Engine {
     totals[GM_KSTAT_SGD].work[0] = c[3];
     totals[GM_KSTAT_SGD].work[1] = c[4];
     totals[GM_KSTAT_SGD].work[2] = sgd_edge_epochs;
+    totals[GM_KSTAT_INSERT].work[0] = ins_pairs;
+    totals[GM_KSTAT_INSERT].work[1] = c[5];
+    totals[GM_KSTAT_INSERT].work[2] = c[6];
   }
 
   void log(const char *msg) const {
@@ -541,7 +556,7 @@ struct Engine {
     ev_stop(20);
   }
 
-  // exact insertion by distance tiles (exact_kernels.hip)
+  // exact insertion by distance tiles (exact_kernels.hip, mfma_kernels.hip)
   void insert_exact(uint64_t b0, uint64_t B) {
     const uint64_t row_tiles = (B + 63) / 64;
     const uint64_t col_tiles = (n + 63) / 64;
@@ -549,7 +564,14 @@ struct Engine {
     while (nsplit < 128 && row_tiles * (uint64_t)nsplit < 1024 &&
            (uint64_t)nsplit * 2 <= col_tiles)
       nsplit *= 2;
-    const uint64_t need = B * (uint64_t)nsplit * W;
+    // BF16 screening: 128 x 128 block tiles, about two blocks per CU
+    const uint64_t brt = (B + 127) / 128, bct = (n + 127) / 128;
+    int bsplit = 1;
+    if (p.knn_mode == GM_KNN_EXACT_BF16)
+      while (bsplit < 128 && brt * (uint64_t)bsplit * 2 <= 2 * (uint64_t)num_cu &&
+             (uint64_t)bsplit * 2 <= bct)
+        bsplit *= 2;
+    const uint64_t need = B * (uint64_t)std::max(nsplit, bsplit) * W;
     if (need > part_cap) {
       if (part)
         (void)hipFree(part);
@@ -558,46 +580,84 @@ struct Engine {
       part = dalloc<uint64_t>(need);
       part_cap = need;
     }
+    ins_pairs += B * n;
     ev_start(GM_KSTAT_INSERT);
     const dim3 grid((unsigned)row_tiles, (unsigned)nsplit);
+    const unsigned *guard = nullptr;
     if (p.knn_mode == GM_KNN_EXACT_BF16) {
-      if (B > tau_cap) {
-        if (tau)
-          (void)hipFree(tau);
-        tau = nullptr;
-        tau_cap = 0;
-        tau = dalloc<float>(B);
-        tau_cap = B;
-      }
-      // |d~ - d| <= coef * (|u|^2 + |j|^2): the hi/lo BF16 split (3.1 *
-      // 2^-14 covers round-to-nearest or truncating conversion), 3D products
-      // summed in FP32 in any order with up to 2 ulp per addition, the two
-      // norms, the d~ roundings and the canonical distance's own error (all
-      // within 11.2 (D + 1) 2^-24), times 1.25
+      ensure_bf16_batch(B);
+      // Error of d~ = s - 2 P against the exact squared distance d*, with
+      // u = 2^-24, s = fl(n_u + n_j), P the screened dot product:
+      //   norms n_u, n_j: each lane's 8 ceil(Db / 512) terms in sequence,
+      //     then a 6-level tree: gamma_n = (8 ceil(Db / 512) + 6) u relative;
+      //   s: u;  d~ (one fma): 2.02 u (|s - 2P| <= 2.02 s);
+      //   hi/lo split with round-to-nearest BF16: 3.01 2^-16 of sum |u_i j_i|
+      //     (<= s / 2) in the dot, so 3.01 2^-16 s in d~;
+      //   summation: each step's 48 exact products summed on the matrix core
+      //     (at most 2u per addition over <= 48 partial sums), steps added in
+      //     round-to-nearest FP32: (96 + nsteps) u of sum |products|;
+      // so |d~ - d*| <= A s.  The canonical distance d (one FMA chain over D
+      // rounded differences) is within (D + 2) u relative of d*.  coef = 1.25
+      // A and cfac = 1.25 (D + 2) u; the factor 1.25 covers second-order
+      // terms and the roundings of the bound arithmetic itself.
       const double u24 = std::ldexp(1.0, -24);
-      const float coef = (float)(1.25 * (3.1 * std::ldexp(1.0, -14) +
-                                         11.2 * (double)(D + 1) * u24));
-      const float sfac = (float)(1.0 + 2.2 * (double)D * u24);
-      k_bf16_prep<<<(unsigned)B, 64, 0, s>>>(X, Dp, Db, Xh, Xl, nrm, b0, B);
+      const double nsteps_d = (double)(Db / 16);
+      const double gnorm = (8.0 * (double)((Db + 511) / 512) + 6.0) * u24;
+      const double A = gnorm + 3.02 * u24 + 3.01 * std::ldexp(1.0, -16) +
+                       1.01 * (96.0 + nsteps_d) * u24;
+      const float coef = (float)(1.25 * A);
+      const float cfac = (float)(1.25 * (double)(D + 2) * u24);
+      const int nsteps = (int)(Db / 16);
+      const unsigned blocks = (unsigned)(brt * bsplit);
+      float *part_f = reinterpret_cast<float *>(part);
+      k_bf16_prep<<<(unsigned)B, 64, 0, s>>>(X, Dp, Db, Xs, Xb, nrm, flag,
+                                             rcnt, qctl, b0, B);
       LAUNCH_CHECK();
-      k_bf16_tiles<1><<<grid, 256, 0, s>>>(X, Dp, Xh, Xl, Db, nrm, pool, pmax,
-                                           tau, b0, B, n, k, nsplit, part,
-                                           flag, coef, sfac);
+      k_bf16_screen<1><<<blocks, 256, 0, s>>>(
+          Xs, Xb, nsteps, nrm, pmax, tau, b0, B, n, bsplit, (int)brt, part_f,
+          rlist, rcnt, RCAP, qctl, coef, cfac);
       LAUNCH_CHECK();
-      k_tau_merge<<<(unsigned)B, 256, 0, s>>>(part, nsplit, tau, B);
+      k_tau_select<<<(unsigned)B, 256, 0, s>>>(part_f, bsplit, tau, B);
       LAUNCH_CHECK();
-      k_bf16_tiles<2><<<grid, 256, 0, s>>>(X, Dp, Xh, Xl, Db, nrm, pool, pmax,
-                                           tau, b0, B, n, k, nsplit, part,
-                                           flag, coef, sfac);
+      k_bf16_screen<2><<<blocks, 256, 0, s>>>(
+          Xs, Xb, nsteps, nrm, pmax, tau, b0, B, n, bsplit, (int)brt, part_f,
+          rlist, rcnt, RCAP, qctl, coef, cfac);
       LAUNCH_CHECK();
-    } else {
-      k_exact_tiles<<<grid, 256, 0, s>>>(X, Dp, pool, pmax, b0, B, n, k,
-                                         nsplit, part, flag, 0);
+      k_bf16_rerank<<<(unsigned)B, 256, RCAP * sizeof(uint64_t), s>>>(
+          X, Dp, rlist, rcnt, RCAP, qctl, pool, pmax, flag, b0, k,
+          prof ? kstats : nullptr);
       LAUNCH_CHECK();
+      // the FP32 tiles below run only if the survivor queue overflowed
+      guard = qctl;
     }
-    k_exact_merge<<<(unsigned)B, 256, 0, s>>>(part, nsplit, pool, b0, B, flag);
+    k_exact_tiles<<<grid, 256, 0, s>>>(X, Dp, pool, pmax, b0, B, n, k, nsplit,
+                                       part, flag, 0, guard);
+    LAUNCH_CHECK();
+    k_exact_merge<<<(unsigned)B, 256, 0, s>>>(part, nsplit, pool, b0, B, flag,
+                                              guard);
     LAUNCH_CHECK();
     ev_stop(1);
+  }
+
+  // batch operand store, tau and survivor queue of the BF16 screening
+  void ensure_bf16_batch(uint64_t B) {
+    if (B <= xb_cap)
+      return;
+    for (void *q : {(void *)Xb, (void *)tau, (void *)rlist, (void *)rcnt})
+      if (q)
+        (void)hipFree(q);
+    Xb = nullptr;
+    tau = nullptr;
+    rlist = nullptr;
+    rcnt = nullptr;
+    xb_cap = 0;
+    const uint64_t rows_ = (B + 127) / 128 * 128;
+    Xb = dalloc<__bf16>(rows_ * Db * 2);
+    HIPC(hipMemsetAsync(Xb, 0, rows_ * Db * 2 * sizeof(__bf16), s));
+    tau = dalloc<float>(B);
+    rlist = dalloc<uint32_t>(B * (uint64_t)RCAP);
+    rcnt = dalloc<unsigned>(B);
+    xb_cap = B;
   }
 
   // ------------------------------------------------------------- update
@@ -1062,6 +1122,8 @@ int gm_create(const gm_params *params, gm_handle **out) {
     if (q.knn_mode < GM_KNN_NNDESCENT || q.knn_mode > GM_KNN_EXACT_BF16)
       throw Fail(GM_ERR_INVALID, "knn_mode must be 0 (nndescent), 1 "
                                  "(exact_fp32) or 2 (exact_bf16)");
+    if (q.knn_mode == GM_KNN_EXACT_BF16 && q.n_instances >= (1ull << 30))
+      throw Fail(GM_ERR_INVALID, "exact_bf16 supports n_instances < 2^30");
     if (q.negative_sample_rate < 1)
       throw Fail(GM_ERR_INVALID, "negative_sample_rate must be >= 1");
     if (q.max_epoch > 1000000)
@@ -1206,6 +1268,7 @@ int gm_reset_kernel_stats(gm_handle *h) {
     for (auto &t : e->totals)
       t = gm_kernel_stats{};
     e->sgd_edge_epochs = 0;
+    e->ins_pairs = 0;
     HIPC(hipMemsetAsync(e->kstats, 0, 8 * sizeof(unsigned long long), e->s));
     HIPC(hipStreamSynchronize(e->s));
   });

@@ -52,7 +52,10 @@ __global__ __launch_bounds__(256) void k_exact_tiles(
     const float *__restrict__ X, uint64_t dp, uint64_t *__restrict__ pool,
     const float *__restrict__ pmax, uint64_t b0, uint64_t nb, uint64_t n,
     int k, int nsplit, uint64_t *__restrict__ part, uint8_t *__restrict__ flag,
-    int new_flag) {
+    int new_flag, const unsigned *__restrict__ guard) {
+  // guard: the BF16 screening's queue state; run only when it overflowed
+  if (guard && guard[1] == 0)
+    return;
   __shared__ float s_vec[128 * 33];
   __shared__ uint64_t s_buf[ER * EB];
   __shared__ uint64_t s_th[ER];
@@ -198,10 +201,11 @@ __global__ __launch_bounds__(256) void k_exact_tiles(
 // Top W over all splits for one batch row -> its pool.
 __global__ __launch_bounds__(256) void k_exact_merge(
     const uint64_t *__restrict__ part, int nsplit, uint64_t *__restrict__ pool,
-    uint64_t b0, uint64_t nb, uint8_t *__restrict__ flag) {
+    uint64_t b0, uint64_t nb, uint8_t *__restrict__ flag,
+    const unsigned *__restrict__ guard) {
   __shared__ uint64_t s[4096];
   const uint64_t bi = blockIdx.x;
-  if (bi >= nb)
+  if (bi >= nb || (guard && guard[1] == 0))
     return;
   const int len = nsplit * W; // power of two, <= 4096
   for (int e = threadIdx.x; e < len; e += blockDim.x)

@@ -56,23 +56,31 @@ __global__ void k_init_embedding(float *emb, int c, const uint32_t *pidx,
 __global__ void k_exact_tiles(const float *X, uint64_t dp, uint64_t *pool,
                               const float *pmax, uint64_t b0, uint64_t nb,
                               uint64_t n, int k, int nsplit, uint64_t *part,
-                              uint8_t *flag, int new_flag);
+                              uint8_t *flag, int new_flag,
+                              const unsigned *guard);
 __global__ void k_exact_merge(const uint64_t *part, int nsplit, uint64_t *pool,
-                              uint64_t b0, uint64_t nb, uint8_t *flag);
+                              uint64_t b0, uint64_t nb, uint8_t *flag,
+                              const unsigned *guard);
 
 // mfma_kernels.hip
 __global__ void k_bf16_prep(const float *X, uint64_t dp, uint64_t db,
-                            __bf16 *Xh, __bf16 *Xl, float *nrm, uint64_t b0,
+                            __bf16 *Xs, __bf16 *Xb, float *nrm, uint8_t *flag,
+                            unsigned *rcnt, unsigned *qctl, uint64_t b0,
                             uint64_t nb);
 template <int PASS>
-__global__ void k_bf16_tiles(const float *X, uint64_t dp, const __bf16 *Xh,
-                             const __bf16 *Xl, uint64_t db, const float *nrm,
-                             uint64_t *pool, const float *pmax,
-                             const float *tau, uint64_t b0, uint64_t nb,
-                             uint64_t n, int k, int nsplit, uint64_t *part,
-                             uint8_t *flag, float coef, float sfac);
-__global__ void k_tau_merge(const uint64_t *part, int nsplit, float *tau,
-                            uint64_t nb);
+__global__ void k_bf16_screen(const __bf16 *Xs, const __bf16 *Xb, int nsteps,
+                              const float *nrm, const float *pmax,
+                              const float *tau, uint64_t b0, uint64_t nb,
+                              uint64_t n, int nsplit, int rtiles, float *part,
+                              uint32_t *rlist, unsigned *rcnt, unsigned rcap,
+                              unsigned *qctl, float coef, float cfac);
+__global__ void k_tau_select(const float *part, int nsplit, float *tau,
+                             uint64_t nb);
+__global__ void k_bf16_rerank(const float *X, uint64_t dp,
+                              const uint32_t *rlist, const unsigned *rcnt,
+                              unsigned rcap, const unsigned *qctl,
+                              uint64_t *pool, const float *pmax, uint8_t *flag,
+                              uint64_t b0, int k, unsigned long long *stats);
 
 // embed_kernels.hip
 __global__ void k_mean_partial(const uint32_t *pidx, const float *pdist,

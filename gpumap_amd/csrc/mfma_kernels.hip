@@ -2,28 +2,39 @@
 //
 // Same result as the FP32 tiles of exact_kernels.hip -- every inserted row
 // gets the exact 32 nearest rows by the canonical FP32 distance, every earlier
-// row is offered every new row -- with the distance work moved to the matrix
-// cores:
+// row is offered every new row that beats its pool maximum -- with the
+// distance work moved to the matrix cores:
 //   * each row x is stored as two BF16 vectors hi = bf16(x), lo = bf16(x - hi)
-//     (x - hi - lo is below 2^-16 |x|) plus its FP32 squared norm;
-//   * a 64 x 64 tile of approximate squared distances
-//       d~ = |u|^2 + |j|^2 - 2 (hi_u.hi_j + hi_u.lo_j + lo_u.hi_j)
+//     (x - hi - lo is below 2^-16 |x|) plus its FP32 squared norm, in an
+//     MFMA-ready layout (below);
+//   * a 32 x 32 block of approximate squared distances
+//       d~ = |u|^2 + |j|^2 - 2 (hi_j.hi_u + hi_j.lo_u + lo_j.hi_u)
 //     takes three v_mfma_f32_32x32x16_bf16 per 16 features (FP32
 //     accumulation);
-//   * |d~ - d| <= bound(u, j) = coef * (|u|^2 + |j|^2) where d is the
-//     canonical distance and coef covers the BF16 split, any FP32 summation
-//     order of the 3D products, the norm and d~ roundings and the canonical
-//     distance's own rounding (derived on the host, engine.hip);
-//   * pass 1 keeps, per batch row and column split, the 32 smallest upper
-//     bounds d~ + bound; their 32nd smallest over all splits, tau, is at least
-//     the row's 32nd smallest exact distance;
-//   * pass 2 recomputes the tiles and re-ranks in exact FP32 every pair whose
-//     lower bound d~ - bound is <= tau (forward candidates) or <= the earlier
-//     row's pool maximum (reverse candidates).  No pair that can enter a pool
-//     is screened out, so the pools equal the FP32 path's bit for bit.
-// Rows are read straight from HBM/L2 into the MFMA operand registers (16 B
-// per lane per 16 features); the FP32 rows are read only for the re-ranked
-// pairs.
+//   * d~ - coef (|u|^2 + |j|^2) <= d / (1 - cfac) and d / (1 + cfac) <=
+//     d~ + coef (|u|^2 + |j|^2), where d is the canonical distance: coef
+//     covers the BF16 split, the summation of each feature step's 48 products
+//     on the matrix core and the round-to-nearest adds across steps, the norm
+//     and d~ roundings; cfac covers the canonical distance's own rounding
+//     (derived on the host, engine.hip);
+//   * pass 1 keeps, per lane and batch row, the 16 smallest upper bounds
+//     d~ + bound it sees; the 32nd smallest of all of them, tau, is an upper
+//     bound of the row's 32nd smallest exact distance (k_tau_select);
+//   * pass 2 recomputes the tiles and lists, per batch row, every column
+//     whose lower bound d~ - bound is <= tau (forward) or <= the column row's
+//     pool maximum (reverse); k_bf16_rerank computes the listed pairs'
+//     canonical distances, writes each batch row's 32 smallest forward keys to
+//     its pool and offers the reverse ones with the lock-free bounded insert,
+//     whose result does not depend on the order.  No pair that can enter a
+//     pool is screened out, so the pools equal the FP32 path's bit for bit;
+//   * a full list sets an overflow flag; the FP32 tiles then redo the
+//     insertion (their kernels take the flag as a guard).
+//
+// Operand layout.  Rows are grouped by 32; for group g and feature step s (16
+// features) the 32 x 16 hi block and the 32 x 16 lo block are stored as
+// 2 x 1 KB in MFMA lane order -- lane l holds row l % 32, features
+// (l / 32) * 8 .. +7 -- so one 16-byte load per lane (1 KB per wave, fully
+// coalesced) fetches a whole operand of v_mfma_f32_32x32x16_bf16.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -34,283 +45,455 @@ namespace {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
 
-template <typename T>
-__device__ void block_bitonic_m(T *a, int len, int seg) {
-  for (int size = 2; size <= seg; size <<= 1)
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int t = threadIdx.x; t < len / 2; t += blockDim.x) {
-        const int i = 2 * stride * (t / stride) + (t % stride);
-        const int j = i + stride;
-        const bool up = (size == seg) || ((i & size) == 0);
-        const T x = a[i], y = a[j];
-        if ((x > y) == up) {
-          a[i] = y;
-          a[j] = x;
-        }
-      }
-      __syncthreads();
-    }
+constexpr int TB = 128;  // batch rows per block tile
+constexpr int TC = 128;  // columns (earlier rows) per block tile
+constexpr int TOPL = 16; // upper bounds kept per lane and batch row
+constexpr uint32_t LFWD = 0x80000000u; // survivor list entry: forward pair
+constexpr uint32_t LREV = 0x40000000u; // survivor list entry: reverse pair
+constexpr uint32_t LID = 0x3fffffffu;  // survivor list entry: column row
+constexpr int RC = 32;                 // features per re-rank chunk
+constexpr int RS = RC + 1;             // padded LDS row of a chunk
+
+// element offset of (row, feature) in the MFMA-ready layout
+__device__ inline uint64_t swz(uint64_t row, uint64_t f, uint64_t nsteps,
+                               int part) {
+  return (((row >> 5) * nsteps + (f >> 4)) * 2 + part) * 512 +
+         ((((f & 15) >> 3) << 5) + (row & 31)) * 8 + (f & 7);
 }
 
-constexpr int MR = 64;    // batch rows per block
-constexpr int MC = 64;    // columns per tile
-constexpr int MB = 64;    // per-row key buffer: W kept + 32 appended per phase
-constexpr int MS = 4096;  // survivor queue (a full tile)
-constexpr int MQ = 1024;  // reverse offers per drain round
-
-// per-row top-W buffers: append a key if below the row's threshold
-__device__ inline void row_append(uint64_t *buf, int *cnt, const uint64_t *th,
-                                  int *need, int r, uint64_t key) {
-  if (key < th[r]) {
-    const int slot = atomicAdd(&cnt[r], 1);
-    buf[r * MB + slot] = key;
-    if (slot + 1 > W)
-      *need = 1;
-  }
-}
-
-// sort every row's buffer and keep the W smallest (all threads)
-__device__ inline void row_compact(uint64_t *buf, int *cnt, uint64_t *th) {
-  for (int e = threadIdx.x; e < MR * MB; e += blockDim.x)
-    if ((e % MB) >= cnt[e / MB])
-      buf[e] = EMPTY;
-  __syncthreads();
-  block_bitonic_m(buf, MR * MB, MB);
-  if (threadIdx.x < MR && cnt[threadIdx.x] >= W) {
-    cnt[threadIdx.x] = W;
-    th[threadIdx.x] = buf[threadIdx.x * MB + W - 1];
-  }
-  __syncthreads();
-}
-
-// after a round of row_append (all threads): compact the rows if any buffer
-// went past W.  The flag is read by every thread before anyone can set it
-// again.
-__device__ inline void settle_rows(uint64_t *buf, int *cnt, uint64_t *th,
-                                   int *need) {
-  __syncthreads();
-  const int nd = *need;
-  __syncthreads();
-  if (nd) {
-    row_compact(buf, cnt, th);
-    if (threadIdx.x == 0)
-      *need = 0;
-    __syncthreads();
-  }
+__device__ inline bf16x8 ld8(const __bf16 *p) {
+  return *reinterpret_cast<const bf16x8 *>(p);
 }
 
 } // namespace
 
-// hi/lo BF16 split and FP32 squared norm of the batch rows.  One 64-thread
-// block per row; the norm is one sequential FMA chain.
+// hi/lo BF16 split of the batch rows into the column operand store Xs (global
+// row index) and the batch operand store Xb (batch-local index), their FP32
+// squared norms, a set flag and an empty survivor list per batch row, and a
+// cleared overflow flag.  One 64-thread block per batch row.
 __global__ __launch_bounds__(64) void k_bf16_prep(
     const float *__restrict__ X, uint64_t dp, uint64_t db,
-    __bf16 *__restrict__ Xh, __bf16 *__restrict__ Xl, float *__restrict__ nrm,
-    uint64_t b0, uint64_t nb) {
-  if (blockIdx.x >= nb)
+    __bf16 *__restrict__ Xs, __bf16 *__restrict__ Xb, float *__restrict__ nrm,
+    uint8_t *__restrict__ flag, unsigned *__restrict__ rcnt,
+    unsigned *__restrict__ qctl, uint64_t b0, uint64_t nb) {
+  const uint64_t r = blockIdx.x;
+  if (r >= nb)
     return;
-  const uint64_t r = b0 + blockIdx.x;
-  const float *x = X + r * dp;
-  for (uint64_t i = threadIdx.x; i < db; i += blockDim.x) {
-    const float v = i < dp ? x[i] : 0.0f;
-    const __bf16 h = (__bf16)v;
-    Xh[r * db + i] = h;
-    Xl[r * db + i] = (__bf16)(v - (float)h);
+  const uint64_t u = b0 + r;
+  const float *x = X + u * dp;
+  const uint64_t nsteps = db / 16;
+  const int lane = threadIdx.x;
+  float s = 0.0f;
+  for (uint64_t t = lane; t < db / 8; t += 64) {
+    const uint64_t f0 = t * 8;
+    bf16x8 hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const float v = f0 + e < dp ? x[f0 + e] : 0.0f;
+      s = __builtin_fmaf(v, v, s);
+      const __bf16 h = (__bf16)v;
+      hi[e] = h;
+      lo[e] = (__bf16)(v - (float)h);
+    }
+    *reinterpret_cast<bf16x8 *>(Xs + swz(u, f0, nsteps, 0)) = hi;
+    *reinterpret_cast<bf16x8 *>(Xs + swz(u, f0, nsteps, 1)) = lo;
+    *reinterpret_cast<bf16x8 *>(Xb + swz(r, f0, nsteps, 0)) = hi;
+    *reinterpret_cast<bf16x8 *>(Xb + swz(r, f0, nsteps, 1)) = lo;
   }
-  if (threadIdx.x == 0) {
-    float s = 0.0f;
-    for (uint64_t i = 0; i < dp; ++i)
-      s = __builtin_fmaf(x[i], x[i], s);
-    nrm[r] = s;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    s += __shfl_xor(s, off, 64);
+  if (lane == 0) {
+    nrm[u] = s;
+    flag[u] = 1;
+    rcnt[r] = 0;
+  }
+  if (r == 0 && lane == 1)
+    qctl[1] = 0;
+}
+
+// One pass of the screening tiles.  Grid: rtiles * nsplit blocks of 256
+// threads; block (rt, split) covers batch rows [128 rt, 128 rt + 128) and the
+// column tiles split, split + nsplit, ...  Blocks of one split are placed on
+// one XCD so that their column reads share its L2.  Wave w computes the
+// 64 x 64 sub-tile (columns (w & 1) * 64, batch rows (w >> 1) * 64) as 2 x 2
+// MFMA blocks; C register i of lane l is column (i / 4) * 8 + (l / 32) * 4 +
+// i % 4 and batch row l % 32 of its block.
+template <int PASS>
+__global__ __launch_bounds__(256) void k_bf16_screen(
+    const __bf16 *__restrict__ Xs, const __bf16 *__restrict__ Xb, int nsteps,
+    const float *__restrict__ nrm, const float *__restrict__ pmax,
+    const float *__restrict__ tau, uint64_t b0, uint64_t nb, uint64_t n,
+    int nsplit, int rtiles, float *__restrict__ part,
+    uint32_t *__restrict__ rlist, unsigned *__restrict__ rcnt, unsigned rcap,
+    unsigned *__restrict__ qctl, float coef, float cfac) {
+  __shared__ float s_nj[TC], s_pm[TC];
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wa = wave & 1, wb = wave >> 1;
+  int rt, split;
+  {
+    const int b = blockIdx.x;
+    if ((nsplit & 7) == 0) {
+      const int q = b >> 3;
+      split = (q / rtiles) * 8 + (b & 7);
+      rt = q % rtiles;
+    } else {
+      rt = b % rtiles;
+      split = b / rtiles;
+    }
+  }
+  const uint64_t r0 = (uint64_t)rt * TB;
+  const uint64_t ntile = (n + TC - 1) / TC;
+
+  // this lane's two batch rows
+  float nu[2], tr[2];
+  bool rok[2];
+#pragma unroll
+  for (int sb = 0; sb < 2; ++sb) {
+    const uint64_t rl = r0 + wb * 64 + sb * 32 + (lane & 31);
+    rok[sb] = rl < nb;
+    nu[sb] = rok[sb] ? nrm[b0 + rl] : 0.0f;
+    tr[sb] = (PASS == 2 && rok[sb]) ? tau[rl] : 0.0f;
+  }
+  float top[2][TOPL];
+#pragma unroll
+  for (int sb = 0; sb < 2; ++sb)
+#pragma unroll
+    for (int q = 0; q < TOPL; ++q)
+      top[sb][q] = INFINITY;
+
+  const uint64_t stepw = 1024; // elements per (group, step): hi + lo blocks
+  const __bf16 *pb0 =
+      Xb + ((r0 / 32 + wb * 2 + 0) * (uint64_t)nsteps) * stepw + lane * 8;
+  const __bf16 *pb1 =
+      Xb + ((r0 / 32 + wb * 2 + 1) * (uint64_t)nsteps) * stepw + lane * 8;
+
+  for (uint64_t t = split; t < ntile; t += nsplit) {
+    const uint64_t c0 = t * TC;
+    __syncthreads(); // previous tile's readers of s_nj / s_pm are done
+    if (tid < TC) {
+      const uint64_t j = c0 + tid;
+      s_nj[tid] = j < n ? nrm[j] : 0.0f;
+      s_pm[tid] = (PASS == 2 && j < b0) ? pmax[j] : -INFINITY;
+    }
+    const __bf16 *pa0 =
+        Xs + ((c0 / 32 + wa * 2 + 0) * (uint64_t)nsteps) * stepw + lane * 8;
+    const __bf16 *pa1 =
+        Xs + ((c0 / 32 + wa * 2 + 1) * (uint64_t)nsteps) * stepw + lane * 8;
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int sa = 0; sa < 2; ++sa)
+#pragma unroll
+      for (int sb = 0; sb < 2; ++sb)
+        acc[sa][sb] = floatx16{};
+    bf16x8 ah0 = ld8(pa0), al0 = ld8(pa0 + 512), ah1 = ld8(pa1),
+           al1 = ld8(pa1 + 512), bh0 = ld8(pb0), bl0 = ld8(pb0 + 512),
+           bh1 = ld8(pb1), bl1 = ld8(pb1 + 512);
+    for (int st = 0; st < nsteps; ++st) {
+      // prefetch the next step while this one runs on the matrix cores
+      const uint64_t o = (uint64_t)(st + 1 < nsteps ? st + 1 : st) * stepw;
+      bf16x8 nah0 = ld8(pa0 + o), nal0 = ld8(pa0 + o + 512),
+             nah1 = ld8(pa1 + o), nal1 = ld8(pa1 + o + 512),
+             nbh0 = ld8(pb0 + o), nbl0 = ld8(pb0 + o + 512),
+             nbh1 = ld8(pb1 + o), nbl1 = ld8(pb1 + o + 512);
+      __builtin_amdgcn_sched_barrier(0); // issue the loads first
+      // this step's 48 products per pair in a fresh accumulator, then one
+      // round-to-nearest FP32 add per step into the running dot product
+      // (bounds the summation error by the step structure, not by D)
+      const floatx16 z = {};
+      {
+        floatx16 ta = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bh0, z, 0, 0, 0);
+        floatx16 tb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bh1, z, 0, 0, 0);
+        ta = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bl0, ta, 0, 0, 0);
+        tb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bl1, tb, 0, 0, 0);
+        ta = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0, bh0, ta, 0, 0, 0);
+        tb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0, bh1, tb, 0, 0, 0);
+        acc[0][0] += ta;
+        acc[0][1] += tb;
+      }
+      {
+        floatx16 ta = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bh0, z, 0, 0, 0);
+        floatx16 tb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bh1, z, 0, 0, 0);
+        ta = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bl0, ta, 0, 0, 0);
+        tb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bl1, tb, 0, 0, 0);
+        ta = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1, bh0, ta, 0, 0, 0);
+        tb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1, bh1, tb, 0, 0, 0);
+        acc[1][0] += ta;
+        acc[1][1] += tb;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      // keep the prefetched operands as values of this iteration (without
+      // this the loads are folded into the loop header and no longer
+      // overlap the matrix work)
+      asm volatile("" : "+v"(nah0), "+v"(nal0), "+v"(nah1), "+v"(nal1));
+      asm volatile("" : "+v"(nbh0), "+v"(nbl0), "+v"(nbh1), "+v"(nbl1));
+      ah0 = nah0;
+      al0 = nal0;
+      ah1 = nah1;
+      al1 = nal1;
+      bh0 = nbh0;
+      bl0 = nbl0;
+      bh1 = nbh1;
+      bl1 = nbl1;
+    }
+    __syncthreads(); // s_nj / s_pm written
+
+#pragma unroll
+    for (int sa = 0; sa < 2; ++sa)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int jl = wa * 64 + sa * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+        const uint64_t j = c0 + jl;
+        const float nj = s_nj[jl];
+        const float pm = s_pm[jl];
+#pragma unroll
+        for (int sb = 0; sb < 2; ++sb) {
+          const uint64_t rl = r0 + wb * 64 + sb * 32 + (lane & 31);
+          const bool ok = rok[sb] && j < n && j != b0 + rl;
+          const float s = nu[sb] + nj;
+          const float dt = __builtin_fmaf(-2.0f, acc[sa][sb][i], s);
+          const float bnd = __builtin_fmaf(coef, s, 1e-30f);
+          if (PASS == 1) {
+            const float ub = fmaxf((dt + bnd) * (1.0f + cfac), 0.0f);
+            if (ok && ub < top[sb][TOPL - 1]) {
+#pragma unroll
+              for (int q = TOPL - 1; q > 0; --q)
+                top[sb][q] = ub < top[sb][q - 1] ? top[sb][q - 1]
+                                                 : fminf(top[sb][q], ub);
+              top[sb][0] = fminf(top[sb][0], ub);
+            }
+          } else {
+            const float lb = (dt - bnd) * (1.0f - cfac);
+            const bool fwd = lb <= tr[sb];
+            const bool rev = lb <= pm; // pm = -inf unless j < b0
+            if (ok && (fwd || rev)) {
+              const unsigned at = atomicAdd(&rcnt[rl], 1u);
+              if (at < rcap)
+                rlist[rl * rcap + at] = (uint32_t)j | (fwd ? LFWD : 0u) |
+                                        (rev ? LREV : 0u);
+              else
+                atomicOr(&qctl[1], 1u);
+            }
+          }
+        }
+      }
+  }
+
+  if (PASS == 1) {
+    // 64 values per (batch row, split): 4 lanes x 16, each list sorted
+#pragma unroll
+    for (int sb = 0; sb < 2; ++sb) {
+      const uint64_t rl = r0 + wb * 64 + sb * 32 + (lane & 31);
+      if (!rok[sb])
+        continue;
+      float *dst = part + (rl * (uint64_t)nsplit + split) * 64 +
+                   (wa * 2 + (lane >> 5)) * TOPL;
+#pragma unroll
+      for (int q = 0; q < TOPL; ++q)
+        dst[q] = top[sb][q];
+    }
   }
 }
 
-template <int PASS>
-__global__ __launch_bounds__(256) void k_bf16_tiles(
-    const float *__restrict__ X, uint64_t dp, const __bf16 *__restrict__ Xh,
-    const __bf16 *__restrict__ Xl, uint64_t db, const float *__restrict__ nrm,
-    uint64_t *__restrict__ pool, const float *__restrict__ pmax,
-    const float *__restrict__ tau, uint64_t b0, uint64_t nb, uint64_t n, int k,
-    int nsplit, uint64_t *__restrict__ part, uint8_t *__restrict__ flag,
-    float coef, float sfac) {
-  __shared__ uint64_t s_buf[MR * MB];
-  __shared__ uint64_t s_th[MR];
-  __shared__ int s_cnt[MR];
-  __shared__ float s_nu[MR], s_tau[MR], s_nj[MC], s_pm[MC];
-  __shared__ uint32_t s_sq[MS];
-  __shared__ uint64_t s_qkey[MQ];
-  __shared__ uint8_t s_qtgt[MQ];
-  __shared__ int s_sn, s_qn, s_need;
+template __global__ void k_bf16_screen<1>(
+    const __bf16 *, const __bf16 *, int, const float *, const float *,
+    const float *, uint64_t, uint64_t, uint64_t, int, int, float *,
+    uint32_t *, unsigned *, unsigned, unsigned *, float, float);
+template __global__ void k_bf16_screen<2>(
+    const __bf16 *, const __bf16 *, int, const float *, const float *,
+    const float *, uint64_t, uint64_t, uint64_t, int, int, float *,
+    uint32_t *, unsigned *, unsigned, unsigned *, float, float);
 
-  const int tid = threadIdx.x;
-  const int wave = tid >> 6, lane = tid & 63;
-  const int wm = wave & 1, wn = wave >> 1;
-  const int hw = tid >> 5, hl = tid & 31, half = (tid >> 5) & 1;
-  const uint64_t r0 = (uint64_t)blockIdx.x * MR;
-  const int rows = (int)min((uint64_t)MR, nb - r0);
-  const int split = blockIdx.y;
-  const uint64_t ntile = (n + MC - 1) / MC;
-
-  if (tid < MR) {
-    s_cnt[tid] = 0;
-    s_th[tid] = EMPTY;
-    s_nu[tid] = tid < rows ? nrm[b0 + r0 + tid] : 0.0f;
-    s_tau[tid] = (PASS == 2 && tid < rows) ? tau[r0 + tid] : 0.0f;
+// tau of one batch row: the 32nd smallest of its nsplit * 64 upper bounds
+// (+inf when it has fewer than 32 other rows).  Radix select on the float
+// bits (all values are >= 0), 8 bits per round.
+__global__ __launch_bounds__(256) void k_tau_select(
+    const float *__restrict__ part, int nsplit, float *__restrict__ tau,
+    uint64_t nb) {
+  __shared__ unsigned hist[256];
+  __shared__ unsigned s_prefix, s_rank;
+  const uint64_t r = blockIdx.x;
+  if (r >= nb)
+    return;
+  const int len = nsplit * 64;
+  const unsigned *v =
+      reinterpret_cast<const unsigned *>(part) + r * (uint64_t)len;
+  if (threadIdx.x == 0) {
+    s_prefix = 0;
+    s_rank = W; // 1-based rank still to find among values with the prefix
   }
-  // this lane's MFMA operand rows
-  const int arow = wm * 32 + (lane & 31);
-  const bool a_ok = arow < rows;
-  const uint64_t ga = b0 + r0 + (a_ok ? arow : 0);
-  const __bf16 *pah = Xh + ga * db + (lane >> 5) * 8;
-  const __bf16 *pal = Xl + ga * db + (lane >> 5) * 8;
-  const bf16x8 zero8 = {};
-  __syncthreads();
-
-  for (uint64_t t = split; t < ntile; t += nsplit) {
-    const uint64_t c0 = t * MC;
-    const int cols = (int)min((uint64_t)MC, n - c0);
-    if (tid < MC) {
-      const uint64_t j = c0 + tid;
-      s_nj[tid] = tid < cols ? nrm[j] : 0.0f;
-      s_pm[tid] = (PASS == 2 && tid < cols && j < b0) ? pmax[j] : -1.0f;
-    }
-    if (tid == 0) {
-      s_sn = 0;
-      s_need = 0;
-    }
-    // ---- approximate distances of the wave's 32 x 32 sub-tile
-    const int bcol = wn * 32 + (lane & 31);
-    const bool b_ok = bcol < cols;
-    const uint64_t gb = c0 + (b_ok ? bcol : 0);
-    const __bf16 *pbh = Xh + gb * db + (lane >> 5) * 8;
-    const __bf16 *pbl = Xl + gb * db + (lane >> 5) * 8;
-    floatx16 acc = {};
-    for (uint64_t k0 = 0; k0 < db; k0 += 16) {
-      const bf16x8 ah = a_ok ? *reinterpret_cast<const bf16x8 *>(pah + k0) : zero8;
-      const bf16x8 al = a_ok ? *reinterpret_cast<const bf16x8 *>(pal + k0) : zero8;
-      const bf16x8 bh = b_ok ? *reinterpret_cast<const bf16x8 *>(pbh + k0) : zero8;
-      const bf16x8 bl = b_ok ? *reinterpret_cast<const bf16x8 *>(pbl + k0) : zero8;
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+  unsigned mask = 0;
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    const unsigned prefix = s_prefix;
+    for (int e = threadIdx.x; e < len; e += blockDim.x) {
+      const unsigned x = v[e];
+      if ((x & mask) == prefix)
+        atomicAdd(&hist[(x >> shift) & 255u], 1u);
     }
     __syncthreads();
-
-    const int lc = wn * 32 + (lane & 31);
-    const uint64_t j = c0 + lc;
-    if (PASS == 1) {
-      // two phases (one per column half) keep each row's appends <= 32
-      for (int ph = 0; ph < 2; ++ph) {
-        if (wn == ph) {
+    if (threadIdx.x < 64) {
+      // wave 0: inclusive scan of the 256 bins, 4 per lane
+      const int l = threadIdx.x;
+      const unsigned hh[4] = {hist[4 * l], hist[4 * l + 1], hist[4 * l + 2],
+                              hist[4 * l + 3]};
+      const unsigned mine = hh[0] + hh[1] + hh[2] + hh[3];
+      unsigned incl = mine;
 #pragma unroll
-          for (int i = 0; i < 16; ++i) {
-            const int lr = wm * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
-            const uint64_t u = b0 + r0 + lr;
-            if (lr >= rows || lc >= cols || j == u)
-              continue;
-            const float s = s_nu[lr] + s_nj[lc];
-            const float dt = __builtin_fmaf(-2.0f, acc[i], s);
-            const float ub = fmaxf(dt + coef * (s * sfac) + 1e-30f, 0.0f);
-            row_append(s_buf, s_cnt, s_th, &s_need, lr,
-                       ((uint64_t)__float_as_uint(ub) << 32) | (uint64_t)j);
-          }
-        }
-        settle_rows(s_buf, s_cnt, s_th, &s_need);
+      for (int off = 1; off < 64; off <<= 1) {
+        const unsigned o = __shfl_up(incl, off, 64);
+        if (l >= off)
+          incl += o;
       }
-    } else {
-      // ---- survivors: lower bound within the row's tau (forward) or the
-      // earlier row's pool maximum (reverse)
+      const unsigned excl = incl - mine;
+      const unsigned rank = s_rank;
+      if (excl < rank && rank <= incl) {
+        unsigned c = excl, bin = 4 * l + 3;
+        for (int q = 0; q < 4; ++q) {
+          if (c + hh[q] >= rank) {
+            bin = 4 * l + q;
+            break;
+          }
+          c += hh[q];
+        }
+        s_prefix = prefix | (bin << shift);
+        s_rank = rank - c;
+      }
+    }
+    mask |= 255u << shift;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    tau[r] = __uint_as_float(s_prefix);
+}
+
+// Canonical distances of one batch row's listed pairs, its forward pool and
+// its reverse offers.  Block r (256 threads) takes batch row u = b0 + r; each
+// wave takes 64 listed columns at a time and walks the features in chunks of
+// 32: the 64 column rows' chunk is staged in LDS with coalesced 128-byte row
+// reads, then every lane continues its own pair's FMA chain over the chunk
+// in feature order (the canonical distance).  Forward keys collect in LDS;
+// the row's 32 smallest become its pool (the row is new, so no other block
+// writes it).  Reverse pairs within the column row's pool maximum are offered
+// two at a time with half-wave inserts.  Does nothing when a list overflowed.
+__global__ __launch_bounds__(256) void k_bf16_rerank(
+    const float *__restrict__ X, uint64_t dp,
+    const uint32_t *__restrict__ rlist, const unsigned *__restrict__ rcnt,
+    unsigned rcap, const unsigned *__restrict__ qctl,
+    uint64_t *__restrict__ pool, const float *__restrict__ pmax,
+    uint8_t *__restrict__ flag, uint64_t b0, int k,
+    unsigned long long *__restrict__ stats) {
+  extern __shared__ uint64_t s_dyn[];
+  __shared__ float slab[4][65 * RS];
+  __shared__ int s_nf;
+  uint64_t *fkeys = s_dyn; // rcap forward keys
+  const uint64_t r = blockIdx.x;
+  if (qctl[1]) {
+    if (stats && r == 0 && threadIdx.x == 0)
+      atomicAdd(&stats[6], 1ull);
+    return;
+  }
+  const unsigned cnt = min(rcnt[r], rcap);
+  if (stats && threadIdx.x == 0)
+    atomicAdd(&stats[5], (unsigned long long)cnt);
+  const uint64_t u = b0 + r;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int hl = threadIdx.x & 31, half = (threadIdx.x >> 5) & 1;
+  float *sl = slab[wave];
+  if (threadIdx.x == 0)
+    s_nf = 0;
+  __syncthreads();
+  const uint32_t *lst = rlist + r * (uint64_t)rcap;
+
+  for (unsigned b = wave * 64; b < cnt; b += 256) {
+    const bool valid = b + lane < cnt;
+    const uint32_t ent = valid ? lst[b + lane] : 0u;
+    const uint64_t j = ent & LID;
+    float acc = 0.0f;
+    for (uint64_t f0 = 0; f0 < dp; f0 += RC) {
+      const int fw = (int)min((uint64_t)RC, dp - f0); // multiple of 4
+      // rows lane / 8 + 8 q, float4 lane % 8: 8 lanes read one 128-byte row
+      const int c4 = (lane & 7) * 4;
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int lr = wm * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
-        const uint64_t u = b0 + r0 + lr;
-        if (lr >= rows || lc >= cols || j == u)
-          continue;
-        const float s = s_nu[lr] + s_nj[lc];
-        const float dt = __builtin_fmaf(-2.0f, acc[i], s);
-        const float lb = dt - coef * (s * sfac) - 1e-30f;
-        const bool fwd = lb <= s_tau[lr];
-        const bool rev = lb <= s_pm[lc];
-        if (fwd || rev) {
-          const int at = atomicAdd(&s_sn, 1);
-          s_sq[at] = ((uint32_t)lr << 16) | ((uint32_t)lc << 2) |
-                     (fwd ? 1u : 0u) | (rev ? 2u : 0u);
+      for (int q = 0; q < 8; ++q) {
+        const int row = (lane >> 3) + 8 * q;
+        const uint64_t jr = __shfl(j, row, 64);
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (c4 < fw && b + row < cnt)
+          v = *reinterpret_cast<const float4 *>(X + jr * dp + f0 + c4);
+        float *d = sl + row * RS + c4;
+        d[0] = v.x;
+        d[1] = v.y;
+        d[2] = v.z;
+        d[3] = v.w;
+      }
+      if (lane < 8) {
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (c4 < fw)
+          v = *reinterpret_cast<const float4 *>(X + u * dp + f0 + c4);
+        float *d = sl + 64 * RS + c4;
+        d[0] = v.x;
+        d[1] = v.y;
+        d[2] = v.z;
+        d[3] = v.w;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int i = 0; i < fw; ++i) {
+        const float t = sl[64 * RS + i] - sl[lane * RS + i];
+        acc = __builtin_fmaf(t, t, acc);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (valid && (ent & LFWD))
+      fkeys[atomicAdd(&s_nf, 1)] = make_key(acc, (uint32_t)j, false);
+    // reverse offers; the loop condition is uniform over the wave, so the
+    // broadcasts run with every lane active
+    const int nact = (int)min(64u, cnt - b);
+    for (int e2 = 0; 2 * e2 < nact; ++e2) {
+      const int e = 2 * e2 + half;
+      const uint32_t ee = __shfl(ent, e, 64);
+      const float ed = __shfl(acc, e, 64);
+      if (e >= nact || !(ee & LREV))
+        continue;
+      const uint64_t ej = ee & LID;
+      if (ed <= pmax[ej] &&
+          pool_offer_half(pool + ej * W, make_key(ed, (uint32_t)u, false), k,
+                          hl, half) &&
+          hl == 0)
+        flag[ej] = 1;
+    }
+  }
+  __syncthreads();
+  // the batch row's pool: its 32 smallest forward keys, sorted
+  const int nf = s_nf;
+  int len = W;
+  while (len < nf)
+    len <<= 1;
+  for (int e = nf + threadIdx.x; e < len; e += blockDim.x)
+    fkeys[e] = EMPTY;
+  __syncthreads();
+  for (int size = 2; size <= len; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < len / 2; t += blockDim.x) {
+        const int i = 2 * stride * (t / stride) + (t % stride);
+        const int jj = i + stride;
+        const bool up = (i & size) == 0;
+        const uint64_t x = fkeys[i], y = fkeys[jj];
+        if ((x > y) == up) {
+          fkeys[i] = y;
+          fkeys[jj] = x;
         }
       }
       __syncthreads();
-      const int sn = s_sn;
-      // exact re-rank in rounds of MQ survivors: forward keys into the row
-      // buffers (two column phases), reverse offers through the CAS insert
-      for (int base = 0; base < sn; base += MQ) {
-        const int cnt = min(MQ, sn - base);
-        if (tid == 0)
-          s_qn = 0;
-        __syncthreads();
-        for (int ph = 0; ph < 2; ++ph) {
-          for (int q = tid; q < cnt; q += blockDim.x) {
-            const uint32_t e = s_sq[base + q];
-            const int lr = (int)(e >> 16), lcq = (int)((e >> 2) & 0x3fff);
-            if ((lcq >> 5) != ph)
-              continue;
-            const uint64_t u = b0 + r0 + lr, jj = c0 + lcq;
-            const float dc = canon_dist_g(X + u * dp, X + jj * dp, (int)dp);
-            if (e & 1u)
-              row_append(s_buf, s_cnt, s_th, &s_need, lr,
-                         make_key(dc, (uint32_t)jj, false));
-            if ((e & 2u) && dc <= s_pm[lcq]) {
-              const int at = atomicAdd(&s_qn, 1);
-              s_qkey[at] = make_key(dc, (uint32_t)u, false);
-              s_qtgt[at] = (uint8_t)lcq;
-            }
-          }
-          settle_rows(s_buf, s_cnt, s_th, &s_need);
-        }
-        const int qn = s_qn;
-        for (int e = hw; e < qn; e += 8) {
-          const uint64_t jj = c0 + s_qtgt[e];
-          if (pool_offer_half(pool + jj * W, s_qkey[e], k, hl, half) && hl == 0)
-            flag[jj] = 1;
-        }
-        __syncthreads();
-      }
     }
-  }
-
-  // this split's W smallest keys per row, sorted
-  row_compact(s_buf, s_cnt, s_th);
-  for (int e = tid; e < rows * W; e += blockDim.x) {
-    const int r = e / W, s = e % W;
-    part[((r0 + r) * (uint64_t)nsplit + split) * W + s] = s_buf[r * MB + s];
-  }
-}
-
-template __global__ void k_bf16_tiles<1>(
-    const float *, uint64_t, const __bf16 *, const __bf16 *, uint64_t,
-    const float *, uint64_t *, const float *, const float *, uint64_t,
-    uint64_t, uint64_t, int, int, uint64_t *, uint8_t *, float, float);
-template __global__ void k_bf16_tiles<2>(
-    const float *, uint64_t, const __bf16 *, const __bf16 *, uint64_t,
-    const float *, uint64_t *, const float *, const float *, uint64_t,
-    uint64_t, uint64_t, int, int, uint64_t *, uint8_t *, float, float);
-
-// tau of one batch row: the W-th smallest upper bound over all splits
-// (+inf when the row has fewer than W other rows).
-__global__ __launch_bounds__(256) void k_tau_merge(
-    const uint64_t *__restrict__ part, int nsplit, float *__restrict__ tau,
-    uint64_t nb) {
-  __shared__ uint64_t s[4096];
-  const uint64_t bi = blockIdx.x;
-  if (bi >= nb)
-    return;
-  const int len = nsplit * W;
-  for (int e = threadIdx.x; e < len; e += blockDim.x)
-    s[e] = part[bi * (uint64_t)len + e];
-  __syncthreads();
-  block_bitonic_m(s, len, len);
-  if (threadIdx.x == 0) {
-    const uint64_t kw = s[W - 1];
-    tau[bi] = kw == EMPTY ? INFINITY : __uint_as_float((uint32_t)(kw >> 32));
-  }
+  if (threadIdx.x < W)
+    pool[u * W + threadIdx.x] = fkeys[threadIdx.x];
 }
 
 } // namespace gm

@@ -186,7 +186,10 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
  *                    queued pool offers}
  *   GM_KSTAT_SGD     SGD epochs; work = {positive samples, negative samples,
  *                    edge-epochs scanned}
- *   GM_KSTAT_INSERT  insertion rounds (candidates + offers); work unused */
+ *   GM_KSTAT_INSERT  insertions (all kernels of one insertion); work =
+ *                    {(batch row, row) pairs, BF16 screening survivors
+ *                    re-ranked in FP32, insertions redone by the FP32 tiles
+ *                    after a survivor-queue overflow} */
 typedef enum gm_kstat {
   GM_KSTAT_JOIN = 0,
   GM_KSTAT_SGD = 1,

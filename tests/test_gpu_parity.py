@@ -184,6 +184,33 @@ def test_sgd_disjoint_edges_match_oracle():
     np.testing.assert_allclose(out_g, out_o, rtol=1e-5, atol=4e-6)
 
 
+def test_bf16_queue_overflow_redoes_insertion_with_fp32_tiles():
+    # 2 distinct points repeated 3000 times: all pairs within a group are at
+    # distance 0, so every duplicate survives the screening; once a row has
+    # more than 2048 of them its survivor list overflows and the batch is
+    # redone by the FP32 tiles
+    g = np.random.default_rng(9)
+    x = np.repeat(g.normal(size=(2, 24)).astype(np.float32), 3000, axis=0)
+    x = np.ascontiguousarray(x[g.permutation(len(x))])
+    n, d = x.shape
+    e1, e2 = capi.Engine(n, d, knn_mode=1), capi.Engine(n, d, knn_mode=2)
+    e2.set_profiling(True)
+    o = oracle.Oracle(n, d, n_threads=16, knn_mode=1)
+    for s in range(0, n, 500):
+        for e in (e1, e2, o):
+            e.append(x[s:s + 500])
+            e.run(500, 500, 0)
+        i1, d1 = e1.knn()
+        i2, d2 = e2.knn()
+        i3, d3 = o.knn()
+        assert np.array_equal(i2, i1) and np.array_equal(d2, d1), s
+        assert np.array_equal(i2, i3) and np.array_equal(d2, d3), s
+        assert np.array_equal(np.sort(e2.pool(), 1), np.sort(o.pool(), 1)), s
+    st = e2.kernel_stats(capi.KSTAT_INSERT)
+    assert st.work[2] >= 1, "no insertion overflowed a survivor list"
+    assert st.work[2] < 12, "every insertion overflowed"
+
+
 def test_bf16_screening_on_image_like_rows():
     # quantised sparse image-like rows (many exact-zero features, values
     # k/255 not representable in BF16): the BF16-screened graph equals the
