@@ -9,6 +9,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 
@@ -390,9 +391,10 @@ __device__ inline float fclip(float v) {
 }
 
 // One positive sample of edge e and its negative samples (reference
-// optimize_batch_kernel :509-589) for any n_components <= 4.  The negative
-// samples' positions are gathered together before their sequential updates
-// of the head (the values a sequential read would see in a Hogwild run).
+// optimize_batch_kernel :509-589) for any n_components <= 4.  The first 8
+// negative samples' positions are gathered together with the edge's two
+// endpoints (one memory round trip), before the sequential updates of the
+// head (the values a sequential read would see in a Hogwild run).
 template <int NC>
 __device__ inline void sgd_edge(float *__restrict__ emb, int32_t j, int32_t kk,
                                 uint64_t e, uint32_t n, float ep, float *nxt_e,
@@ -403,10 +405,25 @@ __device__ inline void sgd_edge(float *__restrict__ emb, int32_t j, int32_t kk,
                                 unsigned long long &c_neg) {
   const float ns = *nxt_e;
   const float ep_neg = ep * nsr_inv;
+  const float nn = *nneg_e;
+  const int n_neg = (int)(((float)epoch - nn) / ep_neg);
   ++c_pos;
   float *cur_p = emb + (uint64_t)j * NC;
   float *oth_p = emb + (uint64_t)kk * NC;
   float cur[NC], oth[NC], g[NC];
+  Rng rng(seed, (uint32_t)e, (uint32_t)(e >> 32), TAG_NEG);
+  uint32_t r[8];
+  float neg[8][NC];
+  int cnt = min(8, n_neg);
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (q < cnt) {
+      r[q] = rng.next() % n;
+      const float *np_ = emb + (uint64_t)r[q] * NC;
+#pragma unroll
+      for (int d = 0; d < NC; ++d)
+        neg[q][d] = np_[d];
+    }
 #pragma unroll
   for (int d = 0; d < NC; ++d) {
     cur[d] = cur_p[d];
@@ -432,23 +449,8 @@ __device__ inline void sgd_edge(float *__restrict__ emb, int32_t j, int32_t kk,
   for (int d = 0; d < NC; ++d)
     unsafeAtomicAdd(oth_p + d, (rounding + -g[d]) - rounding);
   *nxt_e = ns + ep;
-  const float nn = *nneg_e;
-  const int n_neg = (int)(((float)epoch - nn) / ep_neg);
   if (n_neg > 0) {
-    Rng rng(seed, (uint32_t)e, (uint32_t)(e >> 32), TAG_NEG);
-    for (int p0 = 0; p0 < n_neg; p0 += 8) {
-      const int cnt = min(8, n_neg - p0);
-      uint32_t r[8];
-      float neg[8][NC];
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (q < cnt) {
-          r[q] = rng.next() % n;
-          const float *np_ = emb + (uint64_t)r[q] * NC;
-#pragma unroll
-          for (int d = 0; d < NC; ++d)
-            neg[q][d] = np_[d];
-        }
+    for (int p0 = 0;;) {
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         if (q >= cnt || (int32_t)r[q] == j)
@@ -473,6 +475,20 @@ __device__ inline void sgd_edge(float *__restrict__ emb, int32_t j, int32_t kk,
           g[d] += gg;
         }
       }
+      p0 += 8;
+      if (p0 >= n_neg)
+        break;
+      // more than 8 negative samples (an edge skipped many epochs)
+      cnt = min(8, n_neg - p0);
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q < cnt) {
+          r[q] = rng.next() % n;
+          const float *np_ = emb + (uint64_t)r[q] * NC;
+#pragma unroll
+          for (int d = 0; d < NC; ++d)
+            neg[q][d] = np_[d];
+        }
     }
     *nneg_e = nn + (float)n_neg * ep_neg;
   }
@@ -520,35 +536,50 @@ __global__ __launch_bounds__(256) void k_sgd_epoch(
   flush_sgd_stats(stats, c_pos, c_neg);
 }
 
-// Device-wide barrier of the fused SGD: one monotonically increasing arrival
-// counter; barrier number i is passed once the counter reaches (i + 1) *
-// nblocks.  Same-address relaxed atomics are coherent, so no fence is needed
-// for the barrier itself, and the Hogwild updates need no ordering between
-// epochs.  The wait is bounded: a barrier that times out sets bar[2] (the
-// host reports it) and the kernel still runs to completion.
-__device__ inline void grid_barrier(unsigned *bar, unsigned nblocks,
-                                    unsigned index) {
-  __syncthreads();
+// Epoch boundary of the fused SGD kernels: one monotonically increasing
+// arrival counter, incremented once per block per epoch.  A block finishing
+// epoch e may start epoch e + 1 once every block has finished epoch e - lag
+// (counter >= (e - lag + 1) * nblocks); lag 0 is a full grid barrier.  The
+// block barriers are plain s_barrier (no release fence) and the arrival is a
+// relaxed atomic: the Hogwild updates need no ordering between epochs, so the
+// epoch's atomic updates stay in flight across the boundary.  Every thread
+// reads the counter at the start of the epoch (`peeked`, one request per
+// wave) and uses it only at the boundary, so the read's latency overlaps the
+// epoch's own memory accesses and the boundary usually waits for nothing.  The wait is
+// bounded: a boundary that times out sets bar[2] (the host reports it) and
+// the kernel still runs to completion.
+__device__ inline unsigned bar_peek(unsigned *bar) {
+  return __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void epoch_end(unsigned *bar, unsigned nblocks, int epoch,
+                                 int lag, unsigned peeked, unsigned &seen) {
+  __builtin_amdgcn_s_barrier();
   if (threadIdx.x == 0) {
-    const unsigned target = (index + 1u) * nblocks;
+    seen = max(seen, peeked);
     __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned spins = 0;
-    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-           target) {
-      __builtin_amdgcn_s_sleep(1);
-      if (++spins > (1u << 22)) {
-        __hip_atomic_store(bar + 2, 1u, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        break;
+    if (epoch >= lag) {
+      const unsigned target = (unsigned)(epoch - lag + 1) * nblocks;
+      unsigned spins = 0;
+      while (seen < target) {
+        seen = bar_peek(bar);
+        if (seen >= target)
+          break;
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > (1u << 22)) {
+          __hip_atomic_store(bar + 2, 1u, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
       }
     }
   }
-  __syncthreads();
+  __builtin_amdgcn_s_barrier();
 }
 
 // All epochs in one cooperative launch: each thread keeps EPT edges' ids and
-// schedule in registers for the whole run, and a device-wide barrier separates
-// the epochs (the reference launches one kernel per epoch, :785-793).
+// schedule in registers for the whole run, and epoch_end separates the epochs
+// (the reference launches one kernel per epoch, :785-793).
 template <int NC, int EPT>
 __global__ __launch_bounds__(256) void k_sgd_fused(
     float *__restrict__ emb, const int32_t *__restrict__ head,
@@ -556,7 +587,7 @@ __global__ __launch_bounds__(256) void k_sgd_fused(
     uint32_t n, const float *__restrict__ eps, int n_epochs, float alpha0,
     float gamma, float a, float b, float nsr_inv, uint64_t seed,
     const float *__restrict__ rounding_p, unsigned *__restrict__ bar,
-    unsigned long long *__restrict__ stats) {
+    unsigned long long *__restrict__ stats, int lag) {
   const uint64_t nnz = *nnz_p;
   const float rounding = *rounding_p;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -580,8 +611,10 @@ __global__ __launch_bounds__(256) void k_sgd_fused(
     }
   }
   unsigned long long c_pos = 0, c_neg = 0;
+  unsigned seen = 0;
   float alpha = alpha0;
   for (int epoch = 0; epoch < n_epochs; ++epoch) {
+    const unsigned peeked = bar_peek(bar);
     const uint64_t es = seed + (uint64_t)epoch;
 #pragma unroll
     for (int q = 0; q < EPT; ++q)
@@ -590,15 +623,59 @@ __global__ __launch_bounds__(256) void k_sgd_fused(
                      &nx[q], &nn[q], alpha, epoch, gamma, a, b, nsr_inv, es,
                      rounding, c_pos, c_neg);
     alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
-    grid_barrier(bar, gridDim.x, (unsigned)epoch);
+    epoch_end(bar, gridDim.x, epoch, lag, peeked, seen);
   }
   flush_sgd_stats(stats, c_pos, c_neg);
 }
 
-// Host launcher of k_sgd_fused, in the kernels' own translation unit: picks
-// the smallest EPT whose co-resident grid holds every edge, launches it
-// cooperatively and returns hipSuccess; hipErrorInvalidConfiguration when no
-// EPT fits (the caller then launches one kernel per epoch).
+// All epochs in one cooperative launch for edge sets larger than the
+// register-resident grid: the schedule stays in memory (eps, nxt, nneg as in
+// k_sgd_epoch) and each thread walks a grid-strided range of edges per epoch.
+template <int NC>
+__global__ __launch_bounds__(256) void k_sgd_stream(
+    float *__restrict__ emb, const int32_t *__restrict__ head,
+    const int32_t *__restrict__ tail, const uint32_t *__restrict__ nnz_p,
+    uint32_t n, const float *__restrict__ eps, float *__restrict__ nxt,
+    float *__restrict__ nneg, int n_epochs, float alpha0, float gamma, float a,
+    float b, float nsr_inv, uint64_t seed, const float *__restrict__ rounding_p,
+    unsigned *__restrict__ bar, unsigned long long *__restrict__ stats,
+    int lag) {
+  const uint64_t nnz = *nnz_p;
+  const float rounding = *rounding_p;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long c_pos = 0, c_neg = 0;
+  unsigned seen = 0;
+  float alpha = alpha0;
+  for (int epoch = 0; epoch < n_epochs; ++epoch) {
+    const unsigned peeked = bar_peek(bar);
+    const uint64_t es = seed + (uint64_t)epoch;
+    for (uint64_t e = t0; e < nnz; e += stride) {
+      if (nxt[e] > (float)epoch)
+        continue;
+      sgd_edge<NC>(emb, head[e], tail[e], e, n, eps[e], nxt + e, nneg + e,
+                   alpha, epoch, gamma, a, b, nsr_inv, es, rounding, c_pos,
+                   c_neg);
+    }
+    alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
+    epoch_end(bar, gridDim.x, epoch, lag, peeked, seen);
+  }
+  flush_sgd_stats(stats, c_pos, c_neg);
+}
+
+// Host launchers of the cooperative SGD kernels, in the kernels' own
+// translation unit: the smallest EPT whose co-resident grid holds every edge,
+// else the streaming kernel on the co-resident grid.  Returns hipSuccess, or
+// the launch error (the caller then launches one kernel per epoch).
+template <typename K> static int blocks_per_cu(K kernel) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void *>(kernel), 256, 0) !=
+      hipSuccess)
+    return 0;
+  return per_cu;
+}
+
 template <int NC, int EPT>
 static hipError_t try_fused(float *emb, const int32_t *head,
                             const int32_t *tail, const uint32_t *nnz_p,
@@ -606,87 +683,100 @@ static hipError_t try_fused(float *emb, const int32_t *head,
                             float alpha0, float gamma, float a, float b,
                             float nsr_inv, uint64_t seed,
                             const float *rounding_p, unsigned *bar,
-                            unsigned long long *stats, uint64_t nnz,
+                            unsigned long long *stats, int lag, uint64_t nnz,
                             int num_cu, hipStream_t s) {
-  static int per_cu = -1;
-  if (per_cu < 0) {
-    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void *>(&k_sgd_fused<NC, EPT>), 256,
-        0);
-    if (e != hipSuccess) {
-      per_cu = -1;
-      return e;
-    }
-  }
+  static int per_cu = 0; // cached once the occupancy query succeeds
+  if (per_cu <= 0)
+    per_cu = blocks_per_cu(&k_sgd_fused<NC, EPT>);
   const uint64_t need = (nnz + 256ull * EPT - 1) / (256ull * EPT);
-  static const bool dbg = std::getenv("GPUMAP_DEBUG_SGD") != nullptr;
-  if (dbg)
-    fprintf(stderr, "[sgd] nc=%d ept=%d per_cu=%d num_cu=%d need=%llu\n", NC,
-            EPT, per_cu, num_cu, (unsigned long long)need);
   if (need == 0 || need > (uint64_t)per_cu * (uint64_t)num_cu)
     return hipErrorInvalidConfiguration;
-  void *args[] = {&emb,   &head, &tail,    &nnz_p, &n,          &eps,
-                  &n_epochs, &alpha0, &gamma, &a,  &b,          &nsr_inv,
-                  &seed,  &rounding_p, &bar, &stats};
+  void *args[] = {&emb,     &head,   &tail,  &nnz_p, &n,       &eps,
+                  &n_epochs, &alpha0, &gamma, &a,     &b,       &nsr_inv,
+                  &seed,    &rounding_p, &bar, &stats, &lag};
   return hipLaunchCooperativeKernel(
       reinterpret_cast<const void *>(&k_sgd_fused<NC, EPT>),
       dim3((unsigned)need), dim3(256), args, 0, s);
 }
 
 template <int NC>
+static hipError_t try_stream(float *emb, const int32_t *head,
+                             const int32_t *tail, const uint32_t *nnz_p,
+                             uint32_t n, const float *eps, float *nxt,
+                             float *nneg, int n_epochs, float alpha0,
+                             float gamma, float a, float b, float nsr_inv,
+                             uint64_t seed, const float *rounding_p,
+                             unsigned *bar, unsigned long long *stats, int lag,
+                             uint64_t nnz, int num_cu, hipStream_t s) {
+  static int per_cu = 0;
+  if (per_cu <= 0)
+    per_cu = blocks_per_cu(&k_sgd_stream<NC>);
+  if (per_cu <= 0 || nnz == 0)
+    return hipErrorInvalidConfiguration;
+  const uint64_t blocks = std::min<uint64_t>((uint64_t)per_cu * num_cu,
+                                             (nnz + 255) / 256);
+  void *args[] = {&emb,   &head,    &tail,    &nnz_p, &n,  &eps,
+                  &nxt,   &nneg,    &n_epochs, &alpha0, &gamma, &a,
+                  &b,     &nsr_inv, &seed,    &rounding_p, &bar, &stats,
+                  &lag};
+  return hipLaunchCooperativeKernel(
+      reinterpret_cast<const void *>(&k_sgd_stream<NC>),
+      dim3((unsigned)blocks), dim3(256), args, 0, s);
+}
+
+template <int NC>
 static hipError_t try_fused_any(float *emb, const int32_t *head,
                                 const int32_t *tail, const uint32_t *nnz_p,
-                                uint32_t n, const float *eps, int n_epochs,
-                                float alpha0, float gamma, float a, float b,
-                                float nsr_inv, uint64_t seed,
-                                const float *rounding_p, unsigned *bar,
-                                unsigned long long *stats, uint64_t nnz,
-                                int num_cu, hipStream_t s) {
-  hipError_t e = try_fused<NC, 1>(emb, head, tail, nnz_p, n, eps, n_epochs,
-                                  alpha0, gamma, a, b, nsr_inv, seed,
-                                  rounding_p, bar, stats, nnz, num_cu, s);
-  if (e != hipErrorInvalidConfiguration)
-    return e;
-  e = try_fused<NC, 2>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0, gamma,
-                       a, b, nsr_inv, seed, rounding_p, bar, stats, nnz,
-                       num_cu, s);
-  if (e != hipErrorInvalidConfiguration)
-    return e;
-  e = try_fused<NC, 4>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0, gamma,
-                       a, b, nsr_inv, seed, rounding_p, bar, stats, nnz,
-                       num_cu, s);
-  if (e != hipErrorInvalidConfiguration)
-    return e;
-  return try_fused<NC, 8>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0,
-                          gamma, a, b, nsr_inv, seed, rounding_p, bar, stats,
-                          nnz, num_cu, s);
+                                uint32_t n, const float *eps, float *nxt,
+                                float *nneg, int n_epochs, float alpha0,
+                                float gamma, float a, float b, float nsr_inv,
+                                uint64_t seed, const float *rounding_p,
+                                unsigned *bar, unsigned long long *stats,
+                                int lag, uint64_t nnz, int num_cu,
+                                hipStream_t s) {
+#define GM_TRY(EPT)                                                            \
+  {                                                                            \
+    const hipError_t e = try_fused<NC, EPT>(                                   \
+        emb, head, tail, nnz_p, n, eps, n_epochs, alpha0, gamma, a, b,         \
+        nsr_inv, seed, rounding_p, bar, stats, lag, nnz, num_cu, s);           \
+    if (e != hipErrorInvalidConfiguration)                                     \
+      return e;                                                                \
+  }
+  GM_TRY(1)
+  GM_TRY(2)
+  GM_TRY(4)
+  GM_TRY(8)
+#undef GM_TRY
+  return try_stream<NC>(emb, head, tail, nnz_p, n, eps, nxt, nneg, n_epochs,
+                        alpha0, gamma, a, b, nsr_inv, seed, rounding_p, bar,
+                        stats, lag, nnz, num_cu, s);
 }
 
 hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
                             const int32_t *tail, const uint32_t *nnz_p,
-                            uint32_t n, const float *eps, int n_epochs,
-                            float alpha0, float gamma, float a, float b,
-                            float nsr_inv, uint64_t seed,
-                            const float *rounding_p, unsigned *bar,
-                            unsigned long long *stats, uint64_t nnz,
-                            int num_cu, hipStream_t s) {
+                            uint32_t n, const float *eps, float *nxt,
+                            float *nneg, int n_epochs, float alpha0,
+                            float gamma, float a, float b, float nsr_inv,
+                            uint64_t seed, const float *rounding_p,
+                            unsigned *bar, unsigned long long *stats, int lag,
+                            uint64_t nnz, int num_cu, hipStream_t s) {
   switch (nc) {
   case 1:
-    return try_fused_any<1>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0,
-                            gamma, a, b, nsr_inv, seed, rounding_p, bar, stats,
-                            nnz, num_cu, s);
+    return try_fused_any<1>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
+                            n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
+                            rounding_p, bar, stats, lag, nnz, num_cu, s);
   case 2:
-    return try_fused_any<2>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0,
-                            gamma, a, b, nsr_inv, seed, rounding_p, bar, stats,
-                            nnz, num_cu, s);
+    return try_fused_any<2>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
+                            n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
+                            rounding_p, bar, stats, lag, nnz, num_cu, s);
   case 3:
-    return try_fused_any<3>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0,
-                            gamma, a, b, nsr_inv, seed, rounding_p, bar, stats,
-                            nnz, num_cu, s);
+    return try_fused_any<3>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
+                            n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
+                            rounding_p, bar, stats, lag, nnz, num_cu, s);
   default:
-    return try_fused_any<4>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0,
-                            gamma, a, b, nsr_inv, seed, rounding_p, bar, stats,
-                            nnz, num_cu, s);
+    return try_fused_any<4>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
+                            n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
+                            rounding_p, bar, stats, lag, nnz, num_cu, s);
   }
 }
 
@@ -694,7 +784,7 @@ hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
   template __global__ void k_sgd_fused<NC, EPT>(                               \
       float *, const int32_t *, const int32_t *, const uint32_t *, uint32_t,   \
       const float *, int, float, float, float, float, float, uint64_t,         \
-      const float *, unsigned *, unsigned long long *);
+      const float *, unsigned *, unsigned long long *, int);
 GM_SGD_FUSED(1, 1)
 GM_SGD_FUSED(1, 2)
 GM_SGD_FUSED(1, 4)

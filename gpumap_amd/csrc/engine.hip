@@ -774,6 +774,13 @@ struct Engine {
     LAUNCH_CHECK();
   }
 
+  // Epoch lag of the cooperative SGD: a block starts epoch e + 1 once every
+  // block has finished epoch e - SGD_LAG (0 would be a full grid barrier per
+  // epoch, the reference's one launch per epoch).  The Hogwild updates are
+  // unordered within an epoch already; the lag lets the boundary's counter
+  // read overlap the next epoch's work.
+  static constexpr int SGD_LAG = 1;
+
   // SGD on the COO in head/tail/wout with nnz in scal[S_NNZ] and the
   // maximum weight in scal[S_MAXW] (reference run_optimize_layout,
   // embedding_kernel.cuh:726-798).  `cap` bounds nnz for the launch grids.
@@ -801,9 +808,9 @@ struct Engine {
     if (!force_per_epoch) {
       HIPC(hipMemsetAsync(gbar, 0, 4 * sizeof(unsigned), s));
       const hipError_t fe = launch_sgd_fused(
-          c, e, head, tail, scal + S_NNZ, (uint32_t)nrows, eps, n_epochs,
-          alpha0, gamma, a, b, nsr_inv, seed, rounding, gbar, st, cap, num_cu,
-          s);
+          c, e, head, tail, scal + S_NNZ, (uint32_t)nrows, eps, nxt, nneg,
+          n_epochs, alpha0, gamma, a, b, nsr_inv, seed, rounding, gbar, st,
+          SGD_LAG, cap, num_cu, s);
       if (fe == hipSuccess) {
         fused = true;
         fused_used = true;

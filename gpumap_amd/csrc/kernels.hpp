@@ -129,17 +129,18 @@ __global__ void k_sgd_fused(float *emb, const int32_t *head,
                             float alpha0, float gamma, float a, float b,
                             float nsr_inv, uint64_t seed,
                             const float *rounding_p, unsigned *bar,
-                            unsigned long long *stats);
-// cooperative launch of k_sgd_fused (embed_kernels.hip); returns
-// hipErrorInvalidConfiguration when the edges do not fit the resident grid
+                            unsigned long long *stats, int lag);
+// cooperative launch of k_sgd_fused or, for edge sets larger than its
+// resident grid, k_sgd_stream (embed_kernels.hip); returns the launch error
+// when neither can be launched
 hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
                             const int32_t *tail, const uint32_t *nnz_p,
-                            uint32_t n, const float *eps, int n_epochs,
-                            float alpha0, float gamma, float a, float b,
-                            float nsr_inv, uint64_t seed,
-                            const float *rounding_p, unsigned *bar,
-                            unsigned long long *stats, uint64_t nnz,
-                            int num_cu, hipStream_t s);
+                            uint32_t n, const float *eps, float *nxt,
+                            float *nneg, int n_epochs, float alpha0,
+                            float gamma, float a, float b, float nsr_inv,
+                            uint64_t seed, const float *rounding_p,
+                            unsigned *bar, unsigned long long *stats, int lag,
+                            uint64_t nnz, int num_cu, hipStream_t s);
 __global__ void k_reposition(const uint32_t *pidx, const float *pdist,
                              uint64_t n, int k, int euclid, const double *sigma,
                              const double *rho, const uint32_t *rows,
