@@ -391,25 +391,27 @@ __device__ inline float fclip(float v) {
 }
 
 // One positive sample of edge e and its negative samples (reference
-// optimize_batch_kernel :509-589) for any n_components <= 4.  The first 8
-// negative samples' positions are gathered together with the edge's two
-// endpoints (one memory round trip), before the sequential updates of the
-// head (the values a sequential read would see in a Hogwild run).
+// optimize_batch_kernel :509-589) for any n_components <= 4.  Returns the
+// edge's contributions to its head (gh) and tail (gt), each truncated to the
+// rounding grid as the reference does before its atomic adds; the caller adds
+// them to the embedding (seg_add).  The first 8 negative samples' positions
+// are gathered together with the edge's two endpoints (one memory round
+// trip), before the sequential updates of the head (the values a sequential
+// read would see in a Hogwild run).  A negative sample's index is the high
+// word of next() * n.
 template <int NC>
-__device__ inline void sgd_edge(float *__restrict__ emb, int32_t j, int32_t kk,
-                                uint64_t e, uint32_t n, float ep, float *nxt_e,
-                                float *nneg_e, float alpha, int epoch,
+__device__ inline void sgd_edge(const float *__restrict__ emb, int32_t j,
+                                int32_t kk, uint64_t e, uint32_t n, float ep,
+                                float &nx, float &nn, float alpha, int epoch,
                                 float gamma, float a, float b, float nsr_inv,
-                                uint64_t seed, float rounding,
-                                unsigned long long &c_pos,
+                                uint64_t seed, float rounding, float gh[NC],
+                                float gt[NC], unsigned long long &c_pos,
                                 unsigned long long &c_neg) {
-  const float ns = *nxt_e;
   const float ep_neg = ep * nsr_inv;
-  const float nn = *nneg_e;
   const int n_neg = (int)(((float)epoch - nn) / ep_neg);
   ++c_pos;
-  float *cur_p = emb + (uint64_t)j * NC;
-  float *oth_p = emb + (uint64_t)kk * NC;
+  const float *cur_p = emb + (uint64_t)j * NC;
+  const float *oth_p = emb + (uint64_t)kk * NC;
   float cur[NC], oth[NC], g[NC];
   Rng rng(seed, (uint32_t)e, (uint32_t)(e >> 32), TAG_NEG);
   uint32_t r[8];
@@ -418,7 +420,7 @@ __device__ inline void sgd_edge(float *__restrict__ emb, int32_t j, int32_t kk,
 #pragma unroll
   for (int q = 0; q < 8; ++q)
     if (q < cnt) {
-      r[q] = rng.next() % n;
+      r[q] = __umulhi(rng.next(), n);
       const float *np_ = emb + (uint64_t)r[q] * NC;
 #pragma unroll
       for (int d = 0; d < NC; ++d)
@@ -444,11 +446,9 @@ __device__ inline void sgd_edge(float *__restrict__ emb, int32_t j, int32_t kk,
   for (int d = 0; d < NC; ++d) {
     g[d] = fclip(attr * (cur[d] - oth[d])) * alpha;
     cur[d] += g[d];
+    gt[d] = (rounding + -g[d]) - rounding;
   }
-#pragma unroll
-  for (int d = 0; d < NC; ++d)
-    unsafeAtomicAdd(oth_p + d, (rounding + -g[d]) - rounding);
-  *nxt_e = ns + ep;
+  nx += ep;
   if (n_neg > 0) {
     for (int p0 = 0;;) {
 #pragma unroll
@@ -483,18 +483,72 @@ __device__ inline void sgd_edge(float *__restrict__ emb, int32_t j, int32_t kk,
 #pragma unroll
       for (int q = 0; q < 8; ++q)
         if (q < cnt) {
-          r[q] = rng.next() % n;
+          r[q] = __umulhi(rng.next(), n);
           const float *np_ = emb + (uint64_t)r[q] * NC;
 #pragma unroll
           for (int d = 0; d < NC; ++d)
             neg[q][d] = np_[d];
         }
     }
-    *nneg_e = nn + (float)n_neg * ep_neg;
+    nn += (float)n_neg * ep_neg;
   }
 #pragma unroll
   for (int d = 0; d < NC; ++d)
-    unsafeAtomicAdd(cur_p + d, (rounding + g[d]) - rounding);
+    gh[d] = (rounding + g[d]) - rounding;
+}
+
+// Runs of equal row ids over the lanes of a wave (edges in COO order, so a
+// row's edges as head, or as tail, are mostly adjacent).  m bit s: lane +
+// 2^s holds the same run; start: first lane of its run.  Lanes without an
+// edge pass a key of their own (< 0).
+struct Seg {
+  uint32_t m;
+  bool start;
+};
+
+__device__ inline Seg seg_of(int32_t key) {
+  const int lane = threadIdx.x & 63;
+  const int32_t prev = __shfl_up(key, 1, 64);
+  const bool start = lane == 0 || prev != key;
+  const uint64_t sm = __ballot(start);
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  const int run = __popcll(sm & upto);
+  uint32_t m = 0;
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    const int ro = __shfl_down(run, 1 << s, 64);
+    if (lane + (1 << s) < 64 && ro == run)
+      m |= 1u << s;
+  }
+  return Seg{m, start};
+}
+
+// Sum each run's contributions into its first lane and add them to the row
+// there: one atomic per run and component instead of one per edge.  The
+// contributions are multiples of the rounding grid and a run holds at most
+// max-degree of them, so their sum is exact in any order.  All lanes of the
+// wave must call this together.
+template <int NC>
+__device__ inline void seg_add(float *__restrict__ emb, int32_t row, Seg sg,
+                               float v[NC]) {
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+#pragma unroll
+    for (int d = 0; d < NC; ++d) {
+      const float o = __shfl_down(v[d], 1 << s, 64);
+      if ((sg.m >> s) & 1u)
+        v[d] += o;
+    }
+  }
+  bool any = false;
+#pragma unroll
+  for (int d = 0; d < NC; ++d)
+    any |= v[d] != 0.0f;
+  if (sg.start && any) {
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      unsafeAtomicAdd(emb + (uint64_t)row * NC + d, v[d]);
+  }
 }
 
 __device__ inline void flush_sgd_stats(unsigned long long *stats,
@@ -512,6 +566,41 @@ __device__ inline void flush_sgd_stats(unsigned long long *stats,
   }
 }
 
+// One edge of a wave-wide step with the schedule in memory (k_sgd_epoch,
+// k_sgd_stream): sample it if due, then add the wave's contributions.
+template <int NC>
+__device__ inline void sgd_step(float *__restrict__ emb,
+                                const int32_t *__restrict__ head,
+                                const int32_t *__restrict__ tail, uint64_t e,
+                                uint64_t nnz, uint32_t n,
+                                const float *__restrict__ eps,
+                                float *__restrict__ nxt,
+                                float *__restrict__ nneg, float alpha,
+                                int epoch, float gamma, float a, float b,
+                                float nsr_inv, uint64_t seed, float rounding,
+                                unsigned long long &c_pos,
+                                unsigned long long &c_neg) {
+  const bool valid = e < nnz;
+  const int32_t hv = valid ? head[e] : -1 - (int32_t)(threadIdx.x & 63);
+  const int32_t tv = valid ? tail[e] : -1 - (int32_t)(threadIdx.x & 63);
+  float gh[NC], gt[NC];
+#pragma unroll
+  for (int d = 0; d < NC; ++d)
+    gh[d] = gt[d] = 0.0f;
+  if (valid) {
+    float nx = nxt[e];
+    if (!(nx > (float)epoch)) {
+      float nn = nneg[e];
+      sgd_edge<NC>(emb, hv, tv, e, n, eps[e], nx, nn, alpha, epoch, gamma, a,
+                   b, nsr_inv, seed, rounding, gh, gt, c_pos, c_neg);
+      nxt[e] = nx;
+      nneg[e] = nn;
+    }
+  }
+  seg_add<NC>(emb, tv, seg_of(tv), gt);
+  seg_add<NC>(emb, hv, seg_of(hv), gh);
+}
+
 // One epoch of the Hogwild edge sampler, one launch per epoch (used when the
 // edge set is too large for the register-resident kernel below).
 template <int NC>
@@ -525,13 +614,13 @@ __global__ __launch_bounds__(256) void k_sgd_epoch(
   const uint64_t nnz = *nnz_p;
   const float rounding = *rounding_p;
   unsigned long long c_pos = 0, c_neg = 0;
-  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
-       e += (uint64_t)gridDim.x * blockDim.x) {
-    if (nxt[e] > (float)epoch)
-      continue;
-    sgd_edge<NC>(emb, head[e], tail[e], e, n, eps[e], nxt + e, nneg + e,
-                 alpha, epoch, gamma, a, b, nsr_inv, seed, rounding, c_pos,
-                 c_neg);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // whole waves iterate together (seg_add is a wave operation)
+  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint64_t base = t0 - (threadIdx.x & 63); base < nnz; base += stride) {
+    const uint64_t e = base + (threadIdx.x & 63);
+    sgd_step<NC>(emb, head, tail, e, nnz, n, eps, nxt, nneg, alpha, epoch,
+                 gamma, a, b, nsr_inv, seed, rounding, c_pos, c_neg);
   }
   flush_sgd_stats(stats, c_pos, c_neg);
 }
@@ -594,6 +683,7 @@ __global__ __launch_bounds__(256) void k_sgd_fused(
   const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int32_t hh[EPT], tt[EPT];
   float ep[EPT], nx[EPT], nn[EPT];
+  Seg sh[EPT], st[EPT];
 #pragma unroll
   for (int q = 0; q < EPT; ++q) {
     const uint64_t e = t0 + (uint64_t)q * stride;
@@ -604,11 +694,13 @@ __global__ __launch_bounds__(256) void k_sgd_fused(
       nx[q] = ep[q];
       nn[q] = ep[q] * nsr_inv;
     } else {
-      hh[q] = tt[q] = 0;
+      hh[q] = tt[q] = -1 - (int32_t)(threadIdx.x & 63);
       ep[q] = -1.0f;
       nx[q] = INFINITY;
       nn[q] = 0.0f;
     }
+    sh[q] = seg_of(hh[q]);
+    st[q] = seg_of(tt[q]);
   }
   unsigned long long c_pos = 0, c_neg = 0;
   unsigned seen = 0;
@@ -617,11 +709,18 @@ __global__ __launch_bounds__(256) void k_sgd_fused(
     const unsigned peeked = bar_peek(bar);
     const uint64_t es = seed + (uint64_t)epoch;
 #pragma unroll
-    for (int q = 0; q < EPT; ++q)
+    for (int q = 0; q < EPT; ++q) {
+      float gh[NC], gt[NC];
+#pragma unroll
+      for (int d = 0; d < NC; ++d)
+        gh[d] = gt[d] = 0.0f;
       if (!(nx[q] > (float)epoch))
         sgd_edge<NC>(emb, hh[q], tt[q], t0 + (uint64_t)q * stride, n, ep[q],
-                     &nx[q], &nn[q], alpha, epoch, gamma, a, b, nsr_inv, es,
-                     rounding, c_pos, c_neg);
+                     nx[q], nn[q], alpha, epoch, gamma, a, b, nsr_inv, es,
+                     rounding, gh, gt, c_pos, c_neg);
+      seg_add<NC>(emb, tt[q], st[q], gt);
+      seg_add<NC>(emb, hh[q], sh[q], gh);
+    }
     alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
     epoch_end(bar, gridDim.x, epoch, lag, peeked, seen);
   }
@@ -650,13 +749,11 @@ __global__ __launch_bounds__(256) void k_sgd_stream(
   for (int epoch = 0; epoch < n_epochs; ++epoch) {
     const unsigned peeked = bar_peek(bar);
     const uint64_t es = seed + (uint64_t)epoch;
-    for (uint64_t e = t0; e < nnz; e += stride) {
-      if (nxt[e] > (float)epoch)
-        continue;
-      sgd_edge<NC>(emb, head[e], tail[e], e, n, eps[e], nxt + e, nneg + e,
-                   alpha, epoch, gamma, a, b, nsr_inv, es, rounding, c_pos,
-                   c_neg);
-    }
+    // whole waves iterate together (seg_add is a wave operation)
+    for (uint64_t base = t0 - (threadIdx.x & 63); base < nnz; base += stride)
+      sgd_step<NC>(emb, head, tail, base + (threadIdx.x & 63), nnz, n, eps,
+                   nxt, nneg, alpha, epoch, gamma, a, b, nsr_inv, es,
+                   rounding, c_pos, c_neg);
     alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
     epoch_end(bar, gridDim.x, epoch, lag, peeked, seen);
   }
@@ -664,8 +761,9 @@ __global__ __launch_bounds__(256) void k_sgd_stream(
 }
 
 // Host launchers of the cooperative SGD kernels, in the kernels' own
-// translation unit: the smallest EPT whose co-resident grid holds every edge,
-// else the streaming kernel on the co-resident grid.  Returns hipSuccess, or
+// translation unit: the register-resident kernel with 1 or 2 edges per
+// thread when its co-resident grid holds every edge, else the streaming
+// kernel on the co-resident grid.  Returns hipSuccess, or
 // the launch error (the caller then launches one kernel per epoch).
 template <typename K> static int blocks_per_cu(K kernel) {
   int per_cu = 0;
@@ -744,8 +842,6 @@ static hipError_t try_fused_any(float *emb, const int32_t *head,
   }
   GM_TRY(1)
   GM_TRY(2)
-  GM_TRY(4)
-  GM_TRY(8)
 #undef GM_TRY
   return try_stream<NC>(emb, head, tail, nnz_p, n, eps, nxt, nneg, n_epochs,
                         alpha0, gamma, a, b, nsr_inv, seed, rounding_p, bar,
@@ -787,20 +883,12 @@ hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
       const float *, unsigned *, unsigned long long *, int);
 GM_SGD_FUSED(1, 1)
 GM_SGD_FUSED(1, 2)
-GM_SGD_FUSED(1, 4)
-GM_SGD_FUSED(1, 8)
 GM_SGD_FUSED(2, 1)
 GM_SGD_FUSED(2, 2)
-GM_SGD_FUSED(2, 4)
-GM_SGD_FUSED(2, 8)
 GM_SGD_FUSED(3, 1)
 GM_SGD_FUSED(3, 2)
-GM_SGD_FUSED(3, 4)
-GM_SGD_FUSED(3, 8)
 GM_SGD_FUSED(4, 1)
 GM_SGD_FUSED(4, 2)
-GM_SGD_FUSED(4, 4)
-GM_SGD_FUSED(4, 8)
 
 template __global__ void k_sgd_epoch<2>(float *, const int32_t *,
                                         const int32_t *, const uint32_t *,

@@ -477,7 +477,9 @@ void sgd_epochs(float *emb, uint64_t n, int c, const int32_t *head,
       if (n_neg > 0) {
         Rng rng(es, (uint32_t)e, (uint32_t)((uint64_t)e >> 32), TAG_NEG);
         for (int p = 0; p < n_neg; ++p) {
-          const uint32_t r = rng.next() % (uint32_t)n;
+          // index in [0, n): high word of the 32 x 32-bit product
+          const uint32_t r =
+              (uint32_t)(((uint64_t)rng.next() * (uint64_t)n) >> 32);
           if ((int32_t)r == j)
             continue;
           const float *neg_p = emb + (size_t)r * c;
