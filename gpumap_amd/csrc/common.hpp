@@ -204,4 +204,35 @@ __device__ inline int pool_offer_half(uint64_t *__restrict__ row,
   }
 }
 
+// The same bounded insert by a single lane: read the W slots (relaxed loads,
+// possibly at different times), reject a duplicate id or a key not below the
+// maximum, else CAS the maximum slot (lowest index among ties) and retry on
+// failure.  Slots only ever decrease, so the values read are >= the row's
+// current ones; a CAS that succeeds found its slot still holding the
+// maximum read, which is then the row's current maximum -- the row ends as
+// the W smallest distinct keys offered, in any order of the offers.
+// Returns true when the key was inserted.
+__device__ inline bool pool_offer_lane(uint64_t *__restrict__ row,
+                                       uint64_t key) {
+  const uint32_t id = key_id(key);
+  for (;;) {
+    uint64_t mx = 0;
+    int ml = 0;
+    bool dup = false;
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+      const uint64_t v = ld_relaxed(row + s);
+      dup |= (v != EMPTY) && key_id(v) == id;
+      if (v > mx) {
+        mx = v;
+        ml = s;
+      }
+    }
+    if (dup || key >= mx)
+      return false;
+    if (cas64(row + ml, mx, key))
+      return true;
+  }
+}
+
 } // namespace gm

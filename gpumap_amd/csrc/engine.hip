@@ -135,6 +135,7 @@ struct Engine {
   __bf16 *Xs = nullptr, *Xb = nullptr; // column and batch operand stores
   float *nrm = nullptr, *tau = nullptr;
   uint32_t *rlist = nullptr; // survivors per batch row: column | flags
+  uint64_t *fkeys = nullptr; // forward keys of the survivors
   unsigned *rcnt = nullptr;  // survivors per batch row
   unsigned *qctl = nullptr;  // [1]: a survivor list overflowed
   uint64_t xb_cap = 0;
@@ -332,13 +333,14 @@ struct Engine {
       (void)hipFree(gbar);
     gbar = nullptr;
     for (void *q : {(void *)Xs, (void *)Xb, (void *)nrm, (void *)tau,
-                    (void *)rlist, (void *)rcnt, (void *)qctl})
+                    (void *)rlist, (void *)rcnt, (void *)qctl, (void *)fkeys})
       if (q)
         (void)hipFree(q);
     Xs = Xb = nullptr;
     nrm = tau = nullptr;
     rlist = nullptr;
     rcnt = qctl = nullptr;
+    fkeys = nullptr;
     if (pmax)
       (void)hipFree(pmax);
     if (part)
@@ -623,9 +625,12 @@ struct Engine {
           Xs, Xb, nsteps, nrm, pmax, tau, b0, B, n, bsplit, (int)brt, part_f,
           rlist, rcnt, RCAP, qctl, coef, cfac);
       LAUNCH_CHECK();
-      k_bf16_rerank<<<(unsigned)B, 256, RCAP * sizeof(uint64_t), s>>>(
-          X, Dp, rlist, rcnt, RCAP, qctl, pool, pmax, flag, b0, k,
+      k_bf16_rerank<<<dim3((unsigned)B, RCAP / 64), 64, 0, s>>>(
+          X, Dp, rlist, rcnt, RCAP, qctl, fkeys, pool, pmax, flag, b0,
           prof ? kstats : nullptr);
+      LAUNCH_CHECK();
+      k_bf16_select<<<(unsigned)B, 256, RCAP * sizeof(uint64_t), s>>>(
+          fkeys, rcnt, RCAP, qctl, pool, b0, prof ? kstats : nullptr);
       LAUNCH_CHECK();
       // the FP32 tiles below run only if the survivor queue overflowed
       guard = qctl;
@@ -643,13 +648,15 @@ struct Engine {
   void ensure_bf16_batch(uint64_t B) {
     if (B <= xb_cap)
       return;
-    for (void *q : {(void *)Xb, (void *)tau, (void *)rlist, (void *)rcnt})
+    for (void *q : {(void *)Xb, (void *)tau, (void *)rlist, (void *)rcnt,
+                    (void *)fkeys})
       if (q)
         (void)hipFree(q);
     Xb = nullptr;
     tau = nullptr;
     rlist = nullptr;
     rcnt = nullptr;
+    fkeys = nullptr;
     xb_cap = 0;
     const uint64_t rows_ = (B + 127) / 128 * 128;
     Xb = dalloc<__bf16>(rows_ * Db * 2);
@@ -657,6 +664,7 @@ struct Engine {
     tau = dalloc<float>(B);
     rlist = dalloc<uint32_t>(B * (uint64_t)RCAP);
     rcnt = dalloc<unsigned>(B);
+    fkeys = dalloc<uint64_t>(B * (uint64_t)RCAP);
     xb_cap = B;
   }
 

@@ -79,8 +79,13 @@ __global__ void k_tau_select(const float *part, int nsplit, float *tau,
 __global__ void k_bf16_rerank(const float *X, uint64_t dp,
                               const uint32_t *rlist, const unsigned *rcnt,
                               unsigned rcap, const unsigned *qctl,
-                              uint64_t *pool, const float *pmax, uint8_t *flag,
-                              uint64_t b0, int k, unsigned long long *stats);
+                              uint64_t *fkeys, uint64_t *pool,
+                              const float *pmax, uint8_t *flag, uint64_t b0,
+                              unsigned long long *stats);
+__global__ void k_bf16_select(const uint64_t *fkeys, const unsigned *rcnt,
+                              unsigned rcap, const unsigned *qctl,
+                              uint64_t *pool, uint64_t b0,
+                              unsigned long long *stats);
 
 // embed_kernels.hip
 __global__ void k_mean_partial(const uint32_t *pidx, const float *pdist,

@@ -54,6 +54,7 @@ constexpr uint32_t LID = 0x3fffffffu;  // survivor list entry: column row
 constexpr int RC = 32;                 // features per re-rank chunk
 constexpr int RS = RC + 1;             // padded LDS row of a chunk
 
+
 // element offset of (row, feature) in the MFMA-ready layout
 __device__ inline uint64_t swz(uint64_t row, uint64_t f, uint64_t nsteps,
                                int part) {
@@ -369,26 +370,123 @@ __global__ __launch_bounds__(256) void k_tau_select(
     tau[r] = __uint_as_float(s_prefix);
 }
 
-// Canonical distances of one batch row's listed pairs, its forward pool and
-// its reverse offers.  Block r (256 threads) takes batch row u = b0 + r; each
-// wave takes 64 listed columns at a time and walks the features in chunks of
-// 32: the 64 column rows' chunk is staged in LDS with coalesced 128-byte row
-// reads, then every lane continues its own pair's FMA chain over the chunk
-// in feature order (the canonical distance).  Forward keys collect in LDS;
-// the row's 32 smallest become its pool (the row is new, so no other block
-// writes it).  Reverse pairs within the column row's pool maximum are offered
-// two at a time with half-wave inserts.  Does nothing when a list overflowed.
-__global__ __launch_bounds__(256) void k_bf16_rerank(
+// Canonical distances of the listed pairs and the reverse offers.  Block
+// (r, g) is one wave and takes entries [64 g, 64 g + 64) of batch row
+// u = b0 + r's list (blocks past the list end return at once), so the
+// work spreads evenly however uneven the lists are.  The wave walks the
+// features in chunks of 32: the 64 column rows' chunk is staged in LDS from
+// coalesced 128-byte row reads (the next chunk's reads are in flight
+// meanwhile), then every lane continues its own pair's FMA chain over the
+// chunk in feature order (the canonical distance; u's features are
+// wave-uniform loads).  Forward keys go to fkeys at the entry's position
+// (EMPTY for a reverse-only entry) for k_bf16_select; every lane offers its
+// reverse pair, when within the column row's pool maximum, with the
+// single-lane lock-free insert.  Does nothing when a list overflowed.
+__global__ __launch_bounds__(64) void k_bf16_rerank(
     const float *__restrict__ X, uint64_t dp,
     const uint32_t *__restrict__ rlist, const unsigned *__restrict__ rcnt,
     unsigned rcap, const unsigned *__restrict__ qctl,
-    uint64_t *__restrict__ pool, const float *__restrict__ pmax,
-    uint8_t *__restrict__ flag, uint64_t b0, int k,
+    uint64_t *__restrict__ fkeys, uint64_t *__restrict__ pool,
+    const float *__restrict__ pmax, uint8_t *__restrict__ flag, uint64_t b0,
     unsigned long long *__restrict__ stats) {
-  extern __shared__ uint64_t s_dyn[];
-  __shared__ float slab[4][65 * RS];
-  __shared__ int s_nf;
-  uint64_t *fkeys = s_dyn; // rcap forward keys
+  __shared__ float sl[64 * RS];
+  const uint64_t r = blockIdx.x;
+  const unsigned b = blockIdx.y * 64u;
+  const unsigned cnt = min(rcnt[r], rcap);
+  if (b >= cnt || qctl[1])
+    return;
+  const int lane = threadIdx.x;
+  if (stats && b == 0 && lane == 0)
+    atomicAdd(&stats[5], (unsigned long long)cnt);
+  const uint64_t u = b0 + r;
+  const float *__restrict__ xu = X + u * dp;
+  const int c4 = (lane & 7) * 4; // this lane's float4 of a staged row chunk
+  const bool valid = b + lane < cnt;
+  const uint32_t ent = valid ? rlist[r * (uint64_t)rcap + b + lane] : 0u;
+  const uint64_t j = ent & LID;
+  // staged rows lane / 8 + 8 q: loads are branch-free (a row past the list
+  // reads row 0 and a float4 past dp re-reads the row's last one; both are
+  // then replaced by zeros)
+  uint32_t jr[8];
+  uint32_t live = 0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int row = (lane >> 3) + 8 * q;
+    const uint32_t jq = (uint32_t)__shfl((uint32_t)j, row, 64);
+    const bool ok = b + row < cnt;
+    jr[q] = ok ? jq : 0u;
+    live |= ok ? 1u << q : 0u;
+  }
+  auto stage = [&](uint64_t f, float4 out[8]) {
+    const bool fin = f + c4 < dp;
+    const uint64_t col = fin ? f + c4 : dp - 4;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const float4 v =
+          *reinterpret_cast<const float4 *>(X + (uint64_t)jr[q] * dp + col);
+      out[q] = (fin && ((live >> q) & 1u)) ? v
+                                           : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  };
+  float4 nv[8];
+  stage(0, nv);
+  float acc = 0.0f;
+  for (uint64_t f0 = 0; f0 < dp; f0 += RC) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      float *d = sl + ((lane >> 3) + 8 * q) * RS + c4;
+      d[0] = nv[q].x;
+      d[1] = nv[q].y;
+      d[2] = nv[q].z;
+      d[3] = nv[q].w;
+    }
+    __syncthreads();
+    // next chunk's rows, in flight during this chunk's chain
+    stage(f0 + RC, nv);
+    __builtin_amdgcn_sched_barrier(0);
+    // u's chunk (wave-uniform); features past dp are zero on both sides and
+    // fma(0, 0, acc) == acc
+    float xv[RC];
+    if (f0 + RC <= dp) {
+#pragma unroll
+      for (int i = 0; i < RC; ++i)
+        xv[i] = xu[f0 + i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < RC; ++i)
+        xv[i] = f0 + i < dp ? xu[f0 + i] : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < RC; ++i) {
+      const float t = xv[i] - sl[lane * RS + i];
+      acc = __builtin_fmaf(t, t, acc);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // keep the prefetched values as this iteration's (without this the
+    // loads are folded into the next iteration and no longer overlap)
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      asm volatile(""
+                   : "+v"(nv[q].x), "+v"(nv[q].y), "+v"(nv[q].z),
+                     "+v"(nv[q].w));
+    __syncthreads();
+  }
+  if (valid)
+    fkeys[r * (uint64_t)rcap + b + lane] =
+        (ent & LFWD) ? make_key(acc, (uint32_t)j, false) : EMPTY;
+  if (valid && (ent & LREV) && acc <= pmax[j] &&
+      pool_offer_lane(pool + j * W, make_key(acc, (uint32_t)u, false)))
+    flag[j] = 1;
+}
+
+// Batch row u = b0 + r's pool: the 32 smallest of its forward keys (bitonic
+// sort in LDS over the list length rounded up to a power of two).
+__global__ __launch_bounds__(256) void k_bf16_select(
+    const uint64_t *__restrict__ fkeys, const unsigned *__restrict__ rcnt,
+    unsigned rcap, const unsigned *__restrict__ qctl,
+    uint64_t *__restrict__ pool, uint64_t b0,
+    unsigned long long *__restrict__ stats) {
+  extern __shared__ uint64_t s_keys[]; // rcap keys
   const uint64_t r = blockIdx.x;
   if (qctl[1]) {
     if (stats && r == 0 && threadIdx.x == 0)
@@ -396,87 +494,12 @@ __global__ __launch_bounds__(256) void k_bf16_rerank(
     return;
   }
   const unsigned cnt = min(rcnt[r], rcap);
-  if (stats && threadIdx.x == 0)
-    atomicAdd(&stats[5], (unsigned long long)cnt);
-  const uint64_t u = b0 + r;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int hl = threadIdx.x & 31, half = (threadIdx.x >> 5) & 1;
-  float *sl = slab[wave];
-  if (threadIdx.x == 0)
-    s_nf = 0;
-  __syncthreads();
-  const uint32_t *lst = rlist + r * (uint64_t)rcap;
-
-  for (unsigned b = wave * 64; b < cnt; b += 256) {
-    const bool valid = b + lane < cnt;
-    const uint32_t ent = valid ? lst[b + lane] : 0u;
-    const uint64_t j = ent & LID;
-    float acc = 0.0f;
-    for (uint64_t f0 = 0; f0 < dp; f0 += RC) {
-      const int fw = (int)min((uint64_t)RC, dp - f0); // multiple of 4
-      // rows lane / 8 + 8 q, float4 lane % 8: 8 lanes read one 128-byte row
-      const int c4 = (lane & 7) * 4;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int row = (lane >> 3) + 8 * q;
-        const uint64_t jr = __shfl(j, row, 64);
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (c4 < fw && b + row < cnt)
-          v = *reinterpret_cast<const float4 *>(X + jr * dp + f0 + c4);
-        float *d = sl + row * RS + c4;
-        d[0] = v.x;
-        d[1] = v.y;
-        d[2] = v.z;
-        d[3] = v.w;
-      }
-      if (lane < 8) {
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (c4 < fw)
-          v = *reinterpret_cast<const float4 *>(X + u * dp + f0 + c4);
-        float *d = sl + 64 * RS + c4;
-        d[0] = v.x;
-        d[1] = v.y;
-        d[2] = v.z;
-        d[3] = v.w;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      for (int i = 0; i < fw; ++i) {
-        const float t = sl[64 * RS + i] - sl[lane * RS + i];
-        acc = __builtin_fmaf(t, t, acc);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-    if (valid && (ent & LFWD))
-      fkeys[atomicAdd(&s_nf, 1)] = make_key(acc, (uint32_t)j, false);
-    // reverse offers; the loop condition is uniform over the wave, so the
-    // broadcasts run with every lane active
-    const int nact = (int)min(64u, cnt - b);
-    for (int e2 = 0; 2 * e2 < nact; ++e2) {
-      const int e = 2 * e2 + half;
-      const uint32_t ee = __shfl(ent, e, 64);
-      const float ed = __shfl(acc, e, 64);
-      if (e >= nact || !(ee & LREV))
-        continue;
-      const uint64_t ej = ee & LID;
-      if (ed <= pmax[ej] &&
-          pool_offer_half(pool + ej * W, make_key(ed, (uint32_t)u, false), k,
-                          hl, half) &&
-          hl == 0)
-        flag[ej] = 1;
-    }
-  }
-  __syncthreads();
-  // the batch row's pool: its 32 smallest forward keys, sorted
-  const int nf = s_nf;
   int len = W;
-  while (len < nf)
+  while ((unsigned)len < cnt)
     len <<= 1;
-  for (int e = nf + threadIdx.x; e < len; e += blockDim.x)
-    fkeys[e] = EMPTY;
+  const uint64_t *src = fkeys + r * (uint64_t)rcap;
+  for (int e = threadIdx.x; e < len; e += blockDim.x)
+    s_keys[e] = (unsigned)e < cnt ? src[e] : EMPTY;
   __syncthreads();
   for (int size = 2; size <= len; size <<= 1)
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
@@ -484,16 +507,16 @@ __global__ __launch_bounds__(256) void k_bf16_rerank(
         const int i = 2 * stride * (t / stride) + (t % stride);
         const int jj = i + stride;
         const bool up = (i & size) == 0;
-        const uint64_t x = fkeys[i], y = fkeys[jj];
+        const uint64_t x = s_keys[i], y = s_keys[jj];
         if ((x > y) == up) {
-          fkeys[i] = y;
-          fkeys[jj] = x;
+          s_keys[i] = y;
+          s_keys[jj] = x;
         }
       }
       __syncthreads();
     }
   if (threadIdx.x < W)
-    pool[u * W + threadIdx.x] = fkeys[threadIdx.x];
+    pool[(b0 + r) * W + threadIdx.x] = s_keys[threadIdx.x];
 }
 
 } // namespace gm
