@@ -4,10 +4,15 @@ A step is one complete progressive session over the workload, from an empty
 graph until (i) every row is inserted, (ii) the update has converged
 (update_completed) and (iii) one full embedding sweep has run after (ii).
 The op plan is the reference's fixed plan: run(500, 500, 500) per 500-row
-chunk, then run(0, N, N) until the update completes.  ms_per_step is the sum
-of the run() wall times of the step (append_data is outside run(), and the
-rows are staged in HBM before the timed region, so no PCIe is timed); value =
-N / that time.
+chunk, then run(0, N, N) until the update completes, then run(0, 0, N).
+ms_per_step is the sum of the run() wall times of the step (append is
+outside run(), and the rows are staged in HBM before the timed region, so no
+PCIe is timed); value = N / that time.
+
+The headline path is the BF16-MFMA screened exact insertion (knn_mode
+exact_bf16); the same run also times the plain FP32 path (exact FP32
+distance tiles, knn_mode exact_fp32) on the same workload and reports the
+ratio.  Both paths produce the same kNN graph bit for bit.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload image_like]
 
@@ -29,8 +34,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
-FP32_PEAK = 157.3e12       # MI355X FP32 vector, flop/s
+BF16_PEAK = 2.5e15         # MI355X dense BF16 MFMA, flop/s
 CHUNK = 500
+METRIC = ("points embedded/s @recall@15≥0.95, BF16-MFMA vs FP32; "
+          "p50/p99 run() latency")
+MODES = {"nndescent": 0, "exact_fp32": 1, "exact_bf16": 2}
 
 WORKLOAD_DESC = {
     "image_like": "70,000x784 float32 image-like 10-cluster mixture, sparse "
@@ -45,7 +53,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def one_session(capi, xdev, n, d, profile=False, knn_mode=1):
+def one_session(capi, xdev, n, d, profile=False, knn_mode=2):
     e = capi.Engine(n, d, knn_mode=knn_mode)
     if profile:
         e.set_profiling(True)
@@ -76,51 +84,88 @@ def one_session(capi, xdev, n, d, profile=False, knn_mode=1):
     return e, r, lat
 
 
-def roofline_numbers(e, capi, d, c):
+def kernel_numbers(e, capi, d, c):
+    """Per-kernel figures from the engine's HIP events (on its own stream)
+    and device work counters."""
     out = {}
-    js = e.kernel_stats(capi.KSTAT_JOIN)
-    if js.launches:
-        members, pairs, offers = js.work[0], js.work[1], js.work[2]
-        # each list member's vector and 32-slot pool are gathered once per
-        # visit; each queued offer reads one 32-slot pool row (8 B / slot)
-        bytes_ = members * (4 * d + 8 * 32) + offers * 8 * 32
-        flops = pairs * 3 * d  # subtract + fused multiply-add per feature
-        ms = js.device_ms / js.launches
-        out["k_join"] = {
-            "launches": int(js.launches), "avg_ms": ms,
-            "algo_bytes_per_launch": bytes_ / js.launches,
-            "achieved_GBps": bytes_ / (js.device_ms * 1e-3) / 1e9,
-            "achieved_TFLOPs": flops / (js.device_ms * 1e-3) / 1e12,
-            "total_ms": js.device_ms}
     ss = e.kernel_stats(capi.KSTAT_SGD)
     if ss.launches:
-        pos_, neg_, scans = ss.work[0], ss.work[1], ss.work[2]
-        # positive sample: head/tail ids 8 B, eps/next/next-neg 12 B read +
-        # 8 B written, both endpoints read 8c and atomically added 8c;
-        # negative sample: one 4c read; every edge's schedule check reads 4 B
-        bytes_ = pos_ * (28 + 16 * c) + neg_ * 4 * c + scans * 4
+        pos_, neg_ = ss.work[0], ss.work[1]
+        # per positive sample: head/tail ids 8 B, both endpoints read 8c and
+        # atomically written 8c, schedule state 16 B, nsr = 5 negative reads
+        # of 4c: 24 + 36c bytes
+        bytes_ = pos_ * (24 + 36 * c)
         ms = ss.device_ms / ss.launches
-        out["k_sgd_epoch"] = {
+        out["k_sgd_fused"] = {
             "launches": int(ss.launches), "avg_ms": ms,
+            "positive_samples": int(pos_), "negative_samples": int(neg_),
             "algo_bytes_per_launch": bytes_ / ss.launches,
             "achieved_GBps": bytes_ / (ss.device_ms * 1e-3) / 1e9,
             "total_ms": ss.device_ms}
+    st = e.kernel_stats(capi.KSTAT_SGD_STREAM)
+    if st.launches:
+        bytes_ = st.work[0] * (24 + 36 * c)
+        out["k_sgd_stream"] = {
+            "launches": int(st.launches), "avg_ms": st.device_ms / st.launches,
+            "positive_samples": int(st.work[0]),
+            "algo_bytes_per_launch": bytes_ / st.launches,
+            "achieved_GBps": bytes_ / (st.device_ms * 1e-3) / 1e9,
+            "total_ms": st.device_ms}
     ins = e.kernel_stats(capi.KSTAT_INSERT)
     if ins.launches:
         pairs, survivors, redone = ins.work[0], ins.work[1], ins.work[2]
-        ms = ins.device_ms / ins.launches
         # algorithmic work of an exact insertion: one multiply-add per
         # feature for every (batch row, inserted row) pair
         flops = 2.0 * d * pairs
         out["insertion"] = {
-            "launches": int(ins.launches), "avg_ms": ms,
+            "launches": int(ins.launches),
+            "avg_ms": ins.device_ms / ins.launches,
             "pairs": int(pairs), "survivors_reranked": int(survivors),
             "survivors_per_row": survivors / max(1, e.n),
             "insertions_redone_fp32": int(redone),
             "algo_flops_per_launch": flops / ins.launches,
             "achieved_TFLOPs": flops / (ins.device_ms * 1e-3) / 1e12,
             "total_ms": ins.device_ms}
+    js = e.kernel_stats(capi.KSTAT_JOIN)
+    if js.launches:
+        members, offers = js.work[0], js.work[2]
+        bytes_ = members * (4 * d + 8 * 32) + offers * 8 * 32
+        out["k_join"] = {
+            "launches": int(js.launches), "avg_ms": js.device_ms / js.launches,
+            "algo_bytes_per_launch": bytes_ / js.launches,
+            "achieved_GBps": bytes_ / (js.device_ms * 1e-3) / 1e9,
+            "total_ms": js.device_ms}
     return out
+
+
+def measure(capi, xdev, n, d, mode, steps, warmup):
+    import torch
+
+    for i in range(warmup):
+        e, r, lat = one_session(capi, xdev, n, d, knn_mode=mode)
+        log(f"[{mode}] warmup {i}: {sum(lat):.3f} s over {len(lat)} cycles")
+        e.close()
+    torch.cuda.synchronize()
+    totals, lats, last = [], [], None
+    for i in range(steps):
+        e, r, lat = one_session(capi, xdev, n, d, profile=True, knn_mode=mode)
+        totals.append(sum(lat))
+        lats.extend(lat)
+        log(f"[{mode}] step {i}: {sum(lat):.3f} s over {len(lat)} cycles")
+        if last is not None:
+            last.close()
+        last = e
+    torch.cuda.synchronize()
+    return last, totals, lats
+
+
+def quality(ev, x, e, n):
+    idx, _ = e.knn()
+    q = np.random.Generator(np.random.PCG64(7)).choice(n, 2000, replace=False)
+    ei, ed = ev.exact_knn(x, 15, queries=q, device="cuda:0")
+    return {"recall_at_15": ev.recall_at_k(idx, x, ei, ed, queries=q),
+            "trustworthiness_at_15": ev.trustworthiness(x, e.embedding(),
+                                                         sample=5000)}
 
 
 def cpu_baseline(x, budget_rows, knn_mode=1):
@@ -128,8 +173,7 @@ def cpu_baseline(x, budget_rows, knn_mode=1):
 
     xs = np.ascontiguousarray(x[:budget_rows])
     n, d = xs.shape
-    threads = os.cpu_count() or 1
-    threads = min(threads, 64)
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     o = oracle.Oracle(n, d, n_threads=threads, knn_mode=min(knn_mode, 1))
     t_run, pos = 0.0, 0
     while True:
@@ -149,8 +193,21 @@ def cpu_baseline(x, budget_rows, knn_mode=1):
             break
     return {"value": n / t_run, "unit": "points/s", "cores": threads,
             "kind": "port",
-            "sample": f"first {n} rows of the same workload, same op plan and "
-                      f"completion rule, OpenMP oracle ({t_run:.1f} s)"}
+            "sample": f"first {n} of the workload's rows, same op plan and "
+                      f"completion rule, exact FP32 insertion, OpenMP C++ "
+                      f"restatement on {threads} threads ({t_run:.1f} s)"}
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters,
+    from the committed profile summary (profiles/traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t.get(kernel, {}).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
 
 
 def main():
@@ -160,11 +217,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="image_like",
                     choices=sorted(WORKLOAD_DESC))
+    ap.add_argument("--knn-mode", default="exact_bf16", choices=sorted(MODES))
+    ap.add_argument("--compare", default="exact_fp32",
+                    help="comma list of knn modes timed for comparison "
+                         "('' for none)")
     ap.add_argument("--cpu-rows", type=int, default=None)
-    ap.add_argument("--knn-mode", default="exact_fp32",
-                    choices=["nndescent", "exact_fp32", "exact_bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-quality", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run one device copy of MIB MiB (a kernel of "
+                         "known traffic that scales rocprofv3 PMC counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the progressive engine runs one "
@@ -183,55 +245,67 @@ def main():
     torch.cuda.synchronize()
     log(f"workload {args.workload} {x.shape}")
 
-    mode = {"nndescent": 0, "exact_fp32": 1, "exact_bf16": 2}[args.knn_mode]
-    for i in range(args.warmup):
-        e, r, lat = one_session(capi, xdev, n, d, knn_mode=mode)
-        log(f"warmup {i}: {sum(lat):.3f} s over {len(lat)} cycles")
-        e.close()
-
-    torch.cuda.synchronize()
-    totals, lats, last = [], [], None
-    for i in range(args.steps):
-        e, r, lat = one_session(capi, xdev, n, d, profile=True, knn_mode=mode)
-        totals.append(sum(lat))
-        lats.extend(lat)
-        log(f"step {i}: {sum(lat):.3f} s over {len(lat)} cycles")
-        if last is not None:
-            last.close()
-        last = e
-    torch.cuda.synchronize()
-
+    mode = MODES[args.knn_mode]
+    last, totals, lats = measure(capi, xdev, n, d, mode, args.steps,
+                                 args.warmup)
     ms_per_step = 1e3 * float(np.mean(totals))
     value = n / (ms_per_step / 1e3)
-    kern = roofline_numbers(last, capi, d, 2)
-    dom_name = max(kern, key=lambda k: kern[k]["total_ms"]) if kern else None
-    roof = None
-    if dom_name:
-        kd = kern[dom_name]
-        achieved = kd["achieved_GBps"]
-        roof = {"kernel": dom_name, "bound": "hbm", "achieved": achieved,
-                "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / (HBM_PEAK / 1e9), "traffic": None}
-
-    quality = {}
-    if not args.no_quality:
-        idx, dist = last.knn()
-        q = np.random.Generator(np.random.PCG64(7)).choice(n, 2000, replace=False)
-        ei, ed = ev.exact_knn(x, 15, queries=q, device="cuda:0")
-        quality["recall_at_15"] = ev.recall_at_k(idx, x, ei, ed, queries=q)
-        quality["trustworthiness_at_15"] = ev.trustworthiness(
-            x, last.embedding(), sample=5000)
-        log(f"quality {quality}")
+    kern = kernel_numbers(last, capi, d, 2)
+    qual = {} if args.no_quality else quality(ev, x, last, n)
+    if qual:
+        log(f"[{args.knn_mode}] quality {qual}")
     last.close()
+
+    # dominant kernel: the most device time over the timed session
+    timed = {k: v for k, v in kern.items() if k.startswith("k_")}
+    roof = None
+    if timed:
+        name = max(timed, key=lambda k: timed[k]["total_ms"])
+        kd = timed[name]
+        roof = {"kernel": name, "bound": "hbm",
+                "achieved": kd["achieved_GBps"], "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": kd["achieved_GBps"] / (HBM_PEAK / 1e9),
+                "traffic": load_traffic(name),
+                "algo_bytes_per_launch": kd["algo_bytes_per_launch"],
+                "avg_launch_ms": kd["avg_ms"]}
+    if "insertion" in kern and mode == MODES["exact_bf16"]:
+        kern["insertion"]["mfma_frac_algorithmic"] = (
+            kern["insertion"]["achieved_TFLOPs"] * 1e12 / BF16_PEAK)
+
+    if args.copy_probe:
+        src = torch.empty(args.copy_probe << 18, dtype=torch.float32,
+                          device="cuda:0")
+        src.fill_(1.0)
+        dst = src.clone()
+        torch.cuda.synchronize()
+        del src, dst
+
+    compare = {}
+    for m in [c for c in args.compare.split(",") if c and c != args.knn_mode]:
+        e2, tot2, lat2 = measure(capi, xdev, n, d, MODES[m], args.steps,
+                                 min(args.warmup, 1))
+        ms2 = 1e3 * float(np.mean(tot2))
+        entry = {"value": n / (ms2 / 1e3), "ms_per_step": ms2,
+                 "latency_p50_ms": 1e3 * float(np.percentile(lat2, 50)),
+                 "latency_p99_ms": 1e3 * float(np.percentile(lat2, 99)),
+                 "kernels": kernel_numbers(e2, capi, d, 2)}
+        if not args.no_quality:
+            entry["quality"] = quality(ev, x, e2, n)
+        e2.close()
+        entry["headline_speedup"] = value / entry["value"]
+        compare[m] = entry
+        log(f"[{m}] {entry['value']:.0f} points/s, headline is "
+            f"{entry['headline_speedup']:.2f}x")
 
     cpu = None
     if not args.no_cpu_baseline:
-        rows = args.cpu_rows or (4000 if d > 256 else 10_000)
+        # a sample of about 10-30 s of CPU work on 16 cores
+        rows = args.cpu_rows or (15_000 if d > 256 else 10_000)
         cpu = cpu_baseline(x, min(rows, n), knn_mode=mode)
         log(f"cpu baseline {cpu}")
 
     line = {
-        "metric": "points embedded/s @recall@15>=0.95 (FP32 path); p50/p99 run() latency",
+        "metric": METRIC,
         "value": value,
         "unit": "points/s",
         "n_gpus": 1,
@@ -241,17 +315,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "bf16" if mode == MODES["exact_bf16"] else "f32",
         "data": "synthetic",
         "config": {"workload": WORKLOAD_DESC[args.workload], "n_points": n,
-                   "n_features": d, "knn_mode": args.knn_mode, "op_plan": "run(500,500,500) per 500-row "
-                   "chunk, then run(0,N,N) until update_completed, then one "
-                   "full embedding sweep"},
+                   "n_features": d, "knn_mode": args.knn_mode,
+                   "precision": ("distance tiles BF16 MFMA (hi/lo split, FP32 "
+                                 "accumulation) screened by a rigorous error "
+                                 "bound and re-ranked in exact FP32; sigma/rho "
+                                 "FP64; SGD FP32")
+                   if mode == MODES["exact_bf16"] else "FP32",
+                   "op_plan": "run(500,500,500) per 500-row chunk, then "
+                              "run(0,N,N) until update_completed, then one "
+                              "full embedding sweep"},
         "latency_p50_ms": 1e3 * float(np.percentile(lats, 50)),
         "latency_p99_ms": 1e3 * float(np.percentile(lats, 99)),
-        "quality": quality,
+        "quality": qual,
         "kernels": kern,
         "roofline": roof,
+        "compare": compare,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -54,7 +54,7 @@ class GmKernelStats(ctypes.Structure):
                 ("work", u64 * 4)]
 
 
-KSTAT_JOIN, KSTAT_SGD, KSTAT_INSERT = 0, 1, 2
+KSTAT_JOIN, KSTAT_SGD, KSTAT_INSERT, KSTAT_SGD_STREAM = 0, 1, 2, 3
 KNN_NNDESCENT, KNN_EXACT_FP32, KNN_EXACT_BF16 = 0, 1, 2
 
 

@@ -813,6 +813,8 @@ static hipError_t try_stream(float *emb, const int32_t *head,
     return hipErrorInvalidConfiguration;
   const uint64_t blocks = std::min<uint64_t>((uint64_t)per_cu * num_cu,
                                              (nnz + 255) / 256);
+  // its work counters go to their own slots (stats + 8)
+  stats = stats ? stats + 8 : nullptr;
   void *args[] = {&emb,   &head,    &tail,    &nnz_p, &n,  &eps,
                   &nxt,   &nneg,    &n_epochs, &alpha0, &gamma, &a,
                   &b,     &nsr_inv, &seed,    &rounding_p, &bar, &stats,
@@ -831,7 +833,8 @@ static hipError_t try_fused_any(float *emb, const int32_t *head,
                                 uint64_t seed, const float *rounding_p,
                                 unsigned *bar, unsigned long long *stats,
                                 int lag, uint64_t nnz, int num_cu,
-                                hipStream_t s) {
+                                hipStream_t s, int *variant) {
+  *variant = 0;
 #define GM_TRY(EPT)                                                            \
   {                                                                            \
     const hipError_t e = try_fused<NC, EPT>(                                   \
@@ -843,6 +846,7 @@ static hipError_t try_fused_any(float *emb, const int32_t *head,
   GM_TRY(1)
   GM_TRY(2)
 #undef GM_TRY
+  *variant = 1;
   return try_stream<NC>(emb, head, tail, nnz_p, n, eps, nxt, nneg, n_epochs,
                         alpha0, gamma, a, b, nsr_inv, seed, rounding_p, bar,
                         stats, lag, nnz, num_cu, s);
@@ -855,24 +859,25 @@ hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
                             float gamma, float a, float b, float nsr_inv,
                             uint64_t seed, const float *rounding_p,
                             unsigned *bar, unsigned long long *stats, int lag,
-                            uint64_t nnz, int num_cu, hipStream_t s) {
+                            uint64_t nnz, int num_cu, hipStream_t s,
+                            int *variant) {
   switch (nc) {
   case 1:
     return try_fused_any<1>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
                             n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
-                            rounding_p, bar, stats, lag, nnz, num_cu, s);
+                            rounding_p, bar, stats, lag, nnz, num_cu, s, variant);
   case 2:
     return try_fused_any<2>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
                             n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
-                            rounding_p, bar, stats, lag, nnz, num_cu, s);
+                            rounding_p, bar, stats, lag, nnz, num_cu, s, variant);
   case 3:
     return try_fused_any<3>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
                             n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
-                            rounding_p, bar, stats, lag, nnz, num_cu, s);
+                            rounding_p, bar, stats, lag, nnz, num_cu, s, variant);
   default:
     return try_fused_any<4>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
                             n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
-                            rounding_p, bar, stats, lag, nnz, num_cu, s);
+                            rounding_p, bar, stats, lag, nnz, num_cu, s, variant);
   }
 }
 

@@ -144,7 +144,7 @@ struct Engine {
   uint32_t *bcount = nullptr;
   uint32_t *scal = nullptr;
   unsigned long long *fresh = nullptr;
-  unsigned long long *kstats = nullptr; // device work counters (8)
+  unsigned long long *kstats = nullptr; // device work counters (16)
   // profiling: HIP events around the timed kernels on this stream
   bool prof = false;
   struct Timed {
@@ -154,7 +154,8 @@ struct Engine {
   };
   std::vector<Timed> pending;
   gm_kernel_stats totals[GM_KSTAT_COUNT] = {};
-  uint64_t sgd_edge_epochs = 0;
+  uint64_t sgd_edge_epochs = 0, sgd_stream_edge_epochs = 0;
+  int sgd_variant = 0; // last cooperative SGD launch: 0 resident, 1 stream
   // insertion scratch
   uint64_t ins_cap = 0;
   uint32_t *ins_ids = nullptr, *cand = nullptr;
@@ -243,8 +244,8 @@ struct Engine {
     bcount = dalloc<uint32_t>(std::max<uint64_t>(2 * nk, N) / 1024 + 2);
     scal = dalloc<uint32_t>(S_COUNT);
     fresh = dalloc<unsigned long long>(1);
-    kstats = dalloc<unsigned long long>(8);
-    HIPC(hipMemsetAsync(kstats, 0, 8 * sizeof(unsigned long long), s));
+    kstats = dalloc<unsigned long long>(16);
+    HIPC(hipMemsetAsync(kstats, 0, 16 * sizeof(unsigned long long), s));
     fn = dalloc<uint32_t>(N * W);
     fo = dalloc<uint32_t>(N * W);
     newmask = dalloc<uint32_t>(N);
@@ -403,7 +404,7 @@ struct Engine {
       (void)hipEventDestroy(t.b);
     }
     pending.clear();
-    unsigned long long c[8];
+    unsigned long long c[16];
     HIPC(hipMemcpy(c, kstats, sizeof(c), hipMemcpyDeviceToHost));
     totals[GM_KSTAT_JOIN].work[0] = c[0];
     totals[GM_KSTAT_JOIN].work[1] = c[1];
@@ -411,6 +412,9 @@ struct Engine {
     totals[GM_KSTAT_SGD].work[0] = c[3];
     totals[GM_KSTAT_SGD].work[1] = c[4];
     totals[GM_KSTAT_SGD].work[2] = sgd_edge_epochs;
+    totals[GM_KSTAT_SGD_STREAM].work[0] = c[11];
+    totals[GM_KSTAT_SGD_STREAM].work[1] = c[12];
+    totals[GM_KSTAT_SGD_STREAM].work[2] = sgd_stream_edge_epochs;
     totals[GM_KSTAT_INSERT].work[0] = ins_pairs;
     totals[GM_KSTAT_INSERT].work[1] = c[5];
     totals[GM_KSTAT_INSERT].work[2] = c[6];
@@ -818,10 +822,12 @@ struct Engine {
       const hipError_t fe = launch_sgd_fused(
           c, e, head, tail, scal + S_NNZ, (uint32_t)nrows, eps, nxt, nneg,
           n_epochs, alpha0, gamma, a, b, nsr_inv, seed, rounding, gbar, st,
-          SGD_LAG, cap, num_cu, s);
+          SGD_LAG, cap, num_cu, s, &sgd_variant);
       if (fe == hipSuccess) {
         fused = true;
         fused_used = true;
+        if (prof && sgd_variant == 1)
+          pending.back().which = GM_KSTAT_SGD_STREAM;
       } else {
         // too many edges for the resident grid, or cooperative launch not
         // available in this context: one launch per epoch instead
@@ -861,13 +867,14 @@ struct Engine {
       LAUNCH_CHECK();
       alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
     }
-    ev_stop((uint64_t)n_epochs);
+    ev_stop(fused ? 1 : (uint64_t)n_epochs);
     if (prof) {
       uint32_t nnz = 0;
       HIPC(hipMemcpyAsync(h_scal + 10, scal + S_NNZ, 4, hipMemcpyDeviceToHost, s));
       HIPC(hipStreamSynchronize(s));
       nnz = h_scal[10];
-      sgd_edge_epochs += (uint64_t)nnz * (uint64_t)n_epochs;
+      (fused && sgd_variant == 1 ? sgd_stream_edge_epochs : sgd_edge_epochs) +=
+          (uint64_t)nnz * (uint64_t)n_epochs;
     }
   }
 
@@ -1115,7 +1122,7 @@ void gm_default_params(gm_params *o) {
   o->alpha = 1.0f;
   o->verbose = 0;
   o->metric = GM_EUCLIDEAN;
-  o->knn_mode = GM_KNN_EXACT_FP32;
+  o->knn_mode = GM_KNN_EXACT_BF16;
 }
 
 int gm_create(const gm_params *params, gm_handle **out) {
@@ -1283,8 +1290,9 @@ int gm_reset_kernel_stats(gm_handle *h) {
     for (auto &t : e->totals)
       t = gm_kernel_stats{};
     e->sgd_edge_epochs = 0;
+    e->sgd_stream_edge_epochs = 0;
     e->ins_pairs = 0;
-    HIPC(hipMemsetAsync(e->kstats, 0, 8 * sizeof(unsigned long long), e->s));
+    HIPC(hipMemsetAsync(e->kstats, 0, 16 * sizeof(unsigned long long), e->s));
     HIPC(hipStreamSynchronize(e->s));
   });
 }

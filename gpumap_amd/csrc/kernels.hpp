@@ -136,8 +136,9 @@ __global__ void k_sgd_fused(float *emb, const int32_t *head,
                             const float *rounding_p, unsigned *bar,
                             unsigned long long *stats, int lag);
 // cooperative launch of k_sgd_fused or, for edge sets larger than its
-// resident grid, k_sgd_stream (embed_kernels.hip); returns the launch error
-// when neither can be launched
+// resident grid, k_sgd_stream (embed_kernels.hip; *variant 0 or 1, the
+// latter counting its work at stats + 8); returns the launch error when
+// neither can be launched
 hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
                             const int32_t *tail, const uint32_t *nnz_p,
                             uint32_t n, const float *eps, float *nxt,
@@ -145,7 +146,8 @@ hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
                             float gamma, float a, float b, float nsr_inv,
                             uint64_t seed, const float *rounding_p,
                             unsigned *bar, unsigned long long *stats, int lag,
-                            uint64_t nnz, int num_cu, hipStream_t s);
+                            uint64_t nnz, int num_cu, hipStream_t s,
+                            int *variant);
 __global__ void k_reposition(const uint32_t *pidx, const float *pdist,
                              uint64_t n, int k, int euclid, const double *sigma,
                              const double *rho, const uint32_t *rows,

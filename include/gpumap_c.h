@@ -63,7 +63,7 @@ typedef struct gm_params {
   uint64_t n_instances;          /* capacity N */
   uint64_t n_features;           /* D */
   int32_t knn_mode;              /* gm_knn_mode (no counterpart in the
-                                    reference; default GM_KNN_EXACT_FP32) */
+                                    reference; default GM_KNN_EXACT_BF16) */
 } gm_params;
 
 /* Mirrors gpumap::GPUMAPResult (cpp/include/gpumap/gpumap.cuh:38-56). */
@@ -182,19 +182,26 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
  * whole operations, impl.cuh:159-203).  With profiling on, HIP events on the
  * session's stream bracket the timed kernels and device counters record their
  * algorithmic work:
- *   GM_KSTAT_JOIN    local join; work = {list members, distance pairs,
- *                    queued pool offers}
- *   GM_KSTAT_SGD     SGD epochs; work = {positive samples, negative samples,
- *                    edge-epochs scanned}
- *   GM_KSTAT_INSERT  insertions (all kernels of one insertion); work =
- *                    {(batch row, row) pairs, BF16 screening survivors
- *                    re-ranked in FP32, insertions redone by the FP32 tiles
- *                    after a survivor-queue overflow} */
+ *   GM_KSTAT_JOIN        local join; work = {list members, distance pairs,
+ *                        queued pool offers}
+ *   GM_KSTAT_SGD         SGD with the register-resident cooperative kernel
+ *                        (one launch per embedding op) or, when cooperative
+ *                        launch is unavailable, one launch per epoch; work =
+ *                        {positive samples, negative samples, edge-epochs
+ *                        scanned}
+ *   GM_KSTAT_INSERT      insertions (all kernels of one insertion); work =
+ *                        {(batch row, row) pairs, BF16 screening survivors
+ *                        re-ranked in FP32, insertions redone by the FP32
+ *                        tiles after a survivor-list overflow}
+ *   GM_KSTAT_SGD_STREAM  SGD with the streaming cooperative kernel (edge sets
+ *                        larger than the register-resident grid); work as
+ *                        GM_KSTAT_SGD */
 typedef enum gm_kstat {
   GM_KSTAT_JOIN = 0,
   GM_KSTAT_SGD = 1,
   GM_KSTAT_INSERT = 2,
-  GM_KSTAT_COUNT = 3
+  GM_KSTAT_SGD_STREAM = 3,
+  GM_KSTAT_COUNT = 4
 } gm_kstat;
 
 typedef struct gm_kernel_stats {

@@ -37,6 +37,10 @@
 //  D9 Update convergence threshold max(1, 1e-4 n) rows: the reference's
 //     (size_t)(1e-4 n) is 0 below n = 10,000 and `size < 0` never holds
 //     (impl.cuh:261-263,320), so small sessions would never complete.
+//  D10 A negative sample's index is the high word of r * n for a 32-bit
+//     Philox draw r (reference: curand() % n, embedding_kernel.cuh:567): the
+//     same near-uniform choice without an integer division on the device.
+//     The reference's cuRAND streams are not reproducible here anyway.
 //
 // knn_mode 1 (exact insertion, the MI355X path's default): each inserted row's
 // pool is the exact top-32 (canonical distance, ties by id) over all rows

@@ -69,7 +69,7 @@ def test_params_defaults_and_metric_property():
     assert p.min_dist == pytest.approx(0.1)
     assert (p.a, p.b, p.alpha, p.verbose) == (1.0, 1.0, 1.0, False)
     assert p.metric == "euclidean"
-    assert p.knn_mode == "exact_fp32"
+    assert p.knn_mode == "exact_bf16"
     p.knn_mode = "nndescent"
     assert p.knn_mode == "nndescent"
     p.knn_mode = "exact_bf16"
