@@ -390,105 +390,134 @@ __device__ inline float fclip(float v) {
   return fmaxf(fminf(v, 4.0f), -4.0f);
 }
 
+// The first 8 raw draws of edge e's negative-sample stream for one epoch
+// (two Philox4x32-10 blocks; the stream continues at block 2 when an edge
+// needs more than 8 samples).  A sample's index is the high word of draw * n.
+__device__ inline void neg_draws(uint64_t seed, uint64_t e, uint32_t out[8]) {
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t ctr[4] = {(uint32_t)e, (uint32_t)(e >> 32), 0u, TAG_NEG};
+  philox(ctr, key, out);
+  ctr[2] = 1u;
+  philox(ctr, key, out + 4);
+}
+
 // One positive sample of edge e and its negative samples (reference
-// optimize_batch_kernel :509-589) for any n_components <= 4.  Returns the
-// edge's contributions to its head (gh) and tail (gt), each truncated to the
-// rounding grid as the reference does before its atomic adds; the caller adds
-// them to the embedding (seg_add).  The first 8 negative samples' positions
-// are gathered together with the edge's two endpoints (one memory round
-// trip), before the sequential updates of the head (the values a sequential
-// read would see in a Hogwild run).  A negative sample's index is the high
-// word of next() * n.
-template <int NC>
-__device__ inline void sgd_edge(const float *__restrict__ emb, int32_t j,
-                                int32_t kk, uint64_t e, uint32_t n, float ep,
-                                float &nx, float &nn, float alpha, int epoch,
-                                float gamma, float a, float b, float nsr_inv,
-                                uint64_t seed, float rounding, float gh[NC],
-                                float gt[NC], unsigned long long &c_pos,
-                                unsigned long long &c_neg) {
-  const float ep_neg = ep * nsr_inv;
-  const int n_neg = (int)(((float)epoch - nn) / ep_neg);
-  ++c_pos;
-  const float *cur_p = emb + (uint64_t)j * NC;
-  const float *oth_p = emb + (uint64_t)kk * NC;
-  float cur[NC], oth[NC], g[NC];
-  Rng rng(seed, (uint32_t)e, (uint32_t)(e >> 32), TAG_NEG);
+// optimize_batch_kernel :509-589) for any n_components <= 4, in two halves so
+// that a caller can overlap independent work with the memory round trip:
+// sgd_gather issues the loads of the edge's two endpoints and of its first 8
+// negative samples (positions taken before the sequential updates of the
+// head: the values a sequential read would see in a Hogwild run);
+// sgd_update does the arithmetic and returns the edge's contributions to its
+// head (gh) and tail (gt), each truncated to the rounding grid as the
+// reference does before its atomic adds (the caller adds them, seg_add).
+// The gradient coefficients use the hardware reciprocal (1 ulp); the epoch
+// schedule keeps IEEE division.
+template <int NC> struct EdgeLoads {
+  float cur[NC], oth[NC], neg[8][NC];
   uint32_t r[8];
-  float neg[8][NC];
-  int cnt = min(8, n_neg);
+  int n_neg;
+};
+
+template <int NC>
+__device__ inline void sgd_gather(const float *__restrict__ emb, int32_t j,
+                                  int32_t kk, uint32_t n, float ep_neg,
+                                  float nn, int epoch, const uint32_t draw[8],
+                                  EdgeLoads<NC> &L) {
+  L.n_neg = (int)(((float)epoch - nn) / ep_neg);
+  const int cnt = min(8, L.n_neg);
 #pragma unroll
   for (int q = 0; q < 8; ++q)
     if (q < cnt) {
-      r[q] = __umulhi(rng.next(), n);
-      const float *np_ = emb + (uint64_t)r[q] * NC;
+      L.r[q] = __umulhi(draw[q], n);
+      const float *np_ = emb + (uint64_t)L.r[q] * NC;
 #pragma unroll
       for (int d = 0; d < NC; ++d)
-        neg[q][d] = np_[d];
+        L.neg[q][d] = np_[d];
     }
 #pragma unroll
   for (int d = 0; d < NC; ++d) {
-    cur[d] = cur_p[d];
-    oth[d] = oth_p[d];
+    L.cur[d] = emb[(uint64_t)j * NC + d];
+    L.oth[d] = emb[(uint64_t)kk * NC + d];
   }
+}
+
+template <int NC>
+__device__ inline void neg_step(float cur[NC], float g[NC], const float neg[NC],
+                                float alpha, float gamma, float a, float b) {
+  float d2n = 0.0f;
+#pragma unroll
+  for (int d = 0; d < NC; ++d) {
+    const float t = cur[d] - neg[d];
+    d2n += t * t;
+  }
+  float rep = 0.0f;
+  if (d2n > 0.0f)
+    rep = (2.0f * gamma * b) *
+          __builtin_amdgcn_rcpf((0.001f + d2n) * (a * pow_pos(d2n, b) + 1.0f));
+#pragma unroll
+  for (int d = 0; d < NC; ++d) {
+    const float gg =
+        rep > 0.0f ? fclip(rep * (cur[d] - neg[d])) * alpha : 4.0f * alpha;
+    cur[d] += gg;
+    g[d] += gg;
+  }
+}
+
+template <int NC>
+__device__ inline void sgd_update(const float *__restrict__ emb, int32_t j,
+                                  uint64_t e, uint32_t n, float ep, float &nx,
+                                  float &nn, float ep_neg, float alpha,
+                                  float gamma, float a, float b, uint64_t seed,
+                                  float rounding, EdgeLoads<NC> &L,
+                                  float gh[NC], float gt[NC],
+                                  unsigned long long &c_pos,
+                                  unsigned long long &c_neg) {
+  ++c_pos;
+  float g[NC];
   float d2 = 0.0f;
 #pragma unroll
   for (int d = 0; d < NC; ++d) {
-    const float t = cur[d] - oth[d];
+    const float t = L.cur[d] - L.oth[d];
     d2 += t * t;
   }
   float attr = 0.0f;
   if (d2 > 0.0f) {
     const float pb = pow_pos(d2, b);
-    attr = (-2.0f * a * b * (pb / d2)) / (a * pb + 1.0f);
+    attr = (-2.0f * a * b * pb) *
+           __builtin_amdgcn_rcpf(d2 * (a * pb + 1.0f));
   }
 #pragma unroll
   for (int d = 0; d < NC; ++d) {
-    g[d] = fclip(attr * (cur[d] - oth[d])) * alpha;
-    cur[d] += g[d];
+    g[d] = fclip(attr * (L.cur[d] - L.oth[d])) * alpha;
+    L.cur[d] += g[d];
     gt[d] = (rounding + -g[d]) - rounding;
   }
   nx += ep;
+  const int n_neg = L.n_neg;
   if (n_neg > 0) {
-    for (int p0 = 0;;) {
+    const int cnt = min(8, n_neg);
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        if (q >= cnt || (int32_t)r[q] == j)
+    for (int q = 0; q < 8; ++q)
+      if (q < cnt && (int32_t)L.r[q] != j) {
+        ++c_neg;
+        neg_step<NC>(L.cur, g, L.neg[q], alpha, gamma, a, b);
+      }
+    if (n_neg > 8) {
+      // more than 8 negative samples (an edge skipped many epochs): the
+      // stream continues at Philox block 2
+      Rng rng(seed, (uint32_t)e, (uint32_t)(e >> 32), TAG_NEG);
+      rng.ctr[2] = 2u;
+      for (int p = 8; p < n_neg; ++p) {
+        const uint32_t r = __umulhi(rng.next(), n);
+        if ((int32_t)r == j)
           continue;
         ++c_neg;
-        float d2n = 0.0f;
+        float neg[NC];
 #pragma unroll
-        for (int d = 0; d < NC; ++d) {
-          const float t = cur[d] - neg[q][d];
-          d2n += t * t;
-        }
-        float rep = 0.0f;
-        if (d2n > 0.0f)
-          rep = (2.0f * gamma * b) /
-                ((0.001f + d2n) * (a * pow_pos(d2n, b) + 1.0f));
-#pragma unroll
-        for (int d = 0; d < NC; ++d) {
-          const float gg =
-              rep > 0.0f ? fclip(rep * (cur[d] - neg[q][d])) * alpha
-                         : 4.0f * alpha;
-          cur[d] += gg;
-          g[d] += gg;
-        }
+        for (int d = 0; d < NC; ++d)
+          neg[d] = emb[(uint64_t)r * NC + d];
+        neg_step<NC>(L.cur, g, neg, alpha, gamma, a, b);
       }
-      p0 += 8;
-      if (p0 >= n_neg)
-        break;
-      // more than 8 negative samples (an edge skipped many epochs)
-      cnt = min(8, n_neg - p0);
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (q < cnt) {
-          r[q] = __umulhi(rng.next(), n);
-          const float *np_ = emb + (uint64_t)r[q] * NC;
-#pragma unroll
-          for (int d = 0; d < NC; ++d)
-            neg[q][d] = np_[d];
-        }
     }
     nn += (float)n_neg * ep_neg;
   }
@@ -498,9 +527,10 @@ __device__ inline void sgd_edge(const float *__restrict__ emb, int32_t j,
 }
 
 // Runs of equal row ids over the lanes of a wave (edges in COO order, so a
-// row's edges as head, or as tail, are mostly adjacent).  m bit s: lane +
-// 2^s holds the same run; start: first lane of its run.  Lanes without an
-// edge pass a key of their own (< 0).
+// row's edges as head, or as tail, are mostly adjacent), cut at the 16-lane
+// DPP rows.  m bit s (s < 4): lane + 2^s is in the same run and row; start:
+// first lane of its run within its row.  Lanes without an edge pass a key of
+// their own (< 0).
 struct Seg {
   uint32_t m;
   bool start;
@@ -509,36 +539,49 @@ struct Seg {
 __device__ inline Seg seg_of(int32_t key) {
   const int lane = threadIdx.x & 63;
   const int32_t prev = __shfl_up(key, 1, 64);
-  const bool start = lane == 0 || prev != key;
+  const bool start = (lane & 15) == 0 || prev != key;
   const uint64_t sm = __ballot(start);
   const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
   const int run = __popcll(sm & upto);
   uint32_t m = 0;
 #pragma unroll
-  for (int s = 0; s < 6; ++s) {
+  for (int s = 0; s < 4; ++s) {
     const int ro = __shfl_down(run, 1 << s, 64);
-    if (lane + (1 << s) < 64 && ro == run)
+    if ((lane & 15) + (1 << s) < 16 && ro == run)
       m |= 1u << s;
   }
   return Seg{m, start};
 }
 
-// Sum each run's contributions into its first lane and add them to the row
-// there: one atomic per run and component instead of one per edge.  The
-// contributions are multiples of the rounding grid and a run holds at most
-// max-degree of them, so their sum is exact in any order.  All lanes of the
-// wave must call this together.
+// v from lane + 2^S of the same 16-lane row (0 past the row's end)
+template <int S> __device__ inline float dpp_from_above(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(
+      0, __float_as_int(v), 0x100 | (1 << S), 0xf, 0xf, true));
+}
+
+// Sum each run's contributions (within a 16-lane row) into its first lane
+// with four DPP row-shift steps and add them to the row there: one atomic
+// per run and component instead of one per edge.  The contributions are
+// multiples of the rounding grid and a run holds at most max-degree of them,
+// so their sum is exact in any order.  All lanes of the wave must call this
+// together.
 template <int NC>
 __device__ inline void seg_add(float *__restrict__ emb, int32_t row, Seg sg,
                                float v[NC]) {
 #pragma unroll
-  for (int s = 0; s < 6; ++s) {
-#pragma unroll
-    for (int d = 0; d < NC; ++d) {
-      const float o = __shfl_down(v[d], 1 << s, 64);
-      if ((sg.m >> s) & 1u)
-        v[d] += o;
-    }
+  for (int d = 0; d < NC; ++d) {
+    float o = dpp_from_above<0>(v[d]);
+    if (sg.m & 1u)
+      v[d] += o;
+    o = dpp_from_above<1>(v[d]);
+    if (sg.m & 2u)
+      v[d] += o;
+    o = dpp_from_above<2>(v[d]);
+    if (sg.m & 4u)
+      v[d] += o;
+    o = dpp_from_above<3>(v[d]);
+    if (sg.m & 8u)
+      v[d] += o;
   }
   bool any = false;
 #pragma unroll
@@ -591,8 +634,14 @@ __device__ inline void sgd_step(float *__restrict__ emb,
     float nx = nxt[e];
     if (!(nx > (float)epoch)) {
       float nn = nneg[e];
-      sgd_edge<NC>(emb, hv, tv, e, n, eps[e], nx, nn, alpha, epoch, gamma, a,
-                   b, nsr_inv, seed, rounding, gh, gt, c_pos, c_neg);
+      const float ep = eps[e];
+      const float ep_neg = ep * nsr_inv;
+      uint32_t dr[8];
+      neg_draws(seed, e, dr);
+      EdgeLoads<NC> L;
+      sgd_gather<NC>(emb, hv, tv, n, ep_neg, nn, epoch, dr, L);
+      sgd_update<NC>(emb, hv, e, n, ep, nx, nn, ep_neg, alpha, gamma, a, b,
+                     seed, rounding, L, gh, gt, c_pos, c_neg);
       nxt[e] = nx;
       nneg[e] = nn;
     }
@@ -683,6 +732,7 @@ __global__ __launch_bounds__(256) void k_sgd_fused(
   const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int32_t hh[EPT], tt[EPT];
   float ep[EPT], nx[EPT], nn[EPT];
+  uint32_t dr[EPT][8]; // this epoch's first 8 negative-sample draws
   Seg sh[EPT], st[EPT];
 #pragma unroll
   for (int q = 0; q < EPT; ++q) {
@@ -701,6 +751,7 @@ __global__ __launch_bounds__(256) void k_sgd_fused(
     }
     sh[q] = seg_of(hh[q]);
     st[q] = seg_of(tt[q]);
+    neg_draws(seed, e, dr[q]);
   }
   unsigned long long c_pos = 0, c_neg = 0;
   unsigned seen = 0;
@@ -710,14 +761,21 @@ __global__ __launch_bounds__(256) void k_sgd_fused(
     const uint64_t es = seed + (uint64_t)epoch;
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
+      const uint64_t e = t0 + (uint64_t)q * stride;
+      const bool act = !(nx[q] > (float)epoch);
+      const float ep_neg = ep[q] * nsr_inv;
+      EdgeLoads<NC> L;
+      if (act)
+        sgd_gather<NC>(emb, hh[q], tt[q], n, ep_neg, nn[q], epoch, dr[q], L);
+      // next epoch's draws, by every lane while the loads are in flight
+      neg_draws(es + 1, e, dr[q]);
       float gh[NC], gt[NC];
 #pragma unroll
       for (int d = 0; d < NC; ++d)
         gh[d] = gt[d] = 0.0f;
-      if (!(nx[q] > (float)epoch))
-        sgd_edge<NC>(emb, hh[q], tt[q], t0 + (uint64_t)q * stride, n, ep[q],
-                     nx[q], nn[q], alpha, epoch, gamma, a, b, nsr_inv, es,
-                     rounding, gh, gt, c_pos, c_neg);
+      if (act)
+        sgd_update<NC>(emb, hh[q], e, n, ep[q], nx[q], nn[q], ep_neg, alpha,
+                       gamma, a, b, es, rounding, L, gh, gt, c_pos, c_neg);
       seg_add<NC>(emb, tt[q], st[q], gt);
       seg_add<NC>(emb, hh[q], sh[q], gh);
     }
@@ -727,11 +785,116 @@ __global__ __launch_bounds__(256) void k_sgd_fused(
   flush_sgd_stats(stats, c_pos, c_neg);
 }
 
+// All epochs in one cooperative launch for edge sets of at most 512 waves
+// (one edge per thread, one wave per block): the per-cycle embedding batches.
+// Epoch boundary without atomics or block barriers: wave w stores epoch + 1
+// into its arrival slot (bar + 4 + w) when it finishes an epoch, and may
+// start epoch e + 1 once every slot is >= e - lag + 1.  Each lane reads up
+// to 8 slots at the start of the epoch and the wave reduces them at the
+// boundary, so the read overlaps the epoch's own gathers and a wave that is
+// not more than `lag` epochs ahead waits for nothing.  The wait is bounded
+// (timeout flag in bar[2], as epoch_end).
+template <int NC>
+__global__ __launch_bounds__(64) void k_sgd_waves(
+    float *__restrict__ emb, const int32_t *__restrict__ head,
+    const int32_t *__restrict__ tail, const uint32_t *__restrict__ nnz_p,
+    uint32_t n, const float *__restrict__ eps, int n_epochs, float alpha0,
+    float gamma, float a, float b, float nsr_inv, uint64_t seed,
+    const float *__restrict__ rounding_p, unsigned *__restrict__ bar,
+    unsigned long long *__restrict__ stats, int lag) {
+  unsigned *slots = bar + 4;
+  const unsigned nw = gridDim.x;
+  const int lane = threadIdx.x;
+  const uint64_t nnz = *nnz_p;
+  const float rounding = *rounding_p;
+  const uint64_t e = (uint64_t)blockIdx.x * 64 + lane;
+  int32_t hh, tt;
+  float ep, nx, nn;
+  if (e < nnz) {
+    hh = head[e];
+    tt = tail[e];
+    ep = eps[e];
+    nx = ep;
+    nn = ep * nsr_inv;
+  } else {
+    hh = tt = -1 - lane;
+    ep = -1.0f;
+    nx = INFINITY;
+    nn = 0.0f;
+  }
+  const Seg sh = seg_of(hh), st = seg_of(tt);
+  uint32_t dr[8];
+  neg_draws(seed, e, dr);
+  unsigned long long c_pos = 0, c_neg = 0;
+  float alpha = alpha0;
+  for (int epoch = 0; epoch < n_epochs; ++epoch) {
+    unsigned sv[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const unsigned idx = (unsigned)lane + 64u * i;
+      sv[i] = idx < nw ? __hip_atomic_load(slots + idx, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT)
+                       : ~0u;
+    }
+    const uint64_t es = seed + (uint64_t)epoch;
+    const bool act = !(nx > (float)epoch);
+    const float ep_neg = ep * nsr_inv;
+    EdgeLoads<NC> L;
+    if (act)
+      sgd_gather<NC>(emb, hh, tt, n, ep_neg, nn, epoch, dr, L);
+    neg_draws(es + 1, e, dr);
+    float gh[NC], gt[NC];
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      gh[d] = gt[d] = 0.0f;
+    if (act)
+      sgd_update<NC>(emb, hh, e, n, ep, nx, nn, ep_neg, alpha, gamma, a, b,
+                     es, rounding, L, gh, gt, c_pos, c_neg);
+    seg_add<NC>(emb, tt, st, gt);
+    seg_add<NC>(emb, hh, sh, gh);
+    alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
+    // boundary
+    if (lane == 0)
+      __hip_atomic_store(slots + blockIdx.x, (unsigned)epoch + 1u,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (epoch >= lag && epoch + 1 < n_epochs) {
+      const unsigned target = (unsigned)(epoch - lag + 1);
+      unsigned spins = 0;
+      for (;;) {
+        unsigned m = sv[0];
+#pragma unroll
+        for (int i = 1; i < 8; ++i)
+          m = min(m, sv[i]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+          m = min(m, (unsigned)__shfl_xor((int)m, off, 64));
+        if (m >= target)
+          break;
+        if (++spins > (1u << 20)) {
+          if (lane == 0)
+            __hip_atomic_store(bar + 2, 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const unsigned idx = (unsigned)lane + 64u * i;
+          sv[i] = idx < nw ? __hip_atomic_load(slots + idx, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)
+                           : ~0u;
+        }
+      }
+    }
+  }
+  flush_sgd_stats(stats, c_pos, c_neg);
+}
+
 // All epochs in one cooperative launch for edge sets larger than the
 // register-resident grid: the schedule stays in memory (eps, nxt, nneg as in
 // k_sgd_epoch) and each thread walks a grid-strided range of edges per epoch.
 template <int NC>
-__global__ __launch_bounds__(256) void k_sgd_stream(
+__global__ __launch_bounds__(1024) void k_sgd_stream(
     float *__restrict__ emb, const int32_t *__restrict__ head,
     const int32_t *__restrict__ tail, const uint32_t *__restrict__ nnz_p,
     uint32_t n, const float *__restrict__ eps, float *__restrict__ nxt,
@@ -765,10 +928,10 @@ __global__ __launch_bounds__(256) void k_sgd_stream(
 // thread when its co-resident grid holds every edge, else the streaming
 // kernel on the co-resident grid.  Returns hipSuccess, or
 // the launch error (the caller then launches one kernel per epoch).
-template <typename K> static int blocks_per_cu(K kernel) {
+template <typename K> static int blocks_per_cu(K kernel, int threads) {
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, reinterpret_cast<const void *>(kernel), 256, 0) !=
+          &per_cu, reinterpret_cast<const void *>(kernel), threads, 0) !=
       hipSuccess)
     return 0;
   return per_cu;
@@ -785,7 +948,9 @@ static hipError_t try_fused(float *emb, const int32_t *head,
                             int num_cu, hipStream_t s) {
   static int per_cu = 0; // cached once the occupancy query succeeds
   if (per_cu <= 0)
-    per_cu = blocks_per_cu(&k_sgd_fused<NC, EPT>);
+    per_cu = blocks_per_cu(&k_sgd_fused<NC, EPT>, 256);
+  // 256-thread blocks: every block arrives on one counter per epoch, so
+  // fewer, larger blocks keep that serialised atomic traffic small
   const uint64_t need = (nnz + 256ull * EPT - 1) / (256ull * EPT);
   if (need == 0 || need > (uint64_t)per_cu * (uint64_t)num_cu)
     return hipErrorInvalidConfiguration;
@@ -795,6 +960,30 @@ static hipError_t try_fused(float *emb, const int32_t *head,
   return hipLaunchCooperativeKernel(
       reinterpret_cast<const void *>(&k_sgd_fused<NC, EPT>),
       dim3((unsigned)need), dim3(256), args, 0, s);
+}
+
+template <int NC>
+static hipError_t try_waves(float *emb, const int32_t *head,
+                            const int32_t *tail, const uint32_t *nnz_p,
+                            uint32_t n, const float *eps, int n_epochs,
+                            float alpha0, float gamma, float a, float b,
+                            float nsr_inv, uint64_t seed,
+                            const float *rounding_p, unsigned *bar,
+                            unsigned long long *stats, int lag, uint64_t nnz,
+                            int num_cu, hipStream_t s) {
+  static int per_cu = 0;
+  if (per_cu <= 0)
+    per_cu = blocks_per_cu(&k_sgd_waves<NC>, 64);
+  const uint64_t need = (nnz + 63) / 64;
+  if (need == 0 || need > SGD_MAX_WAVES ||
+      need > (uint64_t)per_cu * (uint64_t)num_cu)
+    return hipErrorInvalidConfiguration;
+  void *args[] = {&emb,     &head,   &tail,  &nnz_p, &n,       &eps,
+                  &n_epochs, &alpha0, &gamma, &a,     &b,       &nsr_inv,
+                  &seed,    &rounding_p, &bar, &stats, &lag};
+  return hipLaunchCooperativeKernel(
+      reinterpret_cast<const void *>(&k_sgd_waves<NC>), dim3((unsigned)need),
+      dim3(64), args, 0, s);
 }
 
 template <int NC>
@@ -808,11 +997,12 @@ static hipError_t try_stream(float *emb, const int32_t *head,
                              uint64_t nnz, int num_cu, hipStream_t s) {
   static int per_cu = 0;
   if (per_cu <= 0)
-    per_cu = blocks_per_cu(&k_sgd_stream<NC>);
+    per_cu = blocks_per_cu(&k_sgd_stream<NC>, 1024);
   if (per_cu <= 0 || nnz == 0)
     return hipErrorInvalidConfiguration;
+  // 1024-thread blocks: one arrival per block and epoch on the counter
   const uint64_t blocks = std::min<uint64_t>((uint64_t)per_cu * num_cu,
-                                             (nnz + 255) / 256);
+                                             (nnz + 1023) / 1024);
   // its work counters go to their own slots (stats + 8)
   stats = stats ? stats + 8 : nullptr;
   void *args[] = {&emb,   &head,    &tail,    &nnz_p, &n,  &eps,
@@ -821,7 +1011,7 @@ static hipError_t try_stream(float *emb, const int32_t *head,
                   &lag};
   return hipLaunchCooperativeKernel(
       reinterpret_cast<const void *>(&k_sgd_stream<NC>),
-      dim3((unsigned)blocks), dim3(256), args, 0, s);
+      dim3((unsigned)blocks), dim3(1024), args, 0, s);
 }
 
 template <int NC>
@@ -835,6 +1025,15 @@ static hipError_t try_fused_any(float *emb, const int32_t *head,
                                 int lag, uint64_t nnz, int num_cu,
                                 hipStream_t s, int *variant) {
   *variant = 0;
+  {
+    // small edge sets: per-wave arrival slots, twice the counter kernels' lag
+    const hipError_t e =
+        try_waves<NC>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0, gamma,
+                      a, b, nsr_inv, seed, rounding_p, bar, stats, 2 * lag,
+                      nnz, num_cu, s);
+    if (e != hipErrorInvalidConfiguration)
+      return e;
+  }
 #define GM_TRY(EPT)                                                            \
   {                                                                            \
     const hipError_t e = try_fused<NC, EPT>(                                   \

@@ -174,7 +174,7 @@ struct Engine {
   uint8_t *wflags = nullptr;
   int32_t *head = nullptr, *tail = nullptr;
   int *deg = nullptr, *maxdeg = nullptr;
-  unsigned *gbar = nullptr; // grid barrier of the fused SGD: count, gen, timeout
+  unsigned *gbar = nullptr; // fused SGD: counter, -, timeout, -, wave slots
   int num_cu = 0;
   bool fused_used = false;
   uint32_t *dbatch = nullptr;
@@ -278,8 +278,8 @@ struct Engine {
     rounding = dalloc<float>(1);
     deg = dalloc<int>(N);
     maxdeg = dalloc<int>(1);
-    gbar = dalloc<unsigned>(4);
-    HIPC(hipMemsetAsync(gbar, 0, 4 * sizeof(unsigned), s));
+    gbar = dalloc<unsigned>(4 + SGD_MAX_WAVES);
+    HIPC(hipMemsetAsync(gbar, 0, (4 + SGD_MAX_WAVES) * sizeof(unsigned), s));
     HIPC(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount,
                                device));
     dbatch = dalloc<uint32_t>(N);
@@ -818,7 +818,7 @@ struct Engine {
     ev_start(GM_KSTAT_SGD);
     bool fused = false;
     if (!force_per_epoch) {
-      HIPC(hipMemsetAsync(gbar, 0, 4 * sizeof(unsigned), s));
+      HIPC(hipMemsetAsync(gbar, 0, (4 + SGD_MAX_WAVES) * sizeof(unsigned), s));
       const hipError_t fe = launch_sgd_fused(
           c, e, head, tail, scal + S_NNZ, (uint32_t)nrows, eps, nxt, nneg,
           n_epochs, alpha0, gamma, a, b, nsr_inv, seed, rounding, gbar, st,

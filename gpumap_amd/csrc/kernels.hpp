@@ -135,7 +135,9 @@ __global__ void k_sgd_fused(float *emb, const int32_t *head,
                             float nsr_inv, uint64_t seed,
                             const float *rounding_p, unsigned *bar,
                             unsigned long long *stats, int lag);
-// cooperative launch of k_sgd_fused or, for edge sets larger than its
+// arrival slots of k_sgd_waves after the 4 barrier words (bar + 4)
+constexpr unsigned SGD_MAX_WAVES = 512;
+// cooperative launch of k_sgd_waves, k_sgd_fused or, for edge sets larger than its
 // resident grid, k_sgd_stream (embed_kernels.hip; *variant 0 or 1, the
 // latter counting its work at stats + 8); returns the launch error when
 // neither can be launched
