@@ -204,6 +204,78 @@ __device__ inline int pool_offer_half(uint64_t *__restrict__ row,
   }
 }
 
+// ---------------------------------------------------------- rho / sigma
+// smooth_knn_dist of one row in FP64 (reference smooth_knn_dist_kernel,
+// embedding_kernel.cuh:135-207) by a 32-lane half-wave: lane c < k holds the
+// row's embed-time distance c (0 for the self column, the sqrt of the stored
+// squared distance for euclidean rows, 0 for an invalid entry).  rho = first
+// non-zero distance; sigma by 64-step bisection of
+// sum_{j=2..k-1} exp(-max(0, d_j - rho)/sigma) = log2(k-1), floored at 1e-3
+// times the row mean.  A row without a non-zero distance has rho = 0 and
+// sigma = -1: its sigma is 1e-3 times the global mean distance, which the
+// reader resolves (sigma_of), so the value stored here depends on the row's
+// public kNN list only.  Sums are xor-shuffle trees; the bisection's
+// decisions are uniform over the half-wave.  All 32 lanes must call this.
+__device__ inline void row_sigma_rho(double di, int c, int k, int half,
+                                     double &sigma_out, double &rho_out) {
+  double sum = di;
+#pragma unroll
+  for (int off = 16; off > 0; off >>= 1)
+    sum += __shfl_xor(sum, off, 32);
+  const uint32_t nzm =
+      (uint32_t)((__ballot(c < k && di > 0.0) >> (half * 32)) & 0xffffffffull);
+  if (nzm == 0) {
+    rho_out = 0.0;
+    sigma_out = -1.0;
+    return;
+  }
+  const double r = __shfl(di, __ffs(nzm) - 1, 32);
+  const double row_mean = sum / (double)k;
+  const double target = log2((double)(k - 1));
+  const bool term = c >= 2 && c < k;
+  const double x = di - r;
+  double lo = 0.0, hi = INFINITY, mid = 1.0;
+  for (int it = 0; it < 64; ++it) {
+    double psum = term ? ((x > 0.0) ? exp(-x / mid) : 1.0) : 0.0;
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1)
+      psum += __shfl_xor(psum, off, 32);
+    if (fabs(psum - target) < 1e-5)
+      break;
+    if (psum > target) {
+      hi = mid;
+      mid = (lo + hi) * 0.5;
+    } else {
+      lo = mid;
+      mid = isinf(hi) ? mid * 2.0 : (lo + hi) * 0.5;
+    }
+  }
+  if (!isfinite(mid) || mid <= 0.0)
+    mid = 1.0;
+  const double min_scale = 1e-3 * row_mean; // r > 0 here
+  rho_out = r;
+  sigma_out = mid < min_scale ? min_scale : mid;
+}
+
+// a row's sigma with the global-mean floor resolved
+__device__ inline double sigma_of(const double *sigma, uint64_t i,
+                                  double mean_dist) {
+  const double s = sigma[i];
+  return s < 0.0 ? 1e-3 * mean_dist : s;
+}
+
+// an embed-time distance from a stored squared distance (the reference's
+// sqrt at embed time, embedding_kernel.cuh:627-629): invalid entries
+// (sentinels) stay as they are
+__device__ inline bool valid_embed_d(double d) {
+  return isfinite(d) && d >= 0.0 && d < 3.402823466e38 / 2.0;
+}
+__device__ inline double embed_d(float v, bool euclid) {
+  if (!valid_embed_d((double)v))
+    return (double)v;
+  return euclid ? (double)sqrt_rn(fmaxf(v, 0.0f)) : (double)v;
+}
+
 // The same bounded insert by a single lane: read the W slots (relaxed loads,
 // possibly at different times), reject a duplicate id or a key not below the
 // maximum, else CAS the maximum slot (lowest index among ties) and retry on

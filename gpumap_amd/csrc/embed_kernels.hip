@@ -94,78 +94,31 @@ __global__ void k_mean_partial(const uint32_t *__restrict__ pidx,
   }
 }
 
-__global__ void k_mean_final(const double *__restrict__ psum,
-                             const unsigned long long *__restrict__ pcnt,
-                             int nparts, double *__restrict__ mean) {
-  if (threadIdx.x != 0 || blockIdx.x != 0)
-    return;
-  double s = 0.0;
-  unsigned long long c = 0;
-  for (int i = 0; i < nparts; ++i) {
-    s += psum[i];
-    c += pcnt[i];
+__global__ __launch_bounds__(256) void k_mean_final(
+    const double *__restrict__ psum, const unsigned long long *__restrict__ pcnt,
+    int nparts, double *__restrict__ mean) {
+  // nparts <= 256 partials, summed as a fixed tree
+  __shared__ double s_s[256];
+  __shared__ unsigned long long s_c[256];
+  const int t = threadIdx.x;
+  s_s[t] = t < nparts ? psum[t] : 0.0;
+  s_c[t] = t < nparts ? pcnt[t] : 0ull;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off) {
+      s_s[t] += s_s[t + off];
+      s_c[t] += s_c[t + off];
+    }
+    __syncthreads();
   }
-  *mean = (c > 0 && s > 0.0) ? s / (double)c : 1.0;
+  if (t == 0)
+    *mean = (s_c[0] > 0 && s_s[0] > 0.0) ? s_s[0] / (double)s_c[0] : 1.0;
 }
 
 // -------------------------------------------------------- rho / sigma
-// smooth_knn_dist per row in FP64 (reference smooth_knn_dist_kernel
-// :135-207): rho = first non-zero distance, sigma by 64-step bisection of
-// sum_{j=2..k-1} exp(-max(0, d_j - rho)/sigma) = log2(k-1), floor
-// 1e-3 * (row mean, or global mean when rho = 0).
-__global__ void k_sigma(const uint32_t *__restrict__ pidx,
-                        const float *__restrict__ pdist, uint64_t n, int k,
-                        int euclid, const double *__restrict__ mean_p,
-                        double *__restrict__ sigma, double *__restrict__ rho) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n)
-    return;
-  const double mean_dist = *mean_p;
-  double d[32];
-  for (int c = 0; c < k; ++c) {
-    uint32_t id;
-    embed_entry(pidx, pdist, i, c, k, euclid != 0, &id, &d[c]);
-  }
-  const double target = log2((double)(k - 1));
-  int start_nz = -1;
-  double sum = 0.0;
-  for (int c = 0; c < k; ++c) {
-    const double di = valid_d(d[c]) ? d[c] : 0.0;
-    sum += di;
-    if (di > 0.0 && start_nz == -1)
-      start_nz = c;
-  }
-  if (start_nz == -1) {
-    rho[i] = 0.0;
-    sigma[i] = 1e-3 * mean_dist;
-    return;
-  }
-  const double row_mean = sum / (double)k;
-  const double r = d[start_nz];
-  double lo = 0.0, hi = INFINITY, mid = 1.0;
-  for (int it = 0; it < 64; ++it) {
-    double psum = 0.0;
-    for (int j = 2; j < k; ++j) {
-      const double dj = valid_d(d[j]) ? d[j] : 0.0;
-      const double x = dj - r;
-      psum += (x > 0.0) ? exp(-x / mid) : 1.0;
-    }
-    if (fabs(psum - target) < 1e-5)
-      break;
-    if (psum > target) {
-      hi = mid;
-      mid = (lo + hi) * 0.5;
-    } else {
-      lo = mid;
-      mid = isinf(hi) ? mid * 2.0 : (lo + hi) * 0.5;
-    }
-  }
-  if (!isfinite(mid) || mid <= 0.0)
-    mid = 1.0;
-  const double min_scale = 1e-3 * (r > 0.0 ? row_mean : mean_dist);
-  rho[i] = r;
-  sigma[i] = mid < min_scale ? min_scale : mid;
-}
+// Each row's rho / sigma are computed when its public kNN row is written
+// (k_sync_rows, row_sigma_rho in common.hpp); a sigma < 0 stands for the
+// global-mean floor and is resolved here with sigma_of.
 
 // ------------------------------------------------------------ membership
 // Directed weights of the batch rows and the fuzzy union w = p + q - p q
@@ -177,10 +130,12 @@ __global__ void k_membership(const uint32_t *__restrict__ pidx,
                              const float *__restrict__ pdist, uint64_t n,
                              int k, int euclid, const double *__restrict__ sigma,
                              const double *__restrict__ rho,
+                             const double *__restrict__ mean_p,
                              const uint32_t *__restrict__ batch, uint64_t nb,
                              float *__restrict__ w, uint8_t *__restrict__ flags,
                              uint32_t *__restrict__ maxw_bits) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double mean_dist = *mean_p;
   float wf = 0.0f;
   if (t < nb * (uint64_t)k) {
     const uint64_t i = batch[t / k];
@@ -190,7 +145,7 @@ __global__ void k_membership(const uint32_t *__restrict__ pidx,
     embed_entry(pidx, pdist, i, c, k, euclid != 0, &j, &d);
     uint8_t f = 0;
     if (j < n && j != i && valid_d(d)) {
-      const double p = membership(d, rho[i], sigma[i]);
+      const double p = membership(d, rho[i], sigma_of(sigma, i, mean_dist));
       double q = 0.0;
       bool found = false;
       for (int cj = 0; cj < k; ++cj) {
@@ -199,7 +154,7 @@ __global__ void k_membership(const uint32_t *__restrict__ pidx,
         embed_entry(pidx, pdist, j, cj, k, euclid != 0, &jj, &dj);
         if (jj == i) {
           if (valid_d(dj)) {
-            q = membership(dj, rho[j], sigma[j]);
+            q = membership(dj, rho[j], sigma_of(sigma, j, mean_dist));
             found = true;
           }
           break;
@@ -314,16 +269,25 @@ __global__ void k_sgd_init(const float *__restrict__ w,
   atomicAdd(deg + head[e], 1);
 }
 
-__global__ void k_max_int(const int *__restrict__ v, uint64_t n,
-                          int *__restrict__ out) {
+__global__ __launch_bounds__(256) void k_max_int(const int *__restrict__ v,
+                                                 uint64_t n,
+                                                 int *__restrict__ out) {
+  __shared__ int s_m[4];
   int m = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
     m = max(m, v[i]);
   for (int off = 32; off > 0; off >>= 1)
     m = max(m, __shfl_xor(m, off));
-  if ((threadIdx.x & 63) == 0 && m > 0)
-    atomicMax(out, m);
+  if ((threadIdx.x & 63) == 0)
+    s_m[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) { // one atomic per block
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+      m = max(m, s_m[w]);
+    if (m > 0)
+      atomicMax(out, m);
+  }
 }
 
 __global__ void k_rounding(const int *__restrict__ max_deg, float alpha,
@@ -342,9 +306,10 @@ __global__ void k_rounding(const int *__restrict__ max_deg, float alpha,
 }
 
 // -------------------------------------------------------------- noise
-__global__ void k_minmax_col0(const float *__restrict__ emb, uint64_t n,
-                              int c, uint32_t *__restrict__ mn,
-                              uint32_t *__restrict__ mx) {
+__global__ __launch_bounds__(256) void k_minmax_col0(
+    const float *__restrict__ emb, uint64_t n, int c, uint32_t *__restrict__ mn,
+    uint32_t *__restrict__ mx) {
+  __shared__ uint32_t s_lo[4], s_hi[4];
   uint32_t lo = 0xffffffffu, hi = 0u;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
@@ -357,6 +322,15 @@ __global__ void k_minmax_col0(const float *__restrict__ emb, uint64_t n,
     hi = max(hi, __shfl_xor(hi, off));
   }
   if ((threadIdx.x & 63) == 0) {
+    s_lo[threadIdx.x >> 6] = lo;
+    s_hi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) { // one atomic pair per block
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      lo = min(lo, s_lo[w]);
+      hi = max(hi, s_hi[w]);
+    }
     atomicMin(mn, lo);
     atomicMax(mx, hi);
   }
@@ -1127,13 +1101,14 @@ __global__ void k_reposition(const uint32_t *__restrict__ pidx,
                              const float *__restrict__ pdist, uint64_t n,
                              int k, int euclid, const double *__restrict__ sigma,
                              const double *__restrict__ rho,
-                             const uint32_t *__restrict__ rows, uint64_t nr,
+                             const double *__restrict__ mean_p,
+                             const uint8_t *__restrict__ in_batch,
                              const float *__restrict__ snap,
                              float *__restrict__ emb, int c) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nr)
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || in_batch[i])
     return;
-  const uint64_t i = rows[t];
+  const double sig_i = sigma_of(sigma, i, *mean_p);
   double wsum = 0.0, acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int q = 0; q < k; ++q) {
     uint32_t j;
@@ -1141,14 +1116,27 @@ __global__ void k_reposition(const uint32_t *__restrict__ pidx,
     embed_entry(pidx, pdist, i, q, k, euclid != 0, &j, &d);
     if (j == i || j >= n || !valid_d(d))
       continue;
-    const double wq = membership(d, rho[i], sigma[i]);
+    const double wq = membership(d, rho[i], sig_i);
     wsum += wq;
-    for (int dd = 0; dd < c; ++dd)
-      acc[dd] += wq * snap[(uint64_t)j * c + dd];
+#pragma unroll
+    for (int dd = 0; dd < 4; ++dd)
+      if (dd < c)
+        acc[dd] += wq * snap[(uint64_t)j * c + dd];
   }
-  if (wsum > 0.0)
-    for (int dd = 0; dd < c; ++dd)
-      emb[i * c + dd] = (float)(acc[dd] / wsum);
+  if (wsum > 0.0) {
+#pragma unroll
+    for (int dd = 0; dd < 4; ++dd)
+      if (dd < c)
+        emb[i * c + dd] = (float)(acc[dd] / wsum);
+  }
+}
+
+// in_batch[batch[t]] = v for the op's batch rows
+__global__ void k_mark(const uint32_t *__restrict__ batch, uint64_t nb,
+                       uint8_t *__restrict__ in_batch, uint8_t v) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nb)
+    in_batch[batch[t]] = v;
 }
 
 } // namespace gm

@@ -178,6 +178,7 @@ struct Engine {
   int num_cu = 0;
   bool fused_used = false;
   uint32_t *dbatch = nullptr;
+  uint8_t *bmark = nullptr; // batch rows of the current embedding op
   // pinned host staging
   float *h_stage = nullptr;
   uint64_t h_stage_rows = 0;
@@ -283,6 +284,8 @@ struct Engine {
     HIPC(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount,
                                device));
     dbatch = dalloc<uint32_t>(N);
+    bmark = dalloc<uint8_t>(N);
+    HIPC(hipMemsetAsync(bmark, 0, N, s));
     HIPC(hipHostMalloc((void **)&h_list, N * sizeof(uint32_t)));
     HIPC(hipHostMalloc((void **)&h_scal, S_COUNT * sizeof(uint32_t) + 64));
     HIPC(hipStreamSynchronize(s));
@@ -355,7 +358,7 @@ struct Engine {
                     rev_old, sigma,  rho,    psum,    pcnt,       mean,
                     w,     wflags,  head,    tail,    wout,       eps,
                     nxt,   nneg,    snap,    rounding, deg,       maxdeg,
-                    dbatch};
+                    dbatch, bmark};
     for (void *q : ptrs)
       if (q)
         (void)(void)hipFree(q);
@@ -495,7 +498,8 @@ struct Engine {
                         hipMemcpyHostToDevice, s));
     HIPC(hipMemsetAsync(fresh, 0, sizeof(unsigned long long), s));
     k_sync_rows<<<blocks_for(rows_.size() * 32, 256), 256, 0, s>>>(
-        pool, rows, rows_.size(), pidx, pdist, k, fresh, pmax);
+        pool, rows, rows_.size(), pidx, pdist, k, fresh, pmax, euclid ? 1 : 0,
+        sigma, rho);
     LAUNCH_CHECK();
     unsigned long long f = 0;
     HIPC(hipMemcpyAsync(h_scal, fresh, sizeof(f), hipMemcpyDeviceToHost, s));
@@ -763,16 +767,13 @@ struct Engine {
     k_mean_partial<<<NPART, 256, 0, s>>>(pidx, pdist, n, k, euclid ? 1 : 0,
                                          psum, pcnt);
     LAUNCH_CHECK();
-    k_mean_final<<<1, 1, 0, s>>>(psum, pcnt, NPART, mean);
-    LAUNCH_CHECK();
-    k_sigma<<<blocks_for(n, 128), 128, 0, s>>>(pidx, pdist, n, k,
-                                               euclid ? 1 : 0, mean, sigma, rho);
+    k_mean_final<<<1, 256, 0, s>>>(psum, pcnt, NPART, mean);
     LAUNCH_CHECK();
     HIPC(hipMemsetAsync(scal + S_MAXW, 0, sizeof(uint32_t), s));
     const uint64_t nd = nb * (uint64_t)k;
     k_membership<<<blocks_for(nd, 256), 256, 0, s>>>(
-        pidx, pdist, n, k, euclid ? 1 : 0, sigma, rho, dbatch, nb, w, wflags,
-        scal + S_MAXW);
+        pidx, pdist, n, k, euclid ? 1 : 0, sigma, rho, mean, dbatch, nb, w,
+        wflags, scal + S_MAXW);
     LAUNCH_CHECK();
     const unsigned nblk = blocks_for(2 * nd, 1024);
     k_coo_count<<<nblk, 256, 0, s>>>(w, wflags, nd, scal + S_MAXW,
@@ -808,7 +809,7 @@ struct Engine {
         wout, head, scal + S_NNZ, scal + S_MAXW, n_epochs, nsr_inv, eps, nxt,
         nneg, deg);
     LAUNCH_CHECK();
-    k_max_int<<<std::min<unsigned>(blocks_for(nrows, 256), 1024), 256, 0, s>>>(
+    k_max_int<<<std::min<unsigned>(blocks_for(nrows, 256), 64), 256, 0, s>>>(
         deg, nrows, maxdeg);
     LAUNCH_CHECK();
     k_rounding<<<1, 1, 0, s>>>(maxdeg, alpha0, rounding);
@@ -914,15 +915,6 @@ struct Engine {
       }
     }
     const uint64_t nb = batch.size();
-    std::vector<uint32_t> unsel;
-    if (nb < n) {
-      std::vector<uint8_t> in_batch(n, 0);
-      for (uint32_t id : batch)
-        in_batch[id] = 1;
-      for (uint64_t i = 0; i < n; ++i)
-        if (!in_batch[i])
-          unsel.push_back((uint32_t)i);
-    }
     std::memcpy(h_list, batch.data(), nb * sizeof(uint32_t));
     HIPC(hipMemcpyAsync(dbatch, h_list, nb * sizeof(uint32_t),
                         hipMemcpyHostToDevice, s));
@@ -935,7 +927,7 @@ struct Engine {
     h_scal[9] = 0u;
     HIPC(hipMemcpyAsync(scal + S_MIN, h_scal + 8, 2 * sizeof(uint32_t),
                         hipMemcpyHostToDevice, s));
-    k_minmax_col0<<<std::min<unsigned>(blocks_for(n, 256), 1024), 256, 0, s>>>(
+    k_minmax_col0<<<std::min<unsigned>(blocks_for(n, 256), 64), 256, 0, s>>>(
         emb, n, c, scal + S_MIN, scal + S_MAX);
     LAUNCH_CHECK();
     k_noise<<<blocks_for(nb * c, 256), 256, 0, s>>>(
@@ -949,17 +941,20 @@ struct Engine {
     run_sgd(emb, n, nnz_now, (int)p.max_epoch,
             (int)p.negative_sample_rate, p.a, p.b, p.alpha, 1.0f, 42ull);
 
-    // reposition of unselected rows (heuristic_neighbor_avg_kernel :811-894)
-    if (!unsel.empty()) {
-      HIPC(hipStreamSynchronize(s)); // h_list still feeds the batch upload
-      std::memcpy(h_list, unsel.data(), unsel.size() * sizeof(uint32_t));
-      HIPC(hipMemcpyAsync(rows, h_list, unsel.size() * sizeof(uint32_t),
-                          hipMemcpyHostToDevice, s));
+    // reposition of the rows outside the batch (reference
+    // heuristic_neighbor_avg_kernel :811-894, impl.cuh:422-449), from a
+    // snapshot so that no row reads a neighbour's new value; the batch rows
+    // are marked on the device
+    if (nb < n) {
+      k_mark<<<blocks_for(nb, 256), 256, 0, s>>>(dbatch, nb, bmark, 1);
+      LAUNCH_CHECK();
       HIPC(hipMemcpyAsync(snap, emb, n * c * sizeof(float),
                           hipMemcpyDeviceToDevice, s));
-      k_reposition<<<blocks_for(unsel.size(), 128), 128, 0, s>>>(
-          pidx, pdist, n, k, euclid ? 1 : 0, sigma, rho, rows, unsel.size(),
-          snap, emb, c);
+      k_reposition<<<blocks_for(n, 128), 128, 0, s>>>(
+          pidx, pdist, n, k, euclid ? 1 : 0, sigma, rho, mean, bmark, snap,
+          emb, c);
+      LAUNCH_CHECK();
+      k_mark<<<blocks_for(nb, 256), 256, 0, s>>>(dbatch, nb, bmark, 0);
       LAUNCH_CHECK();
     }
     return (uint32_t)nb;
@@ -1369,7 +1364,14 @@ int gm_debug_fuzzy(gm_handle *h, const uint32_t *batch, uint64_t nb,
       HIPC(hipMemcpyAsync(sigma, e->sigma, e->n * 8, hipMemcpyDeviceToHost, e->s));
     if (rho)
       HIPC(hipMemcpyAsync(rho, e->rho, e->n * 8, hipMemcpyDeviceToHost, e->s));
+    double mean = 0.0;
+    HIPC(hipMemcpyAsync(&mean, e->mean, 8, hipMemcpyDeviceToHost, e->s));
     HIPC(hipStreamSynchronize(e->s));
+    // a stored sigma < 0 stands for the global-mean floor
+    if (sigma)
+      for (uint64_t i = 0; i < e->n; ++i)
+        if (sigma[i] < 0.0)
+          sigma[i] = 1e-3 * mean;
     *nnz = cnt;
   });
 }

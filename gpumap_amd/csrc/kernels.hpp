@@ -42,7 +42,8 @@ __global__ void k_join_reset(const uint32_t *owners,
                              int *foc);
 __global__ void k_sync_rows(const uint64_t *pool, const uint32_t *rows,
                             uint64_t nrows, uint32_t *pidx, float *pdist,
-                            int k, unsigned long long *fresh, float *pmax);
+                            int k, unsigned long long *fresh, float *pmax,
+                            int euclid, double *sigma, double *rho);
 __global__ void k_flag_count(const uint8_t *flag, uint64_t lo, uint64_t hi,
                              uint32_t *bcount);
 __global__ void k_scan_small(uint32_t *v, uint64_t n, uint32_t *total);
@@ -93,14 +94,12 @@ __global__ void k_mean_partial(const uint32_t *pidx, const float *pdist,
                                unsigned long long *pcnt);
 __global__ void k_mean_final(const double *psum, const unsigned long long *pcnt,
                              int nparts, double *mean);
-__global__ void k_sigma(const uint32_t *pidx, const float *pdist, uint64_t n,
-                        int k, int euclid, const double *mean_p, double *sigma,
-                        double *rho);
 __global__ void k_membership(const uint32_t *pidx, const float *pdist,
                              uint64_t n, int k, int euclid,
                              const double *sigma, const double *rho,
-                             const uint32_t *batch, uint64_t nb, float *w,
-                             uint8_t *flags, uint32_t *maxw_bits);
+                             const double *mean, const uint32_t *batch,
+                             uint64_t nb, float *w, uint8_t *flags,
+                             uint32_t *maxw_bits);
 __global__ void k_coo_count(const float *w, const uint8_t *flags, uint64_t nd,
                             const uint32_t *maxw_bits, int n_epochs,
                             uint32_t *bcount);
@@ -152,7 +151,10 @@ hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
                             int *variant);
 __global__ void k_reposition(const uint32_t *pidx, const float *pdist,
                              uint64_t n, int k, int euclid, const double *sigma,
-                             const double *rho, const uint32_t *rows,
-                             uint64_t nr, const float *snap, float *emb, int c);
+                             const double *rho, const double *mean,
+                             const uint8_t *in_batch, const float *snap,
+                             float *emb, int c);
+__global__ void k_mark(const uint32_t *batch, uint64_t nb, uint8_t *in_batch,
+                       uint8_t v);
 
 } // namespace gm

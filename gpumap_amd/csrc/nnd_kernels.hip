@@ -654,35 +654,59 @@ __global__ void k_join_reset(const uint32_t *__restrict__ owners,
 // (reference stage_and_scatter_knn_rows, matrix_utils.cuh:127-174, without
 // the host staging).  Counts the ids that were not in the previous row and
 // records the pool's maximum distance.
-__global__ void k_sync_rows(const uint64_t *__restrict__ pool,
-                            const uint32_t *__restrict__ rows, uint64_t nrows,
-                            uint32_t *__restrict__ pidx,
-                            float *__restrict__ pdist, int k,
-                            unsigned long long *__restrict__ fresh,
-                            float *__restrict__ pmax) {
+// Public top-k rows of the listed rows from their pools (one half-wave per
+// row), the fresh-edge count, the pool maximum (infinite while the pool has
+// an empty slot, the reverse filter of exact insertion), and the row's
+// sigma / rho for the fuzzy set, which depend on the public row only.
+__global__ __launch_bounds__(256) void k_sync_rows(
+    const uint64_t *__restrict__ pool, const uint32_t *__restrict__ rows,
+    uint64_t nrows, uint32_t *__restrict__ pidx, float *__restrict__ pdist,
+    int k, unsigned long long *__restrict__ fresh, float *__restrict__ pmax,
+    int euclid, double *__restrict__ sigma, double *__restrict__ rho) {
+  __shared__ unsigned long long s_fresh;
+  if (threadIdx.x == 0)
+    s_fresh = 0;
+  __syncthreads();
   const uint64_t g = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
   const int hl = threadIdx.x & 31;
-  if (g >= nrows)
-    return;
-  const uint32_t u = rows[g];
-  const uint64_t key = half_sort32(pool[(uint64_t)u * W + hl]);
-  const uint32_t old = hl < k ? pidx[(uint64_t)u * k + hl] : NO_ID;
-  const uint32_t id = key == EMPTY ? NO_ID : key_id(key);
-  bool seen = false;
-  for (int q = 0; q < k; ++q)
-    seen |= (__shfl(old, q, 32) == id);
-  const bool is_fresh = hl < k && id != NO_ID && !seen;
-  const uint64_t fm = (__ballot(is_fresh) >> (threadIdx.x & 32)) & 0xffffffffull;
-  if (hl < k) {
-    pidx[(uint64_t)u * k + hl] = id;
-    pdist[(uint64_t)u * k + hl] = key == EMPTY ? 3.402823466e38f : key_dist(key);
+  const int half = (threadIdx.x >> 5) & 1;
+  if (g < nrows) { // uniform over the half-wave
+    const uint32_t u = rows[g];
+    const uint64_t key = half_sort32(pool[(uint64_t)u * W + hl]);
+    const uint32_t old = hl < k ? pidx[(uint64_t)u * k + hl] : NO_ID;
+    const uint32_t id = key == EMPTY ? NO_ID : key_id(key);
+    bool seen = false;
+    for (int q = 0; q < k; ++q)
+      seen |= (__shfl(old, q, 32) == id);
+    const bool is_fresh = hl < k && id != NO_ID && !seen;
+    const uint64_t fm =
+        (__ballot(is_fresh) >> (threadIdx.x & 32)) & 0xffffffffull;
+    const float dv = key == EMPTY ? 3.402823466e38f : key_dist(key);
+    if (hl < k) {
+      pidx[(uint64_t)u * k + hl] = id;
+      pdist[(uint64_t)u * k + hl] = dv;
+    }
+    if (hl == 0 && fm)
+      atomicAdd(&s_fresh, (unsigned long long)__popcll(fm));
+    if (hl == 31)
+      pmax[u] = key == EMPTY ? INFINITY : key_dist(key);
+    // embed-time entry c: the self column (0), then public slot c - 1
+    const float vs = __shfl(dv, hl == 0 ? 0 : hl - 1, 32);
+    double di = 0.0;
+    if (hl > 0 && hl < k) {
+      const double d = embed_d(vs, euclid != 0);
+      di = valid_embed_d(d) ? d : 0.0;
+    }
+    double sg, rh;
+    row_sigma_rho(di, hl, k, half, sg, rh);
+    if (hl == 0) {
+      sigma[u] = sg;
+      rho[u] = rh;
+    }
   }
-  if (hl == 0 && fm)
-    atomicAdd(fresh, (unsigned long long)__popcll(fm));
-  // pool maximum (infinite while the pool has an empty slot), the reverse
-  // filter of exact insertion
-  if (hl == 31)
-    pmax[u] = key == EMPTY ? INFINITY : key_dist(key);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_fresh)
+    atomicAdd(fresh, s_fresh);
 }
 
 // ------------------------------------------------------------ compaction
