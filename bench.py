@@ -96,7 +96,10 @@ def kernel_numbers(e, capi, d, c):
         # of 4c: 24 + 36c bytes
         bytes_ = pos_ * (24 + 36 * c)
         ms = ss.device_ms / ss.launches
-        out["k_sgd_fused"] = {
+        waves = int(ss.work[3])
+        name = ("k_sgd_waves" if waves == ss.launches else
+                "k_sgd_fused" if waves == 0 else "k_sgd_waves+k_sgd_fused")
+        out[name] = {
             "launches": int(ss.launches), "avg_ms": ms,
             "positive_samples": int(pos_), "negative_samples": int(neg_),
             "algo_bytes_per_launch": bytes_ / ss.launches,

@@ -96,7 +96,7 @@ __global__ void k_mean_partial(const uint32_t *__restrict__ pidx,
 
 __global__ __launch_bounds__(256) void k_mean_final(
     const double *__restrict__ psum, const unsigned long long *__restrict__ pcnt,
-    int nparts, double *__restrict__ mean) {
+    int nparts, double *__restrict__ mean, uint32_t *__restrict__ reset3) {
   // nparts <= 256 partials, summed as a fixed tree
   __shared__ double s_s[256];
   __shared__ unsigned long long s_c[256];
@@ -111,8 +111,16 @@ __global__ __launch_bounds__(256) void k_mean_final(
     }
     __syncthreads();
   }
-  if (t == 0)
+  if (t == 0) {
     *mean = (s_c[0] > 0 && s_s[0] > 0.0) ? s_s[0] / (double)s_c[0] : 1.0;
+    // the accumulators of the kernels that follow: maximum weight, then the
+    // minimum and maximum of embedding column 0 (ordered-uint encodings)
+    if (reset3) {
+      reset3[0] = 0u;
+      reset3[1] = 0xffffffffu;
+      reset3[2] = 0u;
+    }
+  }
 }
 
 // -------------------------------------------------------- rho / sigma
@@ -256,8 +264,18 @@ __global__ void k_sgd_init(const float *__restrict__ w,
                            const uint32_t *__restrict__ maxw_bits,
                            int n_epochs, float nsr_inv,
                            float *__restrict__ eps, float *__restrict__ nxt,
-                           float *__restrict__ nneg, int *__restrict__ deg) {
+                           float *__restrict__ nneg, int *__restrict__ deg,
+                           int *__restrict__ maxdeg, unsigned *__restrict__ bar,
+                           unsigned nbar) {
   const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // block 0 clears what the next kernels accumulate into: the maximum
+  // degree (k_max_int) and the cooperative SGD's barrier words
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0)
+      *maxdeg = 0;
+    for (unsigned i = threadIdx.x; i < nbar; i += blockDim.x)
+      bar[i] = 0u;
+  }
   if (e >= *nnz_p)
     return;
   const float mw = __uint_as_float(*maxw_bits);
@@ -269,10 +287,28 @@ __global__ void k_sgd_init(const float *__restrict__ w,
   atomicAdd(deg + head[e], 1);
 }
 
+__device__ inline float rounding_of(int md, float alpha) {
+  if (md == 0)
+    return 1.0f;
+  const float max_abs = (float)md * 4.0f * fabsf(alpha);
+  const float delta =
+      div_rn(max_abs, 1.0f - 2.0f * (float)md * 1.1920928955078125e-7f);
+  int e;
+  frexpf(delta, &e);
+  return ldexpf(1.0f, e);
+}
+
+// Maximum head degree, and from it the gradient rounding grid (reference
+// :405-460): the last block to finish reads the maximum and writes the grid.
+// `done` counts finished blocks and is left at 0 for the next launch.
 __global__ __launch_bounds__(256) void k_max_int(const int *__restrict__ v,
                                                  uint64_t n,
-                                                 int *__restrict__ out) {
+                                                 int *__restrict__ out,
+                                                 float alpha,
+                                                 float *__restrict__ rounding,
+                                                 unsigned *__restrict__ done) {
   __shared__ int s_m[4];
+  __shared__ bool s_last;
   int m = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -287,32 +323,29 @@ __global__ __launch_bounds__(256) void k_max_int(const int *__restrict__ v,
       m = max(m, s_m[w]);
     if (m > 0)
       atomicMax(out, m);
+    __threadfence();
+    s_last = atomicAdd(done, 1u) == gridDim.x - 1;
+    if (s_last) {
+      __threadfence();
+      const int md = __hip_atomic_load(out, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+      *rounding = rounding_of(md, alpha);
+      *done = 0u;
+    }
   }
-}
-
-__global__ void k_rounding(const int *__restrict__ max_deg, float alpha,
-                           float *__restrict__ rounding) {
-  const int md = *max_deg;
-  if (md == 0) {
-    *rounding = 1.0f;
-    return;
-  }
-  const float max_abs = (float)md * 4.0f * fabsf(alpha);
-  const float delta =
-      div_rn(max_abs, 1.0f - 2.0f * (float)md * 1.1920928955078125e-7f);
-  int e;
-  frexpf(delta, &e);
-  *rounding = ldexpf(1.0f, e);
 }
 
 // -------------------------------------------------------------- noise
 __global__ __launch_bounds__(256) void k_minmax_col0(
     const float *__restrict__ emb, uint64_t n, int c, uint32_t *__restrict__ mn,
-    uint32_t *__restrict__ mx) {
+    uint32_t *__restrict__ mx, int *__restrict__ deg_zero) {
   __shared__ uint32_t s_lo[4], s_hi[4];
   uint32_t lo = 0xffffffffu, hi = 0u;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
+    // the per-row head degrees of the SGD set-up start from zero
+    if (deg_zero)
+      deg_zero[i] = 0;
     const uint32_t o = f2o(emb[i * c]);
     lo = min(lo, o);
     hi = max(hi, o);
@@ -342,13 +375,17 @@ __global__ __launch_bounds__(256) void k_noise(float *__restrict__ emb, int c,
                         const uint32_t *__restrict__ batch, uint64_t nb,
                         const uint32_t *__restrict__ mn,
                         const uint32_t *__restrict__ mx, float noise_scale,
-                        uint64_t seed) {
+                        uint64_t seed, uint8_t *__restrict__ in_batch) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nb * (uint64_t)c)
     return;
   const float scale = noise_scale * (o2f(*mx) - o2f(*mn));
   Rng rng(seed, (uint32_t)t, (uint32_t)(t >> 32), TAG_NOISE);
-  emb[(uint64_t)batch[t / c] * c + t % c] += rng.normal() * scale;
+  const uint32_t row = batch[t / c];
+  emb[(uint64_t)row * c + t % c] += rng.normal() * scale;
+  // batch rows are marked for k_reposition, which clears the marks
+  if (in_batch && t % c == 0)
+    in_batch[row] = 1;
 }
 
 // --------------------------------------------------------------- SGD
@@ -998,7 +1035,7 @@ static hipError_t try_fused_any(float *emb, const int32_t *head,
                                 unsigned *bar, unsigned long long *stats,
                                 int lag, uint64_t nnz, int num_cu,
                                 hipStream_t s, int *variant) {
-  *variant = 0;
+  *variant = 2;
   {
     // small edge sets: per-wave arrival slots, twice the counter kernels' lag
     const hipError_t e =
@@ -1008,6 +1045,7 @@ static hipError_t try_fused_any(float *emb, const int32_t *head,
     if (e != hipErrorInvalidConfiguration)
       return e;
   }
+  *variant = 0;
 #define GM_TRY(EPT)                                                            \
   {                                                                            \
     const hipError_t e = try_fused<NC, EPT>(                                   \
@@ -1102,12 +1140,16 @@ __global__ void k_reposition(const uint32_t *__restrict__ pidx,
                              int k, int euclid, const double *__restrict__ sigma,
                              const double *__restrict__ rho,
                              const double *__restrict__ mean_p,
-                             const uint8_t *__restrict__ in_batch,
+                             uint8_t *__restrict__ in_batch,
                              const float *__restrict__ snap,
                              float *__restrict__ emb, int c) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || in_batch[i])
+  if (i >= n)
     return;
+  if (in_batch[i]) { // a batch row: keeps its place; clear its mark
+    in_batch[i] = 0;
+    return;
+  }
   const double sig_i = sigma_of(sigma, i, *mean_p);
   double wsum = 0.0, acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int q = 0; q < k; ++q) {
@@ -1131,12 +1173,13 @@ __global__ void k_reposition(const uint32_t *__restrict__ pidx,
   }
 }
 
-// in_batch[batch[t]] = v for the op's batch rows
-__global__ void k_mark(const uint32_t *__restrict__ batch, uint64_t nb,
-                       uint8_t *__restrict__ in_batch, uint8_t v) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < nb)
-    in_batch[batch[t]] = v;
+// ctr += nnz * n_epochs: the edge-epochs an SGD launch scans, counted on the
+// device so that measuring needs no read-back of the edge count
+__global__ void k_add_edge_epochs(const uint32_t *__restrict__ nnz_p,
+                                  int n_epochs,
+                                  unsigned long long *__restrict__ ctr) {
+  if (threadIdx.x == 0)
+    atomicAdd(ctr, (unsigned long long)*nnz_p * (unsigned long long)n_epochs);
 }
 
 } // namespace gm

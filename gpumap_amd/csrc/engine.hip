@@ -62,23 +62,26 @@ inline unsigned blocks_for(uint64_t n, unsigned tpb) {
   return (unsigned)((n + tpb - 1) / tpb);
 }
 
-struct DedupQueue { // reference UpdateQueue (queue.cuh:46-80)
-  std::queue<uint32_t> q;
-  std::unordered_set<uint32_t> in;
+// reference UpdateQueue (queue.cuh:46-80): FIFO without duplicates; the
+// membership test is a byte per row (rows are < the session's capacity)
+struct DedupQueue {
+  std::deque<uint32_t> q;
+  std::vector<uint8_t> in;
   size_t size() const { return q.size(); }
   void push(uint32_t v) {
-    if (in.insert(v).second)
-      q.push(v);
+    if (v >= in.size())
+      in.resize(std::max<size_t>((size_t)v + 1, 2 * in.size()), 0);
+    if (!in[v]) {
+      in[v] = 1;
+      q.push_back(v);
+    }
   }
   std::vector<uint32_t> pop_n(size_t n) {
-    std::vector<uint32_t> out;
     n = std::min(n, q.size());
-    out.reserve(n);
-    for (size_t i = 0; i < n; ++i) {
-      out.push_back(q.front());
-      in.erase(q.front());
-      q.pop();
-    }
+    std::vector<uint32_t> out(q.begin(), q.begin() + n);
+    q.erase(q.begin(), q.begin() + n);
+    for (uint32_t v : out)
+      in[v] = 0;
     return out;
   }
 };
@@ -153,9 +156,15 @@ struct Engine {
     uint64_t launches;
   };
   std::vector<Timed> pending;
+  std::vector<hipEvent_t> ev_free; // recorded and folded, ready for reuse
   gm_kernel_stats totals[GM_KSTAT_COUNT] = {};
-  uint64_t sgd_edge_epochs = 0, sgd_stream_edge_epochs = 0;
-  int sgd_variant = 0; // last cooperative SGD launch: 0 resident, 1 stream
+  // device counters kstats[13], kstats[14]: edge-epochs scanned by the
+  // resident and by the streaming cooperative SGD kernels
+  static constexpr int KS_SGD_EE = 13, KS_STREAM_EE = 14;
+  uint64_t sgd_waves_launches = 0;
+  // last cooperative SGD launch: 0 counter-paced resident grid, 1 stream,
+  // 2 per-wave arrival slots
+  int sgd_variant = 0;
   // insertion scratch
   uint64_t ins_cap = 0;
   uint32_t *ins_ids = nullptr, *cand = nullptr;
@@ -184,6 +193,15 @@ struct Engine {
   uint64_t h_stage_rows = 0;
   uint32_t *h_list = nullptr;
   uint32_t *h_scal = nullptr;
+  // rows flagged by the last insertion (ascending), copied back behind the
+  // kernels: a prefix of flag_prefix rows and the count; the host reads them
+  // once the stream has caught up (insert_finish)
+  uint32_t *h_flagged = nullptr;
+  uint64_t flag_prefix = 1024;
+  struct PendingInsert {
+    bool on = false;
+    uint64_t b0 = 0, B = 0, pre = 0;
+  } pins;
 
   // host state (reference Impl members, impl.cuh:33-55)
   uint64_t n = 0;
@@ -205,6 +223,7 @@ struct Engine {
     c = (int)p.n_components;
     euclid = p.metric == GM_EUCLIDEAN;
     HIPC(hipGetDevice(&device));
+    update_queue.in.assign(N, 0);
     try {
       allocate();
     } catch (...) {
@@ -288,6 +307,7 @@ struct Engine {
     HIPC(hipMemsetAsync(bmark, 0, N, s));
     HIPC(hipHostMalloc((void **)&h_list, N * sizeof(uint32_t)));
     HIPC(hipHostMalloc((void **)&h_scal, S_COUNT * sizeof(uint32_t) + 64));
+    HIPC(hipHostMalloc((void **)&h_flagged, (N + 1) * sizeof(uint32_t)));
     HIPC(hipStreamSynchronize(s));
   }
 
@@ -330,6 +350,9 @@ struct Engine {
       (void)hipEventDestroy(t.b);
     }
     pending.clear();
+    for (hipEvent_t ev : ev_free)
+      (void)hipEventDestroy(ev);
+    ev_free.clear();
     if (kstats)
       (void)hipFree(kstats);
     kstats = nullptr;
@@ -368,22 +391,33 @@ struct Engine {
       (void)hipHostFree(h_list);
     if (h_scal)
       (void)hipHostFree(h_scal);
+    if (h_flagged)
+      (void)hipHostFree(h_flagged);
     h_stage = nullptr;
     h_list = nullptr;
     h_scal = nullptr;
+    h_flagged = nullptr;
     if (s)
       (void)hipStreamDestroy(s);
     s = nullptr;
   }
 
+  hipEvent_t ev_get() {
+    if (!ev_free.empty()) {
+      hipEvent_t ev = ev_free.back();
+      ev_free.pop_back();
+      return ev;
+    }
+    hipEvent_t ev = nullptr;
+    HIPC(hipEventCreate(&ev));
+    return ev;
+  }
   hipEvent_t ev_start(int which) {
     if (!prof)
       return nullptr;
-    Timed t{which, nullptr, nullptr, 0};
-    HIPC(hipEventCreate(&t.a));
-    HIPC(hipEventCreate(&t.b));
-    HIPC(hipEventRecord(t.a, s));
+    Timed t{which, ev_get(), ev_get(), 0};
     pending.push_back(t);
+    HIPC(hipEventRecord(t.a, s));
     return t.a;
   }
   void ev_stop(uint64_t launches) {
@@ -393,20 +427,24 @@ struct Engine {
     t.launches = launches;
     HIPC(hipEventRecord(t.b, s));
   }
-  // fold finished events and the device counters into the totals
-  void collect_stats() {
-    if (!prof)
-      return;
-    HIPC(hipStreamSynchronize(s));
+  // fold recorded event pairs into the totals (after run()'s own stream
+  // synchronisation they have completed, so this waits for nothing)
+  void fold_events() {
     for (auto &t : pending) {
+      HIPC(hipEventSynchronize(t.b));
       float ms = 0.0f;
       HIPC(hipEventElapsedTime(&ms, t.a, t.b));
       totals[t.which].launches += t.launches;
       totals[t.which].device_ms += ms;
-      (void)hipEventDestroy(t.a);
-      (void)hipEventDestroy(t.b);
+      ev_free.push_back(t.a);
+      ev_free.push_back(t.b);
     }
     pending.clear();
+  }
+  // the device work counters into the totals (one read-back, on request)
+  void read_counters() {
+    HIPC(hipStreamSynchronize(s));
+    fold_events();
     unsigned long long c[16];
     HIPC(hipMemcpy(c, kstats, sizeof(c), hipMemcpyDeviceToHost));
     totals[GM_KSTAT_JOIN].work[0] = c[0];
@@ -414,10 +452,11 @@ struct Engine {
     totals[GM_KSTAT_JOIN].work[2] = c[2];
     totals[GM_KSTAT_SGD].work[0] = c[3];
     totals[GM_KSTAT_SGD].work[1] = c[4];
-    totals[GM_KSTAT_SGD].work[2] = sgd_edge_epochs;
+    totals[GM_KSTAT_SGD].work[2] = c[KS_SGD_EE];
+    totals[GM_KSTAT_SGD].work[3] = sgd_waves_launches;
     totals[GM_KSTAT_SGD_STREAM].work[0] = c[11];
     totals[GM_KSTAT_SGD_STREAM].work[1] = c[12];
-    totals[GM_KSTAT_SGD_STREAM].work[2] = sgd_stream_edge_epochs;
+    totals[GM_KSTAT_SGD_STREAM].work[2] = c[KS_STREAM_EE];
     totals[GM_KSTAT_INSERT].work[0] = ins_pairs;
     totals[GM_KSTAT_INSERT].work[1] = c[5];
     totals[GM_KSTAT_INSERT].work[2] = c[6];
@@ -464,11 +503,9 @@ struct Engine {
     n += nr;
   }
 
-  // ascending rows with flag set in [0, hi); clears the flags
-  uint64_t collect_flags(uint64_t hi, std::vector<uint32_t> &out) {
-    out.clear();
-    if (hi == 0)
-      return 0;
+  // ascending rows with flag set in [0, hi) into `list`, their count into
+  // scal[S_TOTAL]; clears the flags
+  void compact_flags(uint64_t hi) {
     const unsigned nblk = blocks_for(hi, 1024);
     k_flag_count<<<nblk, 256, 0, s>>>(flag, 0, hi, bcount);
     LAUNCH_CHECK();
@@ -476,6 +513,14 @@ struct Engine {
     LAUNCH_CHECK();
     k_flag_write<<<nblk, 64, 0, s>>>(flag, 0, hi, bcount, list);
     LAUNCH_CHECK();
+  }
+
+  // compact_flags with the list copied to the host
+  uint64_t collect_flags(uint64_t hi, std::vector<uint32_t> &out) {
+    out.clear();
+    if (hi == 0)
+      return 0;
+    compact_flags(hi);
     HIPC(hipMemcpyAsync(h_scal, scal + S_TOTAL, sizeof(uint32_t),
                         hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
@@ -498,8 +543,8 @@ struct Engine {
                         hipMemcpyHostToDevice, s));
     HIPC(hipMemsetAsync(fresh, 0, sizeof(unsigned long long), s));
     k_sync_rows<<<blocks_for(rows_.size() * 32, 256), 256, 0, s>>>(
-        pool, rows, rows_.size(), pidx, pdist, k, fresh, pmax, euclid ? 1 : 0,
-        sigma, rho);
+        pool, rows, rows_.size(), nullptr, pidx, pdist, k, fresh, pmax,
+        euclid ? 1 : 0, sigma, rho);
     LAUNCH_CHECK();
     unsigned long long f = 0;
     HIPC(hipMemcpyAsync(h_scal, fresh, sizeof(f), hipMemcpyDeviceToHost, s));
@@ -510,7 +555,10 @@ struct Engine {
 
   // ---------------------------------------------------------- insertion
   // reference Impl::insert (impl.cuh:220-253) + run_insertion
-  // (insertion_kernel.cuh:926-1095), graph kept on the device.
+  // (insertion_kernel.cuh:926-1095), graph kept on the device.  The rows to
+  // publish (the batch and every earlier row whose pool changed) are found
+  // and synced on the device; the host learns them for its update queue
+  // from a copy made behind the kernels (insert_finish).
   uint32_t insert(uint32_t n_ops) {
     if (insertion_queue.size() == 0 || n_ops == 0)
       return 0;
@@ -529,19 +577,53 @@ struct Engine {
       insert_nndescent(b0, B, window_start, seed);
     else
       insert_exact(b0, B);
-    std::vector<uint32_t> flagged;
-    collect_flags(n, flagged);
-    std::vector<uint32_t> affected(batch.begin(), batch.end());
-    for (uint32_t r : flagged)
-      if (r < b0)
-        affected.push_back(r);
-    sync_rows(affected);
+    // the batch rows are published whether or not their pools were flagged:
+    // the compacted list is the earlier flagged rows (ascending), then
+    // b0 .. b0 + B - 1 (the batch rows are the newest rows, nothing after
+    // them has a pool yet)
+    HIPC(hipMemsetAsync(flag + b0, 1, B, s));
+    compact_flags(n);
+    const unsigned sgrid = (unsigned)std::min<uint64_t>(
+        blocks_for(n * 32, 256), 8 * (uint64_t)num_cu);
+    k_sync_rows<<<sgrid, 256, 0, s>>>(pool, list, n, scal + S_TOTAL, pidx,
+                                      pdist, k, nullptr, pmax, euclid ? 1 : 0,
+                                      sigma, rho);
+    LAUNCH_CHECK();
     k_init_embedding<<<blocks_for(B, 64), 64, 0, s>>>(emb, c, pidx, pdist, k,
                                                       b0, B);
     LAUNCH_CHECK();
-    for (uint32_t r : affected)
-      update_queue.push(r);
+    const uint64_t pre = std::min<uint64_t>(flag_prefix, n);
+    HIPC(hipMemcpyAsync(h_flagged + N, scal + S_TOTAL, sizeof(uint32_t),
+                        hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(h_flagged, list, pre * sizeof(uint32_t),
+                        hipMemcpyDeviceToHost, s));
+    pins = PendingInsert{true, b0, B, pre};
     return (uint32_t)B;
+  }
+
+  // Host side of the last insertion once the stream has caught up: the
+  // batch rows, then the earlier rows whose pools changed (ascending), go to
+  // the update queue, as the reference's insert pushes them.
+  void insert_finish() {
+    if (!pins.on)
+      return;
+    pins.on = false;
+    const uint64_t cnt = h_flagged[N];
+    if (cnt < pins.B)
+      throw Fail(GM_ERR_HIP, "insertion published fewer rows than its batch");
+    const uint64_t nold = cnt - pins.B;
+    if (nold > pins.pre) {
+      HIPC(hipMemcpyAsync(h_flagged + pins.pre, list + pins.pre,
+                          (nold - pins.pre) * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, s));
+      HIPC(hipStreamSynchronize(s));
+    }
+    // next prefix: room for half again as many rows as this one needed
+    flag_prefix = std::max<uint64_t>(1024, nold + nold / 2);
+    for (uint64_t r = 0; r < pins.B; ++r)
+      update_queue.push((uint32_t)(pins.b0 + r));
+    for (uint64_t i = 0; i < nold; ++i)
+      update_queue.push(h_flagged[i]);
   }
 
   // NN-Descent insertion (the reference's method)
@@ -767,9 +849,10 @@ struct Engine {
     k_mean_partial<<<NPART, 256, 0, s>>>(pidx, pdist, n, k, euclid ? 1 : 0,
                                          psum, pcnt);
     LAUNCH_CHECK();
-    k_mean_final<<<1, 256, 0, s>>>(psum, pcnt, NPART, mean);
+    static_assert(S_MIN == S_MAXW + 1 && S_MAX == S_MAXW + 2,
+                  "k_mean_final resets three consecutive scalars");
+    k_mean_final<<<1, 256, 0, s>>>(psum, pcnt, NPART, mean, scal + S_MAXW);
     LAUNCH_CHECK();
-    HIPC(hipMemsetAsync(scal + S_MAXW, 0, sizeof(uint32_t), s));
     const uint64_t nd = nb * (uint64_t)k;
     k_membership<<<blocks_for(nd, 256), 256, 0, s>>>(
         pidx, pdist, n, k, euclid ? 1 : 0, sigma, rho, mean, dbatch, nb, w,
@@ -799,27 +882,27 @@ struct Engine {
   // embedding_kernel.cuh:726-798).  `cap` bounds nnz for the launch grids.
   void run_sgd(float *e, uint64_t nrows, uint64_t cap, int n_epochs, int nsr,
                float a, float b, float alpha0, float gamma, uint64_t seed,
-               bool force_per_epoch = false) {
+               bool force_per_epoch = false, bool deg_zeroed = false) {
     if (cap == 0 || n_epochs <= 0)
       return;
-    HIPC(hipMemsetAsync(deg, 0, nrows * sizeof(int), s));
-    HIPC(hipMemsetAsync(maxdeg, 0, sizeof(int), s));
+    // deg is cleared by the caller's k_minmax_col0 on the embedding path;
+    // k_sgd_init clears maxdeg and the barrier words, and k_max_int counts its
+    // finished blocks in gbar[3] (unused by the SGD kernels)
+    if (!deg_zeroed)
+      HIPC(hipMemsetAsync(deg, 0, nrows * sizeof(int), s));
     const float nsr_inv = 1.0f / (float)nsr;
     k_sgd_init<<<blocks_for(cap, 256), 256, 0, s>>>(
         wout, head, scal + S_NNZ, scal + S_MAXW, n_epochs, nsr_inv, eps, nxt,
-        nneg, deg);
+        nneg, deg, maxdeg, gbar, 4 + SGD_MAX_WAVES);
     LAUNCH_CHECK();
     k_max_int<<<std::min<unsigned>(blocks_for(nrows, 256), 64), 256, 0, s>>>(
-        deg, nrows, maxdeg);
-    LAUNCH_CHECK();
-    k_rounding<<<1, 1, 0, s>>>(maxdeg, alpha0, rounding);
+        deg, nrows, maxdeg, alpha0, rounding, gbar + 3);
     LAUNCH_CHECK();
     const unsigned grid = std::min<unsigned>(blocks_for(cap, 256), 16384);
     unsigned long long *st = prof ? kstats : nullptr;
     ev_start(GM_KSTAT_SGD);
     bool fused = false;
     if (!force_per_epoch) {
-      HIPC(hipMemsetAsync(gbar, 0, (4 + SGD_MAX_WAVES) * sizeof(unsigned), s));
       const hipError_t fe = launch_sgd_fused(
           c, e, head, tail, scal + S_NNZ, (uint32_t)nrows, eps, nxt, nneg,
           n_epochs, alpha0, gamma, a, b, nsr_inv, seed, rounding, gbar, st,
@@ -827,6 +910,8 @@ struct Engine {
       if (fe == hipSuccess) {
         fused = true;
         fused_used = true;
+        if (sgd_variant == 2)
+          ++sgd_waves_launches;
         if (prof && sgd_variant == 1)
           pending.back().which = GM_KSTAT_SGD_STREAM;
       } else {
@@ -870,12 +955,10 @@ struct Engine {
     }
     ev_stop(fused ? 1 : (uint64_t)n_epochs);
     if (prof) {
-      uint32_t nnz = 0;
-      HIPC(hipMemcpyAsync(h_scal + 10, scal + S_NNZ, 4, hipMemcpyDeviceToHost, s));
-      HIPC(hipStreamSynchronize(s));
-      nnz = h_scal[10];
-      (fused && sgd_variant == 1 ? sgd_stream_edge_epochs : sgd_edge_epochs) +=
-          (uint64_t)nnz * (uint64_t)n_epochs;
+      k_add_edge_epochs<<<1, 64, 0, s>>>(
+          scal + S_NNZ, n_epochs,
+          kstats + (fused && sgd_variant == 1 ? KS_STREAM_EE : KS_SGD_EE));
+      LAUNCH_CHECK();
     }
   }
 
@@ -922,39 +1005,36 @@ struct Engine {
     if (embedding_sweeps_done > 2)
       embedding_completed = update_completed;
 
-    // noise (reference add_embedding_noise, embedding_kernel.cuh:953-1007)
-    h_scal[8] = 0xffffffffu;
-    h_scal[9] = 0u;
-    HIPC(hipMemcpyAsync(scal + S_MIN, h_scal + 8, 2 * sizeof(uint32_t),
-                        hipMemcpyHostToDevice, s));
+    // noise (reference add_embedding_noise, embedding_kernel.cuh:953-1007);
+    // the column-0 range accumulators were reset by k_mean_final, and the
+    // range pass also clears the SGD's head degrees; k_noise marks the batch
+    // rows for the reposition below when there is one
     k_minmax_col0<<<std::min<unsigned>(blocks_for(n, 256), 64), 256, 0, s>>>(
-        emb, n, c, scal + S_MIN, scal + S_MAX);
+        emb, n, c, scal + S_MIN, scal + S_MAX, deg);
     LAUNCH_CHECK();
     k_noise<<<blocks_for(nb * c, 256), 256, 0, s>>>(
         emb, c, dbatch, nb, scal + S_MIN, scal + S_MAX, 0.001f,
-        42ull ^ (uint64_t)embedding_sweeps_done);
+        42ull ^ (uint64_t)embedding_sweeps_done, nb < n ? bmark : nullptr);
     LAUNCH_CHECK();
-    // the edge count sizes the launch grids (one 4-byte readback)
-    HIPC(hipMemcpyAsync(h_scal + 13, scal + S_NNZ, 4, hipMemcpyDeviceToHost, s));
-    HIPC(hipStreamSynchronize(s));
-    const uint64_t nnz_now = h_scal[13];
-    run_sgd(emb, n, nnz_now, (int)p.max_epoch,
-            (int)p.negative_sample_rate, p.a, p.b, p.alpha, 1.0f, 42ull);
+    // the launch grids are sized for the most edges the batch can give (the
+    // union keeps at most both directions of each of its nb k entries); the
+    // kernels read the edge count itself from the device
+    run_sgd(emb, n, 2 * nb * (uint64_t)k, (int)p.max_epoch,
+            (int)p.negative_sample_rate, p.a, p.b, p.alpha, 1.0f, 42ull,
+            false, true);
+    if (fused_used)
+      HIPC(hipMemcpyAsync(h_scal + 12, gbar + 2, 4, hipMemcpyDeviceToHost, s));
 
     // reposition of the rows outside the batch (reference
     // heuristic_neighbor_avg_kernel :811-894, impl.cuh:422-449), from a
     // snapshot so that no row reads a neighbour's new value; the batch rows
-    // are marked on the device
+    // were marked by k_noise and k_reposition clears the marks
     if (nb < n) {
-      k_mark<<<blocks_for(nb, 256), 256, 0, s>>>(dbatch, nb, bmark, 1);
-      LAUNCH_CHECK();
       HIPC(hipMemcpyAsync(snap, emb, n * c * sizeof(float),
                           hipMemcpyDeviceToDevice, s));
       k_reposition<<<blocks_for(n, 128), 128, 0, s>>>(
           pidx, pdist, n, k, euclid ? 1 : 0, sigma, rho, mean, bmark, snap,
           emb, c);
-      LAUNCH_CHECK();
-      k_mark<<<blocks_for(nb, 256), 256, 0, s>>>(dbatch, nb, bmark, 0);
       LAUNCH_CHECK();
     }
     return (uint32_t)nb;
@@ -968,6 +1048,7 @@ struct Engine {
       auto t0 = clk::now();
       r->n_insertion_ops = insert(ni);
       HIPC(hipStreamSynchronize(s));
+      insert_finish();
       r->insertion_time = std::chrono::duration<float>(clk::now() - t0).count();
       insertion_completed = insertion_queue.size() == 0 && n == N;
     }
@@ -993,8 +1074,14 @@ struct Engine {
     r->insertion_completed = insertion_completed;
     r->update_completed = update_completed;
     r->embedding_completed = embedding_completed;
-    check_barrier();
-    collect_stats();
+    if (fused_used) {
+      // the timeout flag was copied back behind the SGD launch, before the
+      // embedding phase's synchronisation
+      fused_used = false;
+      if (h_scal[12])
+        throw Fail(GM_ERR_HIP, "SGD device-wide barrier timed out");
+    }
+    fold_events();
   }
 
   void check_barrier() {
@@ -1273,6 +1360,7 @@ int gm_get_kernel_stats(gm_handle *h, int32_t which, gm_kernel_stats *out) {
     check_handle(h);
     if (which < 0 || which >= GM_KSTAT_COUNT || !out)
       throw Fail(GM_ERR_INVALID, "bad kernel-stats request");
+    h->e->read_counters();
     *out = h->e->totals[which];
   });
 }
@@ -1284,8 +1372,8 @@ int gm_reset_kernel_stats(gm_handle *h) {
     HIPC(hipStreamSynchronize(e->s));
     for (auto &t : e->totals)
       t = gm_kernel_stats{};
-    e->sgd_edge_epochs = 0;
-    e->sgd_stream_edge_epochs = 0;
+    e->fold_events();
+    e->sgd_waves_launches = 0;
     e->ins_pairs = 0;
     HIPC(hipMemsetAsync(e->kstats, 0, 16 * sizeof(unsigned long long), e->s));
     HIPC(hipStreamSynchronize(e->s));

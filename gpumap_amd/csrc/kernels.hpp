@@ -41,9 +41,10 @@ __global__ void k_join_reset(const uint32_t *owners,
                              uint64_t *rev_old, uint32_t *owner_mark, int *fnc,
                              int *foc);
 __global__ void k_sync_rows(const uint64_t *pool, const uint32_t *rows,
-                            uint64_t nrows, uint32_t *pidx, float *pdist,
-                            int k, unsigned long long *fresh, float *pmax,
-                            int euclid, double *sigma, double *rho);
+                            uint64_t nrows, const uint32_t *nrows_dev,
+                            uint32_t *pidx, float *pdist, int k,
+                            unsigned long long *fresh, float *pmax, int euclid,
+                            double *sigma, double *rho);
 __global__ void k_flag_count(const uint8_t *flag, uint64_t lo, uint64_t hi,
                              uint32_t *bcount);
 __global__ void k_scan_small(uint32_t *v, uint64_t n, uint32_t *total);
@@ -93,7 +94,7 @@ __global__ void k_mean_partial(const uint32_t *pidx, const float *pdist,
                                uint64_t n, int k, int euclid, double *psum,
                                unsigned long long *pcnt);
 __global__ void k_mean_final(const double *psum, const unsigned long long *pcnt,
-                             int nparts, double *mean);
+                             int nparts, double *mean, uint32_t *reset3);
 __global__ void k_membership(const uint32_t *pidx, const float *pdist,
                              uint64_t n, int k, int euclid,
                              const double *sigma, const double *rho,
@@ -111,14 +112,15 @@ __global__ void k_coo_write(const uint32_t *pidx, int k, const uint32_t *batch,
 __global__ void k_sgd_init(const float *w, const int32_t *head,
                            const uint32_t *nnz_p, const uint32_t *maxw_bits,
                            int n_epochs, float nsr_inv, float *eps, float *nxt,
-                           float *nneg, int *deg);
-__global__ void k_max_int(const int *v, uint64_t n, int *out);
-__global__ void k_rounding(const int *max_deg, float alpha, float *rounding);
+                           float *nneg, int *deg, int *maxdeg, unsigned *bar,
+                           unsigned nbar);
+__global__ void k_max_int(const int *v, uint64_t n, int *out, float alpha,
+                          float *rounding, unsigned *done);
 __global__ void k_minmax_col0(const float *emb, uint64_t n, int c,
-                              uint32_t *mn, uint32_t *mx);
+                              uint32_t *mn, uint32_t *mx, int *deg_zero);
 __global__ void k_noise(float *emb, int c, const uint32_t *batch, uint64_t nb,
                         const uint32_t *mn, const uint32_t *mx,
-                        float noise_scale, uint64_t seed);
+                        float noise_scale, uint64_t seed, uint8_t *in_batch);
 template <int NC>
 __global__ void k_sgd_epoch(float *emb, const int32_t *head,
                             const int32_t *tail, const uint32_t *nnz_p,
@@ -136,10 +138,10 @@ __global__ void k_sgd_fused(float *emb, const int32_t *head,
                             unsigned long long *stats, int lag);
 // arrival slots of k_sgd_waves after the 4 barrier words (bar + 4)
 constexpr unsigned SGD_MAX_WAVES = 512;
-// cooperative launch of k_sgd_waves, k_sgd_fused or, for edge sets larger than its
-// resident grid, k_sgd_stream (embed_kernels.hip; *variant 0 or 1, the
-// latter counting its work at stats + 8); returns the launch error when
-// neither can be launched
+// cooperative launch of k_sgd_waves (*variant 2), k_sgd_fused (0) or, for
+// edge sets larger than its resident grid, k_sgd_stream (1, counting its work
+// at stats + 8) (embed_kernels.hip); returns the launch error when none can
+// be launched
 hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
                             const int32_t *tail, const uint32_t *nnz_p,
                             uint32_t n, const float *eps, float *nxt,
@@ -152,9 +154,9 @@ hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
 __global__ void k_reposition(const uint32_t *pidx, const float *pdist,
                              uint64_t n, int k, int euclid, const double *sigma,
                              const double *rho, const double *mean,
-                             const uint8_t *in_batch, const float *snap,
+                             uint8_t *in_batch, const float *snap,
                              float *emb, int c);
-__global__ void k_mark(const uint32_t *batch, uint64_t nb, uint8_t *in_batch,
-                       uint8_t v);
+__global__ void k_add_edge_epochs(const uint32_t *nnz_p, int n_epochs,
+                                  unsigned long long *ctr);
 
 } // namespace gm

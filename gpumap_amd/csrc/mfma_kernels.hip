@@ -417,6 +417,9 @@ __global__ __launch_bounds__(64) void k_bf16_rerank(
     jr[q] = ok ? jq : 0u;
     live |= ok ? 1u << q : 0u;
   }
+  // (the zeroing is a bit mask rather than a select: a select lets the
+  // compiler move each load under a branch and wait for it there, which
+  // serialises the eight loads)
   auto stage = [&](uint64_t f, float4 out[8]) {
     const bool fin = f + c4 < dp;
     const uint64_t col = fin ? f + c4 : dp - 4;
@@ -424,8 +427,11 @@ __global__ __launch_bounds__(64) void k_bf16_rerank(
     for (int q = 0; q < 8; ++q) {
       const float4 v =
           *reinterpret_cast<const float4 *>(X + (uint64_t)jr[q] * dp + col);
-      out[q] = (fin && ((live >> q) & 1u)) ? v
-                                           : make_float4(0.f, 0.f, 0.f, 0.f);
+      const uint32_t m = (fin && ((live >> q) & 1u)) ? ~0u : 0u;
+      out[q] = make_float4(__uint_as_float(__float_as_uint(v.x) & m),
+                           __uint_as_float(__float_as_uint(v.y) & m),
+                           __uint_as_float(__float_as_uint(v.z) & m),
+                           __uint_as_float(__float_as_uint(v.w) & m));
     }
   };
   float4 nv[8];

@@ -660,17 +660,23 @@ __global__ void k_join_reset(const uint32_t *__restrict__ owners,
 // sigma / rho for the fuzzy set, which depend on the public row only.
 __global__ __launch_bounds__(256) void k_sync_rows(
     const uint64_t *__restrict__ pool, const uint32_t *__restrict__ rows,
-    uint64_t nrows, uint32_t *__restrict__ pidx, float *__restrict__ pdist,
-    int k, unsigned long long *__restrict__ fresh, float *__restrict__ pmax,
+    uint64_t nrows, const uint32_t *__restrict__ nrows_dev,
+    uint32_t *__restrict__ pidx, float *__restrict__ pdist, int k,
+    unsigned long long *__restrict__ fresh, float *__restrict__ pmax,
     int euclid, double *__restrict__ sigma, double *__restrict__ rho) {
   __shared__ unsigned long long s_fresh;
   if (threadIdx.x == 0)
     s_fresh = 0;
   __syncthreads();
-  const uint64_t g = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+  // the row count is nrows, or the device count when one is given (then
+  // nrows only bounds it); half-waves stride over the rows
+  const uint64_t cnt =
+      nrows_dev ? min(nrows, (uint64_t)*nrows_dev) : nrows;
+  const uint64_t stride = ((uint64_t)gridDim.x * blockDim.x) >> 5;
   const int hl = threadIdx.x & 31;
   const int half = (threadIdx.x >> 5) & 1;
-  if (g < nrows) { // uniform over the half-wave
+  for (uint64_t g = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+       g < cnt; g += stride) { // uniform over the half-wave
     const uint32_t u = rows[g];
     const uint64_t key = half_sort32(pool[(uint64_t)u * W + hl]);
     const uint32_t old = hl < k ? pidx[(uint64_t)u * k + hl] : NO_ID;
@@ -686,7 +692,7 @@ __global__ __launch_bounds__(256) void k_sync_rows(
       pidx[(uint64_t)u * k + hl] = id;
       pdist[(uint64_t)u * k + hl] = dv;
     }
-    if (hl == 0 && fm)
+    if (fresh && hl == 0 && fm)
       atomicAdd(&s_fresh, (unsigned long long)__popcll(fm));
     if (hl == 31)
       pmax[u] = key == EMPTY ? INFINITY : key_dist(key);
@@ -705,7 +711,7 @@ __global__ __launch_bounds__(256) void k_sync_rows(
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0 && s_fresh)
+  if (fresh && threadIdx.x == 0 && s_fresh)
     atomicAdd(fresh, s_fresh);
 }
 

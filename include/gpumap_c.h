@@ -184,11 +184,12 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
  * algorithmic work:
  *   GM_KSTAT_JOIN        local join; work = {list members, distance pairs,
  *                        queued pool offers}
- *   GM_KSTAT_SGD         SGD with the register-resident cooperative kernel
- *                        (one launch per embedding op) or, when cooperative
- *                        launch is unavailable, one launch per epoch; work =
- *                        {positive samples, negative samples, edge-epochs
- *                        scanned}
+ *   GM_KSTAT_SGD         SGD with a register-resident cooperative kernel
+ *                        (one launch per embedding op: k_sgd_waves for up to
+ *                        512 waves of edges, else k_sgd_fused) or, when
+ *                        cooperative launch is unavailable, one launch per
+ *                        epoch; work = {positive samples, negative samples,
+ *                        edge-epochs scanned, launches that ran k_sgd_waves}
  *   GM_KSTAT_INSERT      insertions (all kernels of one insertion); work =
  *                        {(batch row, row) pairs, BF16 screening survivors
  *                        re-ranked in FP32, insertions redone by the FP32
@@ -210,6 +211,9 @@ typedef struct gm_kernel_stats {
   uint64_t work[4];
 } gm_kernel_stats;
 
+/* Profiling adds no host synchronisation to gm_run: event pairs are folded
+ * after run()'s own per-phase synchronisation and the device counters are
+ * read back by gm_get_kernel_stats (which synchronises the session). */
 int gm_set_profiling(gm_handle *h, int32_t enable);
 int gm_get_kernel_stats(gm_handle *h, int32_t which, gm_kernel_stats *out);
 int gm_reset_kernel_stats(gm_handle *h);
