@@ -716,11 +716,11 @@ struct Engine {
           rlist, rcnt, RCAP, qctl, coef, cfac);
       LAUNCH_CHECK();
       k_bf16_rerank<<<dim3((unsigned)B, RCAP / 64), 64, 0, s>>>(
-          X, Dp, rlist, rcnt, RCAP, qctl, fkeys, pool, pmax, flag, b0,
+          X, Dp, rlist, rcnt, RCAP, qctl, fkeys, pool, pmax, flag, b0, tau, B,
           prof ? kstats : nullptr);
       LAUNCH_CHECK();
-      k_bf16_select<<<(unsigned)B, 256, RCAP * sizeof(uint64_t), s>>>(
-          fkeys, rcnt, RCAP, qctl, pool, b0, prof ? kstats : nullptr);
+      k_bf16_select<<<(unsigned)B, 64, RCAP * sizeof(uint64_t), s>>>(
+          fkeys, rcnt + B, RCAP, qctl, pool, b0, prof ? kstats : nullptr);
       LAUNCH_CHECK();
       // the FP32 tiles below run only if the survivor queue overflowed
       guard = qctl;
@@ -753,7 +753,7 @@ struct Engine {
     HIPC(hipMemsetAsync(Xb, 0, rows_ * Db * 2 * sizeof(__bf16), s));
     tau = dalloc<float>(B);
     rlist = dalloc<uint32_t>(B * (uint64_t)RCAP);
-    rcnt = dalloc<unsigned>(B);
+    rcnt = dalloc<unsigned>(2 * B); // survivors, then forward keys in tau
     fkeys = dalloc<uint64_t>(B * (uint64_t)RCAP);
     xb_cap = B;
   }

@@ -79,12 +79,13 @@ __global__ void k_bf16_screen(const __bf16 *Xs, const __bf16 *Xb, int nsteps,
 __global__ void k_tau_select(const float *part, int nsplit, float *tau,
                              uint64_t nb);
 __global__ void k_bf16_rerank(const float *X, uint64_t dp,
-                              const uint32_t *rlist, const unsigned *rcnt,
+                              const uint32_t *rlist, unsigned *rcnt,
                               unsigned rcap, const unsigned *qctl,
                               uint64_t *fkeys, uint64_t *pool,
                               const float *pmax, uint8_t *flag, uint64_t b0,
+                              const float *tau, uint64_t nb,
                               unsigned long long *stats);
-__global__ void k_bf16_select(const uint64_t *fkeys, const unsigned *rcnt,
+__global__ void k_bf16_select(const uint64_t *fkeys, const unsigned *fcnt,
                               unsigned rcap, const unsigned *qctl,
                               uint64_t *pool, uint64_t b0,
                               unsigned long long *stats);

@@ -108,6 +108,7 @@ __global__ __launch_bounds__(64) void k_bf16_prep(
     nrm[u] = s;
     flag[u] = 1;
     rcnt[r] = 0;
+    rcnt[nb + r] = 0; // the row's forward keys within tau (k_bf16_rerank)
   }
   if (r == 0 && lane == 1)
     qctl[1] = 0;
@@ -307,67 +308,49 @@ template __global__ void k_bf16_screen<2>(
     uint32_t *, unsigned *, unsigned, unsigned *, float, float);
 
 // tau of one batch row: the 32nd smallest of its nsplit * 64 upper bounds
-// (+inf when it has fewer than 32 other rows).  Radix select on the float
-// bits (all values are >= 0), 8 bits per round.
+// (+inf when it has fewer than 32 other rows), found by bisection over the
+// float bit patterns (all values are >= 0) with block-wide counts; the
+// values stay in registers and no bin is contended.
 __global__ __launch_bounds__(256) void k_tau_select(
     const float *__restrict__ part, int nsplit, float *__restrict__ tau,
     uint64_t nb) {
-  __shared__ unsigned hist[256];
-  __shared__ unsigned s_prefix, s_rank;
+  __shared__ unsigned s_cnt[4];
   const uint64_t r = blockIdx.x;
   if (r >= nb)
     return;
-  const int len = nsplit * 64;
+  const int len = nsplit * 64; // at most 128 * 64 = 32 values per thread
   const unsigned *v =
       reinterpret_cast<const unsigned *>(part) + r * (uint64_t)len;
-  if (threadIdx.x == 0) {
-    s_prefix = 0;
-    s_rank = W; // 1-based rank still to find among values with the prefix
-  }
-  unsigned mask = 0;
-  for (int shift = 24; shift >= 0; shift -= 8) {
-    hist[threadIdx.x] = 0;
-    __syncthreads();
-    const unsigned prefix = s_prefix;
-    for (int e = threadIdx.x; e < len; e += blockDim.x) {
-      const unsigned x = v[e];
-      if ((x & mask) == prefix)
-        atomicAdd(&hist[(x >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-      // wave 0: inclusive scan of the 256 bins, 4 per lane
-      const int l = threadIdx.x;
-      const unsigned hh[4] = {hist[4 * l], hist[4 * l + 1], hist[4 * l + 2],
-                              hist[4 * l + 3]};
-      const unsigned mine = hh[0] + hh[1] + hh[2] + hh[3];
-      unsigned incl = mine;
+  unsigned x[32];
 #pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const unsigned o = __shfl_up(incl, off, 64);
-        if (l >= off)
-          incl += o;
-      }
-      const unsigned excl = incl - mine;
-      const unsigned rank = s_rank;
-      if (excl < rank && rank <= incl) {
-        unsigned c = excl, bin = 4 * l + 3;
-        for (int q = 0; q < 4; ++q) {
-          if (c + hh[q] >= rank) {
-            bin = 4 * l + q;
-            break;
-          }
-          c += hh[q];
-        }
-        s_prefix = prefix | (bin << shift);
-        s_rank = rank - c;
-      }
-    }
-    mask |= 255u << shift;
+  for (int i = 0; i < 32; ++i) {
+    const int e = (int)threadIdx.x + 256 * i;
+    x[i] = e < len ? v[e] : 0xffffffffu;
+  }
+  // the smallest t with #{x <= t} >= W, by bisection over the 32-bit
+  // patterns (the values are compared as unsigned patterns, which orders
+  // the non-negative floats); counts are exact, so t is the W-th smallest
+  unsigned lo = 0u, hi = 0xffffffffu;
+  for (int it = 0; it < 32 && lo < hi; ++it) {
+    const unsigned mid = lo + ((hi - lo) >> 1);
+    unsigned c = 0;
+#pragma unroll
+    for (int i = 0; i < 32; ++i)
+      c += x[i] <= mid ? 1u : 0u;
+    for (int off = 32; off > 0; off >>= 1)
+      c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0)
+      s_cnt[threadIdx.x >> 6] = c;
     __syncthreads();
+    const unsigned tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    __syncthreads();
+    if (tot >= (unsigned)W)
+      hi = mid;
+    else
+      lo = mid + 1u;
   }
   if (threadIdx.x == 0)
-    tau[r] = __uint_as_float(s_prefix);
+    tau[r] = __uint_as_float(lo);
 }
 
 // Canonical distances of the listed pairs and the reverse offers.  Block
@@ -378,16 +361,17 @@ __global__ __launch_bounds__(256) void k_tau_select(
 // coalesced 128-byte row reads (the next chunk's reads are in flight
 // meanwhile), then every lane continues its own pair's FMA chain over the
 // chunk in feature order (the canonical distance; u's features are
-// wave-uniform loads).  Forward keys go to fkeys at the entry's position
-// (EMPTY for a reverse-only entry) for k_bf16_select; every lane offers its
+// wave-uniform loads).  Forward keys within the row's tau are appended to
+// the row's key list in fkeys for k_bf16_select; every lane offers its
 // reverse pair, when within the column row's pool maximum, with the
 // single-lane lock-free insert.  Does nothing when a list overflowed.
 __global__ __launch_bounds__(64) void k_bf16_rerank(
     const float *__restrict__ X, uint64_t dp,
-    const uint32_t *__restrict__ rlist, const unsigned *__restrict__ rcnt,
+    const uint32_t *__restrict__ rlist, unsigned *__restrict__ rcnt,
     unsigned rcap, const unsigned *__restrict__ qctl,
     uint64_t *__restrict__ fkeys, uint64_t *__restrict__ pool,
     const float *__restrict__ pmax, uint8_t *__restrict__ flag, uint64_t b0,
+    const float *__restrict__ tau, uint64_t nb,
     unsigned long long *__restrict__ stats) {
   __shared__ float sl[64 * RS];
   const uint64_t r = blockIdx.x;
@@ -477,39 +461,74 @@ __global__ __launch_bounds__(64) void k_bf16_rerank(
                      "+v"(nv[q].w));
     __syncthreads();
   }
-  if (valid)
-    fkeys[r * (uint64_t)rcap + b + lane] =
-        (ent & LFWD) ? make_key(acc, (uint32_t)j, false) : EMPTY;
+  // forward keys within tau, appended to the row's key list (rcnt[nb + r]
+  // counts them): the row's 32 nearest are among them, since the 32 columns
+  // with the smallest upper bounds all have exact distances <= tau
+  const bool fw = valid && (ent & LFWD) && acc <= tau[r];
+  const uint64_t fm = __ballot(fw);
+  unsigned base = 0;
+  if (lane == 0 && fm)
+    base = atomicAdd(&rcnt[nb + r], (unsigned)__popcll(fm));
+  base = (unsigned)__shfl((int)base, 0, 64);
+  if (fw)
+    fkeys[r * (uint64_t)rcap + base +
+          __popcll(fm & ((1ull << lane) - 1ull))] =
+        make_key(acc, (uint32_t)j, false);
   if (valid && (ent & LREV) && acc <= pmax[j] &&
       pool_offer_lane(pool + j * W, make_key(acc, (uint32_t)u, false)))
     flag[j] = 1;
 }
 
-// Batch row u = b0 + r's pool: the 32 smallest of its forward keys (bitonic
-// sort in LDS over the list length rounded up to a power of two).
-__global__ __launch_bounds__(256) void k_bf16_select(
-    const uint64_t *__restrict__ fkeys, const unsigned *__restrict__ rcnt,
+// Batch row u = b0 + r's pool: the 32 smallest of its forward keys (those
+// within tau, fcnt[r] of them), one wave per row.  Up to 64 keys are sorted in registers (bitonic network over
+// the lanes); longer lists are sorted in LDS (bitonic over the list length
+// rounded up to a power of two).
+__device__ inline uint64_t shfl_xor_u64(uint64_t v, int m) {
+  const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
+  const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__global__ __launch_bounds__(64) void k_bf16_select(
+    const uint64_t *__restrict__ fkeys, const unsigned *__restrict__ fcnt,
     unsigned rcap, const unsigned *__restrict__ qctl,
     uint64_t *__restrict__ pool, uint64_t b0,
     unsigned long long *__restrict__ stats) {
   extern __shared__ uint64_t s_keys[]; // rcap keys
   const uint64_t r = blockIdx.x;
+  const int lane = threadIdx.x;
   if (qctl[1]) {
-    if (stats && r == 0 && threadIdx.x == 0)
+    if (stats && r == 0 && lane == 0)
       atomicAdd(&stats[6], 1ull);
     return;
   }
-  const unsigned cnt = min(rcnt[r], rcap);
-  int len = W;
+  const unsigned cnt = min(fcnt[r], rcap);
+  const uint64_t *src = fkeys + r * (uint64_t)rcap;
+  if (cnt <= 64) {
+    uint64_t key = (unsigned)lane < cnt ? src[lane] : EMPTY;
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1)
+#pragma unroll
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        const uint64_t o = shfl_xor_u64(key, stride);
+        // ascending runs where lane & size == 0; the lower lane of a pair
+        // keeps the smaller key in an ascending run
+        const bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0);
+        key = keep_min ? (o < key ? o : key) : (o > key ? o : key);
+      }
+    if (lane < W)
+      pool[(b0 + r) * W + lane] = key;
+    return;
+  }
+  int len = 128;
   while ((unsigned)len < cnt)
     len <<= 1;
-  const uint64_t *src = fkeys + r * (uint64_t)rcap;
-  for (int e = threadIdx.x; e < len; e += blockDim.x)
+  for (int e = lane; e < len; e += 64)
     s_keys[e] = (unsigned)e < cnt ? src[e] : EMPTY;
   __syncthreads();
   for (int size = 2; size <= len; size <<= 1)
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int t = threadIdx.x; t < len / 2; t += blockDim.x) {
+      for (int t = lane; t < len / 2; t += 64) {
         const int i = 2 * stride * (t / stride) + (t % stride);
         const int jj = i + stride;
         const bool up = (i & size) == 0;
@@ -521,8 +540,8 @@ __global__ __launch_bounds__(256) void k_bf16_select(
       }
       __syncthreads();
     }
-  if (threadIdx.x < W)
-    pool[(b0 + r) * W + threadIdx.x] = s_keys[threadIdx.x];
+  if (lane < W)
+    pool[(b0 + r) * W + lane] = s_keys[lane];
 }
 
 } // namespace gm
