@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <numeric>
@@ -141,6 +142,12 @@ struct Engine {
   uint64_t *fkeys = nullptr; // forward keys of the survivors
   unsigned *rcnt = nullptr;  // survivors per batch row
   unsigned *qctl = nullptr;  // [1]: a survivor list overflowed
+  // pass-1 lower bounds of every (column, batch row) pair, column-major, so
+  // that the survivor lists need no second pass over the tiles; absent when
+  // N x batch floats exceed lbm_max_bytes (8 GiB, or the environment's
+  // GPUMAP_AMD_LBM_MAX_BYTES at creation)
+  float *lbm = nullptr;
+  uint64_t lbm_max_bytes = 8ull << 30;
   uint64_t xb_cap = 0;
   static constexpr unsigned RCAP = 2048; // survivors per batch row
   uint64_t ins_pairs = 0; // (batch row, column) pairs screened or computed
@@ -224,6 +231,8 @@ struct Engine {
     euclid = p.metric == GM_EUCLIDEAN;
     HIPC(hipGetDevice(&device));
     update_queue.in.assign(N, 0);
+    if (const char *v = std::getenv("GPUMAP_AMD_LBM_MAX_BYTES"))
+      lbm_max_bytes = std::strtoull(v, nullptr, 10);
     try {
       allocate();
     } catch (...) {
@@ -360,9 +369,11 @@ struct Engine {
       (void)hipFree(gbar);
     gbar = nullptr;
     for (void *q : {(void *)Xs, (void *)Xb, (void *)nrm, (void *)tau,
-                    (void *)rlist, (void *)rcnt, (void *)qctl, (void *)fkeys})
+                    (void *)rlist, (void *)rcnt, (void *)qctl, (void *)fkeys,
+                    (void *)lbm})
       if (q)
         (void)hipFree(q);
+    lbm = nullptr;
     Xs = Xb = nullptr;
     nrm = tau = nullptr;
     rlist = nullptr;
@@ -707,13 +718,24 @@ struct Engine {
       LAUNCH_CHECK();
       k_bf16_screen<1><<<blocks, 256, 0, s>>>(
           Xs, Xb, nsteps, nrm, pmax, tau, b0, B, n, bsplit, (int)brt, part_f,
-          rlist, rcnt, RCAP, qctl, coef, cfac);
+          rlist, rcnt, RCAP, qctl, coef, cfac, lbm);
       LAUNCH_CHECK();
       k_tau_select<<<(unsigned)B, 256, 0, s>>>(part_f, bsplit, tau, B);
       LAUNCH_CHECK();
-      k_bf16_screen<2><<<blocks, 256, 0, s>>>(
-          Xs, Xb, nsteps, nrm, pmax, tau, b0, B, n, bsplit, (int)brt, part_f,
-          rlist, rcnt, RCAP, qctl, coef, cfac);
+      if (lbm) {
+        // waves = row groups x column ranges, about 16 waves per CU
+        const uint64_t groups = (B + 63) / 64;
+        const uint64_t ranges =
+            std::max<uint64_t>(1, 16 * (uint64_t)num_cu / groups);
+        const uint64_t cpw = std::max<uint64_t>(8, (n + ranges - 1) / ranges);
+        const uint64_t waves = groups * ((n + cpw - 1) / cpw);
+        k_bf16_lbscan<<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(
+            lbm, B, n, b0, tau, pmax, rlist, rcnt, RCAP, qctl, cpw);
+      } else {
+        k_bf16_screen<2><<<blocks, 256, 0, s>>>(
+            Xs, Xb, nsteps, nrm, pmax, tau, b0, B, n, bsplit, (int)brt,
+            part_f, rlist, rcnt, RCAP, qctl, coef, cfac, nullptr);
+      }
       LAUNCH_CHECK();
       k_bf16_rerank<<<dim3((unsigned)B, RCAP / 64), 64, 0, s>>>(
           X, Dp, rlist, rcnt, RCAP, qctl, fkeys, pool, pmax, flag, b0, tau, B,
@@ -739,7 +761,7 @@ struct Engine {
     if (B <= xb_cap)
       return;
     for (void *q : {(void *)Xb, (void *)tau, (void *)rlist, (void *)rcnt,
-                    (void *)fkeys})
+                    (void *)fkeys, (void *)lbm})
       if (q)
         (void)hipFree(q);
     Xb = nullptr;
@@ -747,6 +769,7 @@ struct Engine {
     rlist = nullptr;
     rcnt = nullptr;
     fkeys = nullptr;
+    lbm = nullptr;
     xb_cap = 0;
     const uint64_t rows_ = (B + 127) / 128 * 128;
     Xb = dalloc<__bf16>(rows_ * Db * 2);
@@ -755,6 +778,13 @@ struct Engine {
     rlist = dalloc<uint32_t>(B * (uint64_t)RCAP);
     rcnt = dalloc<unsigned>(2 * B); // survivors, then forward keys in tau
     fkeys = dalloc<uint64_t>(B * (uint64_t)RCAP);
+    if (N * B * sizeof(float) <= lbm_max_bytes) {
+      // optional: without it the survivors come from a second tile pass
+      if (hipMalloc((void **)&lbm, N * B * sizeof(float)) != hipSuccess) {
+        (void)hipGetLastError();
+        lbm = nullptr;
+      }
+    }
     xb_cap = B;
   }
 

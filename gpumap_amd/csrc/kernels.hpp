@@ -75,7 +75,12 @@ __global__ void k_bf16_screen(const __bf16 *Xs, const __bf16 *Xb, int nsteps,
                               const float *tau, uint64_t b0, uint64_t nb,
                               uint64_t n, int nsplit, int rtiles, float *part,
                               uint32_t *rlist, unsigned *rcnt, unsigned rcap,
-                              unsigned *qctl, float coef, float cfac);
+                              unsigned *qctl, float coef, float cfac,
+                              float *lbm);
+__global__ void k_bf16_lbscan(const float *lbm, uint64_t nb, uint64_t n,
+                              uint64_t b0, const float *tau, const float *pmax,
+                              uint32_t *rlist, unsigned *rcnt, unsigned rcap,
+                              unsigned *qctl, uint64_t cols_per_wave);
 __global__ void k_tau_select(const float *part, int nsplit, float *tau,
                              uint64_t nb);
 __global__ void k_bf16_rerank(const float *X, uint64_t dp,

@@ -128,7 +128,8 @@ __global__ __launch_bounds__(256) void k_bf16_screen(
     const float *__restrict__ tau, uint64_t b0, uint64_t nb, uint64_t n,
     int nsplit, int rtiles, float *__restrict__ part,
     uint32_t *__restrict__ rlist, unsigned *__restrict__ rcnt, unsigned rcap,
-    unsigned *__restrict__ qctl, float coef, float cfac) {
+    unsigned *__restrict__ qctl, float coef, float cfac,
+    float *__restrict__ lbm) {
   __shared__ float s_nj[TC], s_pm[TC];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -257,6 +258,9 @@ __global__ __launch_bounds__(256) void k_bf16_screen(
           const float dt = __builtin_fmaf(-2.0f, acc[sa][sb][i], s);
           const float bnd = __builtin_fmaf(coef, s, 1e-30f);
           if (PASS == 1) {
+            // the lower bound of every pair, column-major, for k_bf16_lbscan
+            if (lbm && rok[sb] && j < n)
+              lbm[j * nb + rl] = (dt - bnd) * (1.0f - cfac);
             const float ub = fmaxf((dt + bnd) * (1.0f + cfac), 0.0f);
             if (ok && ub < top[sb][TOPL - 1]) {
 #pragma unroll
@@ -301,11 +305,85 @@ __global__ __launch_bounds__(256) void k_bf16_screen(
 template __global__ void k_bf16_screen<1>(
     const __bf16 *, const __bf16 *, int, const float *, const float *,
     const float *, uint64_t, uint64_t, uint64_t, int, int, float *,
-    uint32_t *, unsigned *, unsigned, unsigned *, float, float);
+    uint32_t *, unsigned *, unsigned, unsigned *, float, float, float *);
 template __global__ void k_bf16_screen<2>(
     const __bf16 *, const __bf16 *, int, const float *, const float *,
     const float *, uint64_t, uint64_t, uint64_t, int, int, float *,
-    uint32_t *, unsigned *, unsigned, unsigned *, float, float);
+    uint32_t *, unsigned *, unsigned, unsigned *, float, float, float *);
+
+// The survivor lists from the lower bounds stored by the first pass (in
+// place of a second pass over the tiles): the pair (batch row rl, column j)
+// is listed as forward when lb <= tau[rl] and as reverse when lb <=
+// pmax[j] for an earlier row j, exactly as k_bf16_screen<2> lists it (the
+// order within a list differs, which nothing downstream depends on).  A
+// wave takes 64 batch rows (one per lane, tau in a register) and a range of
+// columns, eight columns per step: each step's eight loads of
+// lbm[j * nb + rl] are 256-byte rows, in flight together.
+__global__ __launch_bounds__(256) void k_bf16_lbscan(
+    const float *__restrict__ lbm, uint64_t nb, uint64_t n, uint64_t b0,
+    const float *__restrict__ tau, const float *__restrict__ pmax,
+    uint32_t *__restrict__ rlist, unsigned *__restrict__ rcnt, unsigned rcap,
+    unsigned *__restrict__ qctl, uint64_t cols_per_wave) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wid = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint64_t ngroups = (nb + 63) / 64;
+  const uint64_t g = wid % ngroups;
+  const uint64_t j0 = (wid / ngroups) * cols_per_wave;
+  if (j0 >= n)
+    return;
+  const uint64_t j1 = min(n, j0 + cols_per_wave);
+  const uint64_t rl = g * 64 + lane;
+  const bool rok = rl < nb;
+  const float tr = rok ? tau[rl] : -INFINITY;
+  const uint64_t u = b0 + rl;
+  // a lane's entries are buffered and appended four at a time: one atomic
+  // (and its round trip) per four entries instead of one per entry
+  uint32_t buf0 = 0, buf1 = 0, buf2 = 0, buf3 = 0;
+  unsigned nbuf = 0;
+  auto flush = [&]() {
+    const unsigned at = atomicAdd(&rcnt[rl], nbuf);
+    const uint32_t e4[4] = {buf0, buf1, buf2, buf3};
+#pragma unroll
+    for (unsigned q = 0; q < 4; ++q)
+      if (q < nbuf) {
+        if (at + q < rcap)
+          rlist[rl * rcap + at + q] = e4[q];
+        else
+          atomicOr(&qctl[1], 1u);
+      }
+    nbuf = 0;
+  };
+  auto emit = [&](uint64_t j, float lb, float pm) {
+    const bool fwd = lb <= tr;
+    const bool rev = lb <= pm;
+    if (rok && j != u && (fwd || rev)) {
+      const uint32_t ent = (uint32_t)j | (fwd ? LFWD : 0u) | (rev ? LREV : 0u);
+      buf3 = nbuf == 3 ? ent : buf3;
+      buf2 = nbuf == 2 ? ent : buf2;
+      buf1 = nbuf == 1 ? ent : buf1;
+      buf0 = nbuf == 0 ? ent : buf0;
+      if (++nbuf == 4)
+        flush();
+    }
+  };
+  const uint64_t rc = rok ? rl : 0; // an in-bounds address for idle lanes
+  uint64_t j = j0;
+  for (; j + 8 <= j1; j += 8) {
+    float lb[8], pm[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      lb[q] = lbm[(j + q) * nb + rc];
+      pm[q] = j + q < b0 ? pmax[j + q] : -INFINITY;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      emit(j + q, lb[q], pm[q]);
+  }
+  for (; j < j1; ++j)
+    emit(j, lbm[j * nb + rc], j < b0 ? pmax[j] : -INFINITY);
+  if (nbuf)
+    flush();
+}
 
 // tau of one batch row: the 32nd smallest of its nsplit * 64 upper bounds
 // (+inf when it has fewer than 32 other rows), found by bisection over the

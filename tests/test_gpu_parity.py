@@ -232,6 +232,25 @@ def test_bf16_screening_on_image_like_rows():
     assert np.array_equal(np.sort(engines[1].pool(), 1), np.sort(o.pool(), 1))
 
 
+@pytest.mark.parametrize("lb_store", [True, False])
+def test_bf16_screening_survivor_paths(lb_store, monkeypatch):
+    # the survivor lists come from the stored first-pass lower bounds, or,
+    # when the store is not allocated, from a second pass over the tiles;
+    # both give the FP32 tiles' graph
+    if not lb_store:
+        monkeypatch.setenv("GPUMAP_AMD_LBM_MAX_BYTES", "0")
+    x = ev.image_like(n=2000, d=200)
+    n, d = x.shape
+    e1, e2 = capi.Engine(n, d, knn_mode=1), capi.Engine(n, d, knn_mode=2)
+    for s in range(0, n, 400):
+        for e in (e1, e2):
+            e.append(x[s:s + 400])
+            e.run(400, 400, 0)
+    (i1, d1), (i2, d2) = e1.knn(), e2.knn()
+    assert np.array_equal(i1, i2) and np.array_equal(d1, d2)
+    assert np.array_equal(np.sort(e1.pool(), 1), np.sort(e2.pool(), 1))
+
+
 def test_sgd_fused_equals_per_epoch_launches():
     # conflict-free edges, attraction only: the single cooperative launch and
     # the one-launch-per-epoch path compute the same epochs
