@@ -17,12 +17,15 @@
 //     on the matrix core and the round-to-nearest adds across steps, the norm
 //     and d~ roundings; cfac covers the canonical distance's own rounding
 //     (derived on the host, engine.hip);
-//   * pass 1 keeps, per lane and batch row, the 16 smallest upper bounds
-//     d~ + bound it sees; the 32nd smallest of all of them, tau, is an upper
-//     bound of the row's 32nd smallest exact distance (k_tau_select);
-//   * pass 2 recomputes the tiles and lists, per batch row, every column
-//     whose lower bound d~ - bound is <= tau (forward) or <= the column row's
-//     pool maximum (reverse); k_bf16_rerank computes the listed pairs'
+//   * pass 1 keeps, per lane and batch row, the TOPL smallest upper bounds
+//     d~ + bound it sees (dropping larger ones can only raise what follows);
+//     the 32nd smallest of all of them, tau, is an upper bound of the row's
+//     32nd smallest exact distance (k_tau_select); it also stores every
+//     pair's lower bound d~ - bound;
+//   * the survivor scan (k_bf16_lbscan, or a second pass over the tiles when
+//     the store does not fit) lists, per batch row, every column whose lower
+//     bound is <= tau (forward) or <= the column row's pool maximum
+//     (reverse); k_bf16_rerank computes the listed pairs'
 //     canonical distances, writes each batch row's 32 smallest forward keys to
 //     its pool and offers the reverse ones with the lock-free bounded insert,
 //     whose result does not depend on the order.  No pair that can enter a
@@ -47,7 +50,11 @@ typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
 
 constexpr int TB = 128;  // batch rows per block tile
 constexpr int TC = 128;  // columns (earlier rows) per block tile
-constexpr int TOPL = 16; // upper bounds kept per lane and batch row
+// upper bounds kept per lane and batch row: a lane sees a quarter of a
+// block's columns for a row, so a list of 8 is cut short only when more than
+// 8 of the row's 32 nearest fall among them (tau is then larger, still valid)
+constexpr int TOPL = 8;
+constexpr int PARTW = 4 * TOPL; // pass-1 values per (batch row, split)
 constexpr uint32_t LFWD = 0x80000000u; // survivor list entry: forward pair
 constexpr uint32_t LREV = 0x40000000u; // survivor list entry: reverse pair
 constexpr uint32_t LID = 0x3fffffffu;  // survivor list entry: column row
@@ -121,8 +128,10 @@ __global__ __launch_bounds__(64) void k_bf16_prep(
 // 64 x 64 sub-tile (columns (w & 1) * 64, batch rows (w >> 1) * 64) as 2 x 2
 // MFMA blocks; C register i of lane l is column (i / 4) * 8 + (l / 32) * 4 +
 // i % 4 and batch row l % 32 of its block.
+// Two blocks per CU (at most 256 registers per lane): the second wave on each
+// SIMD runs its matrix work while the first waits for operands.
 template <int PASS>
-__global__ __launch_bounds__(256) void k_bf16_screen(
+__global__ __launch_bounds__(256, 2) void k_bf16_screen(
     const __bf16 *__restrict__ Xs, const __bf16 *__restrict__ Xb, int nsteps,
     const float *__restrict__ nrm, const float *__restrict__ pmax,
     const float *__restrict__ tau, uint64_t b0, uint64_t nb, uint64_t n,
@@ -287,13 +296,13 @@ __global__ __launch_bounds__(256) void k_bf16_screen(
   }
 
   if (PASS == 1) {
-    // 64 values per (batch row, split): 4 lanes x 16, each list sorted
+    // PARTW values per (batch row, split): 4 lanes x TOPL, each list sorted
 #pragma unroll
     for (int sb = 0; sb < 2; ++sb) {
       const uint64_t rl = r0 + wb * 64 + sb * 32 + (lane & 31);
       if (!rok[sb])
         continue;
-      float *dst = part + (rl * (uint64_t)nsplit + split) * 64 +
+      float *dst = part + (rl * (uint64_t)nsplit + split) * PARTW +
                    (wa * 2 + (lane >> 5)) * TOPL;
 #pragma unroll
       for (int q = 0; q < TOPL; ++q)
@@ -385,7 +394,7 @@ __global__ __launch_bounds__(256) void k_bf16_lbscan(
     flush();
 }
 
-// tau of one batch row: the 32nd smallest of its nsplit * 64 upper bounds
+// tau of one batch row: the 32nd smallest of its nsplit * PARTW upper bounds
 // (+inf when it has fewer than 32 other rows), found by bisection over the
 // float bit patterns (all values are >= 0) with block-wide counts; the
 // values stay in registers and no bin is contended.
@@ -396,12 +405,13 @@ __global__ __launch_bounds__(256) void k_tau_select(
   const uint64_t r = blockIdx.x;
   if (r >= nb)
     return;
-  const int len = nsplit * 64; // at most 128 * 64 = 32 values per thread
+  constexpr int PER = 128 * PARTW / 256; // values per thread for nsplit 128
+  const int len = nsplit * PARTW;
   const unsigned *v =
       reinterpret_cast<const unsigned *>(part) + r * (uint64_t)len;
-  unsigned x[32];
+  unsigned x[PER];
 #pragma unroll
-  for (int i = 0; i < 32; ++i) {
+  for (int i = 0; i < PER; ++i) {
     const int e = (int)threadIdx.x + 256 * i;
     x[i] = e < len ? v[e] : 0xffffffffu;
   }
@@ -413,7 +423,7 @@ __global__ __launch_bounds__(256) void k_tau_select(
     const unsigned mid = lo + ((hi - lo) >> 1);
     unsigned c = 0;
 #pragma unroll
-    for (int i = 0; i < 32; ++i)
+    for (int i = 0; i < PER; ++i)
       c += x[i] <= mid ? 1u : 0u;
     for (int off = 32; off > 0; off >>= 1)
       c += __shfl_xor(c, off, 64);
