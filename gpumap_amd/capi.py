@@ -77,6 +77,7 @@ def lib():
         L.gm_initialize.argtypes = [vp, u64, u64]
         L.gm_append_data.argtypes = [vp, P(f32), u64, u64]
         L.gm_run.argtypes = [vp, u32, u32, u32, P(GmResult)]
+        L.gm_run_batch.argtypes = [P(vp), i32, P(u32), P(GmResult)]
         L.gm_close.argtypes = [vp]
         L.gm_destroy.argtypes = [vp]
         L.gm_destroy.restype = None
@@ -181,6 +182,21 @@ class Engine:
         check(lib().gm_run(self.h, i, u, e, ctypes.byref(r)))
         self.n = r.n_inserted_instances
         return r
+
+    @staticmethod
+    def run_batch(engines, ops):
+        """One cycle of several sessions together (gm_run_batch); `ops` holds
+        one (insertion, update, embedding) triple per engine."""
+        cnt = len(engines)
+        hs = (ctypes.c_void_p * cnt)(
+            *[e.h.value if isinstance(e.h, ctypes.c_void_p) else e.h
+              for e in engines])
+        flat = np.ascontiguousarray(np.asarray(ops, np.uint32).reshape(cnt, 3))
+        rs = (GmResult * cnt)()
+        check(lib().gm_run_batch(hs, cnt, _p(flat, u32), rs))
+        for e, r in zip(engines, rs):
+            e.n = r.n_inserted_instances
+        return list(rs)
 
     def knn(self):
         idx = np.zeros((self.n, self.k), np.uint32)

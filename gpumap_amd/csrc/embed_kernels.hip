@@ -799,26 +799,25 @@ __global__ __launch_bounds__(256) void k_sgd_fused(
 // All epochs in one cooperative launch for edge sets of at most 512 waves
 // (one edge per thread, one wave per block): the per-cycle embedding batches.
 // Epoch boundary without atomics or block barriers: wave w stores epoch + 1
-// into its arrival slot (bar + 4 + w) when it finishes an epoch, and may
-// start epoch e + 1 once every slot is >= e - lag + 1.  Each lane reads up
-// to 8 slots at the start of the epoch and the wave reduces them at the
-// boundary, so the read overlaps the epoch's own gathers and a wave that is
-// not more than `lag` epochs ahead waits for nothing.  The wait is bounded
-// (timeout flag in bar[2], as epoch_end).
+// into its arrival slot slots[w] when it finishes an epoch, and may start
+// epoch e + 1 once every slot of its edge set is >= e - lag + 1.  Each lane
+// reads up to 8 slots at the start of the epoch and the wave reduces them at
+// the boundary, so the read overlaps the epoch's own gathers and a wave that
+// is not more than `lag` epochs ahead waits for nothing.  The wait is bounded
+// (*timeout is set, as epoch_end's bar[2]).  sgd_waves_body is one wave's
+// work; k_sgd_waves runs one edge set, k_sgd_waves_multi several (one
+// session each) side by side, each paced only by its own waves.
 template <int NC>
-__global__ __launch_bounds__(64) void k_sgd_waves(
+__device__ __attribute__((always_inline)) inline void sgd_waves_body(
     float *__restrict__ emb, const int32_t *__restrict__ head,
-    const int32_t *__restrict__ tail, const uint32_t *__restrict__ nnz_p,
-    uint32_t n, const float *__restrict__ eps, int n_epochs, float alpha0,
-    float gamma, float a, float b, float nsr_inv, uint64_t seed,
-    const float *__restrict__ rounding_p, unsigned *__restrict__ bar,
-    unsigned long long *__restrict__ stats, int lag) {
-  unsigned *slots = bar + 4;
-  const unsigned nw = gridDim.x;
-  const int lane = threadIdx.x;
-  const uint64_t nnz = *nnz_p;
-  const float rounding = *rounding_p;
-  const uint64_t e = (uint64_t)blockIdx.x * 64 + lane;
+    const int32_t *__restrict__ tail, uint64_t nnz, uint32_t n,
+    const float *__restrict__ eps, int n_epochs, float alpha0, float gamma,
+    float a, float b, float nsr_inv, uint64_t seed, float rounding,
+    unsigned *__restrict__ slots, unsigned nw, unsigned w,
+    unsigned *__restrict__ timeout, unsigned long long *__restrict__ stats,
+    int lag) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t e = (uint64_t)w * 64 + lane;
   int32_t hh, tt;
   float ep, nx, nn;
   if (e < nnz) {
@@ -866,8 +865,8 @@ __global__ __launch_bounds__(64) void k_sgd_waves(
     alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
     // boundary
     if (lane == 0)
-      __hip_atomic_store(slots + blockIdx.x, (unsigned)epoch + 1u,
-                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(slots + w, (unsigned)epoch + 1u, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     if (epoch >= lag && epoch + 1 < n_epochs) {
       const unsigned target = (unsigned)(epoch - lag + 1);
       unsigned spins = 0;
@@ -883,7 +882,7 @@ __global__ __launch_bounds__(64) void k_sgd_waves(
           break;
         if (++spins > (1u << 20)) {
           if (lane == 0)
-            __hip_atomic_store(bar + 2, 1u, __ATOMIC_RELAXED,
+            __hip_atomic_store(timeout, 1u, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
           break;
         }
@@ -899,6 +898,39 @@ __global__ __launch_bounds__(64) void k_sgd_waves(
     }
   }
   flush_sgd_stats(stats, c_pos, c_neg);
+}
+
+template <int NC>
+__global__ __launch_bounds__(64) void k_sgd_waves(
+    float *__restrict__ emb, const int32_t *__restrict__ head,
+    const int32_t *__restrict__ tail, const uint32_t *__restrict__ nnz_p,
+    uint32_t n, const float *__restrict__ eps, int n_epochs, float alpha0,
+    float gamma, float a, float b, float nsr_inv, uint64_t seed,
+    const float *__restrict__ rounding_p, unsigned *__restrict__ bar,
+    unsigned long long *__restrict__ stats, int lag) {
+  sgd_waves_body<NC>(emb, head, tail, *nnz_p, n, eps, n_epochs, alpha0, gamma,
+                     a, b, nsr_inv, seed, *rounding_p, bar + 4, gridDim.x,
+                     blockIdx.x, bar + 2, stats, lag);
+}
+
+// Block b runs wave b - job.wave0 of the job whose wave range holds b; the
+// slots of job j are bar[4 + job.wave0 ...], the timeout flag bar[2].
+template <int NC>
+__global__ __launch_bounds__(64) void k_sgd_waves_multi(SgdBatch batch,
+                                                        unsigned *__restrict__ bar,
+                                                        int lag) {
+  const unsigned b = blockIdx.x;
+  int j = 0;
+  for (int q = 1; q < batch.njobs; ++q)
+    if (b >= batch.job[q].wave0)
+      j = q;
+  const SgdJob &J = batch.job[j];
+  if (b >= J.wave0 + J.nwaves)
+    return; // not part of any job (never launched so)
+  sgd_waves_body<NC>(J.emb, J.head, J.tail, *J.nnz_p, J.n, J.eps, J.n_epochs,
+                     J.alpha0, J.gamma, J.a, J.b, J.nsr_inv, J.seed,
+                     *J.rounding_p, bar + 4 + J.wave0, J.nwaves,
+                     b - J.wave0, bar + 2, J.stats, lag);
 }
 
 // All epochs in one cooperative launch for edge sets larger than the
@@ -1061,6 +1093,37 @@ static hipError_t try_fused_any(float *emb, const int32_t *head,
   return try_stream<NC>(emb, head, tail, nnz_p, n, eps, nxt, nneg, n_epochs,
                         alpha0, gamma, a, b, nsr_inv, seed, rounding_p, bar,
                         stats, lag, nnz, num_cu, s);
+}
+
+template <int NC>
+static hipError_t try_batch(const SgdBatch &batch, unsigned waves,
+                            unsigned *bar, int lag, int num_cu,
+                            hipStream_t s) {
+  static int per_cu = 0;
+  if (per_cu <= 0)
+    per_cu = blocks_per_cu(&k_sgd_waves_multi<NC>, 64);
+  if (waves == 0 || waves > (unsigned)per_cu * (unsigned)num_cu)
+    return hipErrorInvalidConfiguration;
+  SgdBatch bb = batch;
+  void *args[] = {&bb, &bar, &lag};
+  return hipLaunchCooperativeKernel(
+      reinterpret_cast<const void *>(&k_sgd_waves_multi<NC>), dim3(waves),
+      dim3(64), args, 0, s);
+}
+
+hipError_t launch_sgd_batch(int nc, const SgdBatch &batch, unsigned waves,
+                            unsigned *bar, int lag, int num_cu,
+                            hipStream_t s) {
+  switch (nc) {
+  case 1:
+    return try_batch<1>(batch, waves, bar, lag, num_cu, s);
+  case 2:
+    return try_batch<2>(batch, waves, bar, lag, num_cu, s);
+  case 3:
+    return try_batch<3>(batch, waves, bar, lag, num_cu, s);
+  default:
+    return try_batch<4>(batch, waves, bar, lag, num_cu, s);
+  }
 }
 
 hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,

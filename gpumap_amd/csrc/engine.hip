@@ -28,6 +28,7 @@
 #include <stdexcept>
 #include <string>
 #include <unordered_set>
+#include <mutex>
 #include <vector>
 
 namespace gm {
@@ -112,6 +113,11 @@ enum Scalar {
 };
 
 } // namespace
+
+// Held from the start of a cooperative SGD launch until it has completed, so
+// that no two such launches (of any sessions in this process) are in flight
+// on the device at once: each relies on all its blocks being resident.
+std::mutex g_coop_mu;
 
 struct Engine {
   gm_params p;
@@ -910,17 +916,15 @@ struct Engine {
   // SGD on the COO in head/tail/wout with nnz in scal[S_NNZ] and the
   // maximum weight in scal[S_MAXW] (reference run_optimize_layout,
   // embedding_kernel.cuh:726-798).  `cap` bounds nnz for the launch grids.
-  void run_sgd(float *e, uint64_t nrows, uint64_t cap, int n_epochs, int nsr,
-               float a, float b, float alpha0, float gamma, uint64_t seed,
-               bool force_per_epoch = false, bool deg_zeroed = false) {
-    if (cap == 0 || n_epochs <= 0)
-      return;
-    // deg is cleared by the caller's k_minmax_col0 on the embedding path;
-    // k_sgd_init clears maxdeg and the barrier words, and k_max_int counts its
-    // finished blocks in gbar[3] (unused by the SGD kernels)
+  // The SGD's schedule and rounding grid: epochs_per_sample and epoch state
+  // per edge, head degrees, the grid from the maximum degree.  deg is cleared
+  // by the caller's k_minmax_col0 on the embedding path; k_sgd_init clears
+  // maxdeg and the barrier words, and k_max_int counts its finished blocks in
+  // gbar[3] (unused by the SGD kernels).
+  void sgd_setup(uint64_t nrows, uint64_t cap, int n_epochs, float nsr_inv,
+                 float alpha0, bool deg_zeroed) {
     if (!deg_zeroed)
       HIPC(hipMemsetAsync(deg, 0, nrows * sizeof(int), s));
-    const float nsr_inv = 1.0f / (float)nsr;
     k_sgd_init<<<blocks_for(cap, 256), 256, 0, s>>>(
         wout, head, scal + S_NNZ, scal + S_MAXW, n_epochs, nsr_inv, eps, nxt,
         nneg, deg, maxdeg, gbar, 4 + SGD_MAX_WAVES);
@@ -928,6 +932,15 @@ struct Engine {
     k_max_int<<<std::min<unsigned>(blocks_for(nrows, 256), 64), 256, 0, s>>>(
         deg, nrows, maxdeg, alpha0, rounding, gbar + 3);
     LAUNCH_CHECK();
+  }
+
+  void run_sgd(float *e, uint64_t nrows, uint64_t cap, int n_epochs, int nsr,
+               float a, float b, float alpha0, float gamma, uint64_t seed,
+               bool force_per_epoch = false, bool deg_zeroed = false) {
+    if (cap == 0 || n_epochs <= 0)
+      return;
+    const float nsr_inv = 1.0f / (float)nsr;
+    sgd_setup(nrows, cap, n_epochs, nsr_inv, alpha0, deg_zeroed);
     const unsigned grid = std::min<unsigned>(blocks_for(cap, 256), 16384);
     unsigned long long *st = prof ? kstats : nullptr;
     ev_start(GM_KSTAT_SGD);
@@ -995,6 +1008,21 @@ struct Engine {
   // -------------------------------------------------------------- embed
   // reference Impl::embed (impl.cuh:335-500)
   uint32_t embed(uint32_t n_ops) {
+    const uint32_t nb = embed_begin(n_ops, nullptr);
+    embed_end();
+    return nb;
+  }
+
+  // The embedding op up to its SGD.  With `job` null the SGD runs here on
+  // the session's stream; otherwise, when the op's edge set fits the
+  // per-wave kernel, its schedule is set up and `job` describes its SGD for a
+  // batched launch (job->nwaves = 0 when the SGD ran here after all).  The
+  // reposition follows in embed_end, once the SGD is done.
+  uint64_t pend_nb = 0;
+  uint32_t embed_begin(uint32_t n_ops, SgdJob *job) {
+    if (job)
+      job->nwaves = 0;
+    pend_nb = 0;
     if (n_ops == 0 || n == 0)
       return 0;
     std::vector<uint32_t> batch;
@@ -1049,17 +1077,34 @@ struct Engine {
     // the launch grids are sized for the most edges the batch can give (the
     // union keeps at most both directions of each of its nb k entries); the
     // kernels read the edge count itself from the device
-    run_sgd(emb, n, 2 * nb * (uint64_t)k, (int)p.max_epoch,
-            (int)p.negative_sample_rate, p.a, p.b, p.alpha, 1.0f, 42ull,
-            false, true);
+    const uint64_t cap = 2 * nb * (uint64_t)k;
+    const uint64_t waves = (cap + 63) / 64;
+    pend_nb = nb;
+    if (job && cap > 0 && p.max_epoch > 0 && waves <= SGD_MAX_WAVES) {
+      const float nsr_inv = 1.0f / (float)p.negative_sample_rate;
+      sgd_setup(n, cap, (int)p.max_epoch, nsr_inv, p.alpha, true);
+      *job = SgdJob{emb,     head,         tail,
+                    scal + S_NNZ, eps,     rounding,
+                    prof ? kstats : nullptr, 42ull, (uint32_t)n,
+                    (int)p.max_epoch, p.alpha, 1.0f, p.a, p.b, nsr_inv, 0u,
+                    (unsigned)waves};
+      return (uint32_t)nb;
+    }
+    run_sgd(emb, n, cap, (int)p.max_epoch, (int)p.negative_sample_rate, p.a,
+            p.b, p.alpha, 1.0f, 42ull, false, true);
     if (fused_used)
       HIPC(hipMemcpyAsync(h_scal + 12, gbar + 2, 4, hipMemcpyDeviceToHost, s));
+    return (uint32_t)nb;
+  }
 
-    // reposition of the rows outside the batch (reference
-    // heuristic_neighbor_avg_kernel :811-894, impl.cuh:422-449), from a
-    // snapshot so that no row reads a neighbour's new value; the batch rows
-    // were marked by k_noise and k_reposition clears the marks
-    if (nb < n) {
+  // reposition of the rows outside the op's batch (reference
+  // heuristic_neighbor_avg_kernel :811-894, impl.cuh:422-449), from a
+  // snapshot so that no row reads a neighbour's new value; the batch rows
+  // were marked by k_noise and k_reposition clears the marks
+  void embed_end() {
+    const uint64_t nb = pend_nb;
+    pend_nb = 0;
+    if (nb > 0 && nb < n) {
       HIPC(hipMemcpyAsync(snap, emb, n * c * sizeof(float),
                           hipMemcpyDeviceToDevice, s));
       k_reposition<<<blocks_for(n, 128), 128, 0, s>>>(
@@ -1067,7 +1112,6 @@ struct Engine {
           emb, c);
       LAUNCH_CHECK();
     }
-    return (uint32_t)nb;
   }
 
   // reference Impl::run (impl.cuh:146-218)
@@ -1091,12 +1135,22 @@ struct Engine {
       r->n_update_changed_edges = ch;
     }
     r->embedding_pending_edges = n;
-    if (!(ne == 0 || n == 0 || embedding_completed)) {
+    if (embedding_due(ne)) {
+      std::lock_guard<std::mutex> lk(g_coop_mu);
       auto t0 = clk::now();
       r->n_embedding_ops = embed(ne);
       HIPC(hipStreamSynchronize(s));
       r->embedding_time = std::chrono::duration<float>(clk::now() - t0).count();
     }
+    finish_result(r);
+  }
+
+  bool embedding_due(uint32_t ne) const {
+    return !(ne == 0 || n == 0 || embedding_completed);
+  }
+
+  // the result's state fields once the cycle's work has completed
+  void finish_result(gm_result *r) {
     r->n_inserted_instances = n;
     r->insertion_queue_size = insertion_queue.size();
     r->update_queue_size = update_queue.size();
@@ -1143,6 +1197,202 @@ void dl_deleter(DLManagedTensor *self) {
   Export *x = reinterpret_cast<Export *>(self);
   engine_unref(static_cast<Engine *>(self->manager_ctx));
   delete x;
+}
+
+// ------------------------------------------------------- batched cycles
+// One cycle of several sessions at once (gm_run_batch): every session's
+// insertion kernels are enqueued before any phase waits, so they overlap on
+// the device; the per-cycle SGDs of all sessions whose edge sets fit the
+// per-wave kernel run as one cooperative launch of k_sgd_waves_multi, each
+// paced only by its own waves; every session's results are those of gm_run
+// on its own (the phases' times are the batch's).  The batched launch uses a
+// stream, arrival slots and a pinned timeout flag shared per device.
+struct BatchCtx {
+  int device = -1;
+  hipStream_t s = nullptr;
+  unsigned *bar = nullptr;       // 4 words, then one slot per wave (the
+                                 // launches run in order on s and reuse it)
+  unsigned *h_flag = nullptr;    // pinned: each chunk's timeout flag
+  int num_cu = 0;
+};
+static constexpr unsigned BATCH_MAX_WAVES = 2048; // per launch
+static constexpr int BATCH_MAX_CHUNKS = 64;
+std::mutex g_batch_mu;
+std::vector<BatchCtx> g_batch;
+
+BatchCtx &batch_ctx(int device) {
+  for (auto &b : g_batch)
+    if (b.device == device)
+      return b;
+  BatchCtx b;
+  b.device = device;
+  HIPC(hipStreamCreate(&b.s));
+  HIPC(hipMalloc((void **)&b.bar, (4 + (size_t)BATCH_MAX_WAVES) *
+                                      sizeof(unsigned)));
+  HIPC(hipHostMalloc((void **)&b.h_flag, BATCH_MAX_CHUNKS * sizeof(unsigned)));
+  HIPC(hipDeviceGetAttribute(&b.num_cu, hipDeviceAttributeMultiprocessorCount,
+                             device));
+  g_batch.push_back(b);
+  return g_batch.back();
+}
+
+void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
+               gm_result *rs) {
+  using clk = std::chrono::steady_clock;
+  auto secs = [](clk::time_point t0) {
+    return std::chrono::duration<float>(clk::now() - t0).count();
+  };
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  BatchCtx &bc = batch_ctx(es[0]->device);
+  for (int i = 0; i < cnt; ++i)
+    std::memset(&rs[i], 0, sizeof(gm_result));
+
+  // insertion: all sessions' kernels first, then the waits
+  std::vector<char> did(cnt, 0);
+  auto t0 = clk::now();
+  for (int i = 0; i < cnt; ++i)
+    if (!es[i]->insertion_completed) {
+      did[i] = 1;
+      rs[i].n_insertion_ops = es[i]->insert(ops[3 * i]);
+    }
+  for (int i = 0; i < cnt; ++i)
+    if (did[i]) {
+      HIPC(hipStreamSynchronize(es[i]->s));
+      es[i]->insert_finish();
+    }
+  float t = secs(t0);
+  for (int i = 0; i < cnt; ++i)
+    if (did[i]) {
+      rs[i].insertion_time = t;
+      es[i]->insertion_completed =
+          es[i]->insertion_queue.size() == 0 && es[i]->n == es[i]->N;
+    }
+
+  // update
+  t0 = clk::now();
+  for (int i = 0; i < cnt; ++i) {
+    did[i] = !(ops[3 * i + 1] == 0 || es[i]->update_completed);
+    if (did[i]) {
+      uint64_t ch = 0;
+      rs[i].n_update_ops = es[i]->update(ops[3 * i + 1], ch);
+      rs[i].n_update_changed_edges = ch;
+    }
+  }
+  for (int i = 0; i < cnt; ++i)
+    if (did[i])
+      HIPC(hipStreamSynchronize(es[i]->s));
+  t = secs(t0);
+  for (int i = 0; i < cnt; ++i)
+    if (did[i])
+      rs[i].update_time = t;
+
+  // embedding: each session up to its SGD on its own stream; an SGD too
+  // large for the batched kernel runs in embed_begin and completes before
+  // the next cooperative launch
+  std::lock_guard<std::mutex> coop(g_coop_mu);
+  t0 = clk::now();
+  std::vector<SgdJob> jobs(cnt);
+  std::vector<int> jidx;
+  for (int i = 0; i < cnt; ++i) {
+    rs[i].embedding_pending_edges = es[i]->n;
+    did[i] = es[i]->embedding_due(ops[3 * i + 2]);
+    if (did[i]) {
+      rs[i].n_embedding_ops = es[i]->embed_begin(ops[3 * i + 2], &jobs[i]);
+      if (jobs[i].nwaves)
+        jidx.push_back(i);
+      else
+        HIPC(hipStreamSynchronize(es[i]->s));
+    }
+  }
+  // the batched SGD launches: jobs of one component count, at most
+  // SGD_MAX_JOBS jobs and BATCH_MAX_WAVES waves per launch
+  int nchunks = 0;
+  unsigned *const bar = bc.bar;
+  std::vector<char> taken(jidx.size(), 0);
+  for (size_t a = 0; a < jidx.size(); ++a) {
+    if (taken[a])
+      continue;
+    const int c = es[jidx[a]]->c;
+    SgdBatch b{};
+    std::vector<int> members;
+    unsigned waves = 0;
+    for (size_t q = a; q < jidx.size() && b.njobs < SGD_MAX_JOBS; ++q) {
+      const int i = jidx[q];
+      if (taken[q] || es[i]->c != c ||
+          waves + jobs[i].nwaves > BATCH_MAX_WAVES)
+        continue;
+      taken[q] = 1;
+      b.job[b.njobs] = jobs[i];
+      b.job[b.njobs].wave0 = waves;
+      waves += jobs[i].nwaves;
+      ++b.njobs;
+      members.push_back(i);
+    }
+    if (nchunks >= BATCH_MAX_CHUNKS)
+      throw Fail(GM_ERR_INVALID, "too many sessions in one batched cycle");
+    // the batch stream waits for every member's set-up
+    for (int i : members) {
+      hipEvent_t ev = es[i]->ev_get();
+      HIPC(hipEventRecord(ev, es[i]->s));
+      HIPC(hipStreamWaitEvent(bc.s, ev, 0));
+      es[i]->ev_free.push_back(ev);
+    }
+    HIPC(hipMemsetAsync(bar, 0, (4 + (size_t)waves) * sizeof(unsigned), bc.s));
+    std::vector<Engine::Timed> tm;
+    for (int i : members)
+      if (es[i]->prof) {
+        Engine::Timed x{GM_KSTAT_SGD, es[i]->ev_get(), es[i]->ev_get(), 1};
+        HIPC(hipEventRecord(x.a, bc.s));
+        tm.push_back(x);
+      }
+    const hipError_t err = launch_sgd_batch(c, b, waves, bar,
+                                            2 * Engine::SGD_LAG, bc.num_cu,
+                                            bc.s);
+    if (err != hipSuccess) {
+      (void)hipGetLastError();
+      throw Fail(GM_ERR_HIP, std::string("batched SGD launch failed: ") +
+                                 hipGetErrorString(err));
+    }
+    size_t ti = 0;
+    for (int i : members)
+      if (es[i]->prof) {
+        HIPC(hipEventRecord(tm[ti].b, bc.s));
+        es[i]->pending.push_back(tm[ti++]);
+      }
+    HIPC(hipMemcpyAsync(bc.h_flag + nchunks, bar + 2, sizeof(unsigned),
+                        hipMemcpyDeviceToHost, bc.s));
+    // each member's stream continues after the launch
+    hipEvent_t done = es[members[0]]->ev_get();
+    HIPC(hipEventRecord(done, bc.s));
+    for (int i : members) {
+      HIPC(hipStreamWaitEvent(es[i]->s, done, 0));
+      ++es[i]->sgd_waves_launches;
+      if (es[i]->prof) {
+        k_add_edge_epochs<<<1, 64, 0, es[i]->s>>>(
+            jobs[i].nnz_p, jobs[i].n_epochs,
+            es[i]->kstats + Engine::KS_SGD_EE);
+        LAUNCH_CHECK();
+      }
+    }
+    es[members[0]]->ev_free.push_back(done);
+    ++nchunks;
+  }
+  for (int i = 0; i < cnt; ++i)
+    if (did[i])
+      es[i]->embed_end();
+  for (int i = 0; i < cnt; ++i)
+    if (did[i])
+      HIPC(hipStreamSynchronize(es[i]->s));
+  HIPC(hipStreamSynchronize(bc.s));
+  t = secs(t0);
+  for (int i = 0; i < cnt; ++i)
+    if (did[i])
+      rs[i].embedding_time = t;
+  for (int q = 0; q < nchunks; ++q)
+    if (bc.h_flag[q])
+      throw Fail(GM_ERR_HIP, "batched SGD device-wide barrier timed out");
+  for (int i = 0; i < cnt; ++i)
+    es[i]->finish_result(&rs[i]);
 }
 
 __global__ void k_philox_probe(const uint32_t *in, uint32_t *out) {
@@ -1378,6 +1628,26 @@ int gm_append_data_device(gm_handle *h, const float *device_rows,
   });
 }
 
+int gm_run_batch(gm_handle *const *hs, int32_t count, const uint32_t *ops,
+                 gm_result *results) {
+  return guarded([&] {
+    if (!hs || !ops || !results || count < 1)
+      throw Fail(GM_ERR_INVALID, "bad batched run request");
+    std::vector<gm::Engine *> es((size_t)count);
+    for (int32_t i = 0; i < count; ++i) {
+      check_handle(hs[i]);
+      es[i] = hs[i]->e;
+      if (es[i]->device != es[0]->device)
+        throw Fail(GM_ERR_INVALID, "batched sessions must share one device");
+      for (int32_t j = 0; j < i; ++j)
+        if (es[j] == es[i])
+          throw Fail(GM_ERR_INVALID, "a session appears twice in one batch");
+    }
+    HIPC(hipSetDevice(es[0]->device));
+    gm::run_batch(es.data(), count, ops, results);
+  });
+}
+
 int gm_set_profiling(gm_handle *h, int32_t enable) {
   return guarded([&] {
     check_handle(h);
@@ -1529,6 +1799,7 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
                         e->s));
     HIPC(hipMemcpyAsync(e->scal + gm::S_MAXW, &sc[1], 4, hipMemcpyHostToDevice,
                         e->s));
+    std::lock_guard<std::mutex> lk(gm::g_coop_mu);
     if (mw > 0.0f && nnz > 0)
       e->run_sgd(e->emb, n, (uint64_t)nnz, n_epochs, nsr, a, b, alpha0, gamma,
                  seed, per_epoch != 0);

@@ -144,6 +144,33 @@ __global__ void k_sgd_fused(float *emb, const int32_t *head,
                             unsigned long long *stats, int lag);
 // arrival slots of k_sgd_waves after the 4 barrier words (bar + 4)
 constexpr unsigned SGD_MAX_WAVES = 512;
+
+// One session's per-cycle SGD in a batched launch (k_sgd_waves_multi): its
+// edge set, schedule and parameters, and its range of waves in the grid.
+struct SgdJob {
+  float *emb;
+  const int32_t *head, *tail;
+  const uint32_t *nnz_p;
+  const float *eps, *rounding_p;
+  unsigned long long *stats;
+  uint64_t seed;
+  uint32_t n;
+  int n_epochs;
+  float alpha0, gamma, a, b, nsr_inv;
+  unsigned wave0, nwaves;
+};
+constexpr int SGD_MAX_JOBS = 16;
+struct SgdBatch { // passed by value as a kernel argument
+  SgdJob job[SGD_MAX_JOBS];
+  int njobs;
+};
+// one cooperative launch of k_sgd_waves_multi over the jobs' waves (job q
+// starts at wave job[q].wave0, ascending, each <= SGD_MAX_WAVES waves); bar
+// holds 4 words and then one slot per wave, zeroed by the caller; returns
+// hipErrorInvalidConfiguration when the waves cannot all be resident
+hipError_t launch_sgd_batch(int nc, const SgdBatch &batch, unsigned waves,
+                            unsigned *bar, int lag, int num_cu,
+                            hipStream_t s);
 // cooperative launch of k_sgd_waves (*variant 2), k_sgd_fused (0) or, for
 // edge sets larger than its resident grid, k_sgd_stream (1, counting its work
 // at stats + 8) (embed_kernels.hip); returns the launch error when none can

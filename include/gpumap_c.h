@@ -124,6 +124,17 @@ int gm_append_data(gm_handle *h, const float *host_rows, uint64_t rows,
 int gm_run(gm_handle *h, uint32_t n_insertion_ops, uint32_t n_update_ops,
            uint32_t n_embedding_ops, gm_result *out);
 
+/* One cycle of several sessions together (no counterpart in the reference,
+ * whose sessions run one at a time; the serving form of its run()):
+ * session i runs ops[3i], ops[3i+1], ops[3i+2] insertion, update and
+ * embedding ops exactly as gm_run would, with the same results, except that
+ * the three phase times are the batch's.  The sessions' insertion kernels
+ * overlap on the device and their per-cycle SGDs run as one cooperative
+ * launch, each paced only by its own edges.  All sessions must be on one
+ * device and distinct; calls on one device are serialised. */
+int gm_run_batch(gm_handle *const *handles, int32_t count,
+                 const uint32_t *ops, gm_result *results);
+
 /* GPUMAP::close -- gpumap.cuh:87, impl.cuh:502-507: synchronises the device.
  * Arrays stay readable until gm_destroy. */
 int gm_close(gm_handle *h);
