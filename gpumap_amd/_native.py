@@ -17,6 +17,21 @@ if not (os.path.exists(LIB) and os.path.exists(core_path())):
         f"(missing {os.path.basename(LIB)} or _core); run "
         "`python -m gpumap_amd.build`")
 
+
+def _share_torch_runtime() -> None:
+    """Load torch (when installed) before the engine.  torch carries its
+    own HIP runtime; loaded first, its runtime is the one the engine binds
+    to, so the DLPack getters and the engine share one runtime.  With the
+    engine's runtime loaded first, torch's own initialisation later fails
+    in the same process (it finds no devices)."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:  # pragma: no cover - torch is part of the image
+        pass
+
+
+_share_torch_runtime()
+
 # the binding resolves libgpumap_amd.so through its rpath; loading it here
 # first makes the failure message precise if the HIP runtime is missing
 _lib = ctypes.CDLL(LIB, mode=ctypes.RTLD_GLOBAL)

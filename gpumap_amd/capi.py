@@ -70,6 +70,12 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
+        # one HIP runtime per process: torch's, loaded first (see
+        # gpumap_amd._native)
+        try:
+            import torch  # noqa: F401
+        except ImportError:  # pragma: no cover - torch is part of the image
+            pass
         L = ctypes.CDLL(LIB_PATH)
         vp = ctypes.c_void_p
         L.gm_default_params.argtypes = [P(GmParams)]

@@ -6,11 +6,14 @@
 // device memory alive.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
 
+#include <array>
 #include <cctype>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/gpumap_c.h"
 #include "dlpack_abi.hpp"
@@ -193,6 +196,31 @@ struct Model {
   }
 };
 
+// One cycle of several sessions together (gm_run_batch); ops holds one
+// (insertion, update, embedding) triple per model.
+std::vector<Result> run_batch(const std::vector<Model *> &models,
+                              const std::vector<std::array<uint32_t, 3>> &ops) {
+  if (models.size() != ops.size() || models.empty())
+    throw std::invalid_argument("run_batch needs one op triple per model");
+  std::vector<gm_handle *> hs;
+  std::vector<uint32_t> flat;
+  for (size_t i = 0; i < models.size(); ++i) {
+    hs.push_back(models[i]->s->h);
+    flat.insert(flat.end(), ops[i].begin(), ops[i].end());
+  }
+  std::vector<gm_result> rs(models.size());
+  int rc;
+  {
+    py::gil_scoped_release nogil;
+    rc = gm_run_batch(hs.data(), (int32_t)hs.size(), flat.data(), rs.data());
+  }
+  check(rc);
+  std::vector<Result> out(models.size());
+  for (size_t i = 0; i < rs.size(); ++i)
+    out[i].r = rs[i];
+  return out;
+}
+
 } // namespace
 
 PYBIND11_MODULE(_core, m) {
@@ -308,4 +336,9 @@ PYBIND11_MODULE(_core, m) {
            [](const Model &mm) { return mm.get(GM_NEIGHBOR_INDICES); })
       .def("get_neighbor_dists",
            [](const Model &mm) { return mm.get(GM_NEIGHBOR_DISTS); });
+
+  m.def("run_batch", &run_batch, py::arg("models"), py::arg("ops"),
+        "One cycle of several sessions together: each runs its op triple "
+        "as GPUMAP.run would, with the same results; the device work of "
+        "the sessions overlaps and their SGDs share one launch.");
 }

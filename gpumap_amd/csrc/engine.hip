@@ -520,15 +520,15 @@ struct Engine {
     n += nr;
   }
 
-  // ascending rows with flag set in [0, hi) into `list`, their count into
-  // scal[S_TOTAL]; clears the flags
-  void compact_flags(uint64_t hi) {
+  // ascending rows in [0, hi) with flag set or in [f0, f1) into `list`, their
+  // count into scal[S_TOTAL]; clears the flags
+  void compact_flags(uint64_t hi, uint64_t f0 = 0, uint64_t f1 = 0) {
     const unsigned nblk = blocks_for(hi, 1024);
-    k_flag_count<<<nblk, 256, 0, s>>>(flag, 0, hi, bcount);
+    k_flag_count<<<nblk, 256, 0, s>>>(flag, 0, hi, bcount, f0, f1);
     LAUNCH_CHECK();
     k_scan_small<<<1, 1024, 0, s>>>(bcount, nblk, scal + S_TOTAL);
     LAUNCH_CHECK();
-    k_flag_write<<<nblk, 64, 0, s>>>(flag, 0, hi, bcount, list);
+    k_flag_write<<<nblk, 64, 0, s>>>(flag, 0, hi, bcount, list, f0, f1);
     LAUNCH_CHECK();
   }
 
@@ -598,8 +598,7 @@ struct Engine {
     // the compacted list is the earlier flagged rows (ascending), then
     // b0 .. b0 + B - 1 (the batch rows are the newest rows, nothing after
     // them has a pool yet)
-    HIPC(hipMemsetAsync(flag + b0, 1, B, s));
-    compact_flags(n);
+    compact_flags(n, b0, b0 + B);
     const unsigned sgrid = (unsigned)std::min<uint64_t>(
         blocks_for(n * 32, 256), 8 * (uint64_t)num_cu);
     k_sync_rows<<<sgrid, 256, 0, s>>>(pool, list, n, scal + S_TOTAL, pidx,
@@ -1247,90 +1246,45 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
   for (int i = 0; i < cnt; ++i)
     std::memset(&rs[i], 0, sizeof(gm_result));
 
-  // insertion: all sessions' kernels first, then the waits
+  // Every session's insertion kernels are enqueued first.  The sessions are
+  // then finished in order: wait for its insertion, its host bookkeeping and
+  // update, then its embedding op up to the SGD, so that the host prepares
+  // session i's embedding while the device still runs the later sessions'
+  // insertions.  A session's phase times run from the batch start to the
+  // end of its insertion, over its update, and from the start of its
+  // embedding op to the end of the batch.  An SGD too large for the batched
+  // kernel runs in embed_begin and completes before the next cooperative
+  // launch.
   std::vector<char> did(cnt, 0);
-  auto t0 = clk::now();
+  const auto tb = clk::now();
   for (int i = 0; i < cnt; ++i)
     if (!es[i]->insertion_completed) {
       did[i] = 1;
       rs[i].n_insertion_ops = es[i]->insert(ops[3 * i]);
     }
-  for (int i = 0; i < cnt; ++i)
-    if (did[i]) {
-      HIPC(hipStreamSynchronize(es[i]->s));
-      es[i]->insert_finish();
-    }
-  float t = secs(t0);
-  for (int i = 0; i < cnt; ++i)
-    if (did[i]) {
-      rs[i].insertion_time = t;
-      es[i]->insertion_completed =
-          es[i]->insertion_queue.size() == 0 && es[i]->n == es[i]->N;
-    }
-
-  // update
-  t0 = clk::now();
-  for (int i = 0; i < cnt; ++i) {
-    did[i] = !(ops[3 * i + 1] == 0 || es[i]->update_completed);
-    if (did[i]) {
-      uint64_t ch = 0;
-      rs[i].n_update_ops = es[i]->update(ops[3 * i + 1], ch);
-      rs[i].n_update_changed_edges = ch;
-    }
-  }
-  for (int i = 0; i < cnt; ++i)
-    if (did[i])
-      HIPC(hipStreamSynchronize(es[i]->s));
-  t = secs(t0);
-  for (int i = 0; i < cnt; ++i)
-    if (did[i])
-      rs[i].update_time = t;
-
-  // embedding: each session up to its SGD on its own stream; an SGD too
-  // large for the batched kernel runs in embed_begin and completes before
-  // the next cooperative launch
   std::lock_guard<std::mutex> coop(g_coop_mu);
-  t0 = clk::now();
   std::vector<SgdJob> jobs(cnt);
-  std::vector<int> jidx;
-  for (int i = 0; i < cnt; ++i) {
-    rs[i].embedding_pending_edges = es[i]->n;
-    did[i] = es[i]->embedding_due(ops[3 * i + 2]);
-    if (did[i]) {
-      rs[i].n_embedding_ops = es[i]->embed_begin(ops[3 * i + 2], &jobs[i]);
-      if (jobs[i].nwaves)
-        jidx.push_back(i);
-      else
-        HIPC(hipStreamSynchronize(es[i]->s));
-    }
-  }
-  // the batched SGD launches: jobs of one component count, at most
-  // SGD_MAX_JOBS jobs and BATCH_MAX_WAVES waves per launch
+  std::vector<char> emb(cnt, 0);
+  std::vector<clk::time_point> te(cnt);
   int nchunks = 0;
   unsigned *const bar = bc.bar;
-  std::vector<char> taken(jidx.size(), 0);
-  for (size_t a = 0; a < jidx.size(); ++a) {
-    if (taken[a])
-      continue;
-    const int c = es[jidx[a]]->c;
+  // One batched SGD launch over `members` (one component count, at most
+  // SGD_MAX_JOBS jobs and BATCH_MAX_WAVES waves), after their set-up; their
+  // streams continue after it.  Launches run in order on the batch stream.
+  auto launch_group = [&](const std::vector<int> &members) {
+    if (members.empty())
+      return;
+    if (nchunks >= BATCH_MAX_CHUNKS)
+      throw Fail(GM_ERR_INVALID, "too many sessions in one batched cycle");
+    const int c = es[members[0]]->c;
     SgdBatch b{};
-    std::vector<int> members;
     unsigned waves = 0;
-    for (size_t q = a; q < jidx.size() && b.njobs < SGD_MAX_JOBS; ++q) {
-      const int i = jidx[q];
-      if (taken[q] || es[i]->c != c ||
-          waves + jobs[i].nwaves > BATCH_MAX_WAVES)
-        continue;
-      taken[q] = 1;
+    for (int i : members) {
       b.job[b.njobs] = jobs[i];
       b.job[b.njobs].wave0 = waves;
       waves += jobs[i].nwaves;
       ++b.njobs;
-      members.push_back(i);
     }
-    if (nchunks >= BATCH_MAX_CHUNKS)
-      throw Fail(GM_ERR_INVALID, "too many sessions in one batched cycle");
-    // the batch stream waits for every member's set-up
     for (int i : members) {
       hipEvent_t ev = es[i]->ev_get();
       HIPC(hipEventRecord(ev, es[i]->s));
@@ -1361,7 +1315,6 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
       }
     HIPC(hipMemcpyAsync(bc.h_flag + nchunks, bar + 2, sizeof(unsigned),
                         hipMemcpyDeviceToHost, bc.s));
-    // each member's stream continues after the launch
     hipEvent_t done = es[members[0]]->ev_get();
     HIPC(hipEventRecord(done, bc.s));
     for (int i : members) {
@@ -1376,18 +1329,76 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
     }
     es[members[0]]->ev_free.push_back(done);
     ++nchunks;
+  };
+  // One launch for all prepared sessions of a component count (a launch is
+  // closed early only at SGD_MAX_JOBS jobs or BATCH_MAX_WAVES waves):
+  // splitting the sessions into launches that overlap the later sessions'
+  // insertions measured slower, as the two contend for the CUs.
+  const unsigned gmax = (unsigned)SGD_MAX_JOBS;
+  std::vector<std::vector<int>> open; // one open group per component count
+  std::vector<unsigned> open_waves;
+  auto group_of = [&](int c) -> size_t {
+    for (size_t g = 0; g < open.size(); ++g)
+      if (!open[g].empty() && es[open[g][0]]->c == c)
+        return g;
+    open.emplace_back();
+    open_waves.push_back(0);
+    return open.size() - 1;
+  };
+  for (int i = 0; i < cnt; ++i) {
+    Engine &E = *es[i];
+    if (did[i]) {
+      HIPC(hipStreamSynchronize(E.s));
+      E.insert_finish();
+      rs[i].insertion_time = secs(tb);
+      E.insertion_completed = E.insertion_queue.size() == 0 && E.n == E.N;
+    }
+    if (!(ops[3 * i + 1] == 0 || E.update_completed)) {
+      const auto t0 = clk::now();
+      uint64_t ch = 0;
+      rs[i].n_update_ops = E.update(ops[3 * i + 1], ch);
+      rs[i].n_update_changed_edges = ch;
+      HIPC(hipStreamSynchronize(E.s));
+      rs[i].update_time = secs(t0);
+    }
+    rs[i].embedding_pending_edges = E.n;
+    emb[i] = E.embedding_due(ops[3 * i + 2]);
+    if (!emb[i])
+      continue;
+    te[i] = clk::now();
+    rs[i].n_embedding_ops = E.embed_begin(ops[3 * i + 2], &jobs[i]);
+    if (!jobs[i].nwaves) {
+      HIPC(hipStreamSynchronize(E.s));
+      continue;
+    }
+    size_t g = group_of(E.c);
+    if (open_waves[g] + jobs[i].nwaves > BATCH_MAX_WAVES ||
+        open[g].size() >= (size_t)SGD_MAX_JOBS) {
+      launch_group(open[g]);
+      open[g].clear();
+      open_waves[g] = 0;
+      g = group_of(E.c);
+    }
+    open[g].push_back(i);
+    open_waves[g] += jobs[i].nwaves;
+    if (open[g].size() >= gmax) {
+      launch_group(open[g]);
+      open[g].clear();
+      open_waves[g] = 0;
+    }
   }
+  for (auto &m : open)
+    launch_group(m);
   for (int i = 0; i < cnt; ++i)
-    if (did[i])
+    if (emb[i])
       es[i]->embed_end();
   for (int i = 0; i < cnt; ++i)
-    if (did[i])
+    if (emb[i])
       HIPC(hipStreamSynchronize(es[i]->s));
   HIPC(hipStreamSynchronize(bc.s));
-  t = secs(t0);
   for (int i = 0; i < cnt; ++i)
-    if (did[i])
-      rs[i].embedding_time = t;
+    if (emb[i])
+      rs[i].embedding_time = secs(te[i]);
   for (int q = 0; q < nchunks; ++q)
     if (bc.h_flag[q])
       throw Fail(GM_ERR_HIP, "batched SGD device-wide barrier timed out");

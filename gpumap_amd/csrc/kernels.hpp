@@ -46,10 +46,11 @@ __global__ void k_sync_rows(const uint64_t *pool, const uint32_t *rows,
                             unsigned long long *fresh, float *pmax, int euclid,
                             double *sigma, double *rho);
 __global__ void k_flag_count(const uint8_t *flag, uint64_t lo, uint64_t hi,
-                             uint32_t *bcount);
+                             uint32_t *bcount, uint64_t f0, uint64_t f1);
 __global__ void k_scan_small(uint32_t *v, uint64_t n, uint32_t *total);
 __global__ void k_flag_write(uint8_t *flag, uint64_t lo, uint64_t hi,
-                             const uint32_t *boff, uint32_t *out);
+                             const uint32_t *boff, uint32_t *out, uint64_t f0,
+                             uint64_t f1);
 __global__ void k_init_embedding(float *emb, int c, const uint32_t *pidx,
                                  const float *pdist, int k, uint64_t b0,
                                  uint64_t nb);

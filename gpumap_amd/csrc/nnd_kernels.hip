@@ -716,11 +716,12 @@ __global__ __launch_bounds__(256) void k_sync_rows(
 }
 
 // ------------------------------------------------------------ compaction
-// Ascending list of rows with flag != 0 in [lo, hi) (flags are cleared).
-// Pass 1 counts per 1024-row block, pass 2 scans the block counts in one
-// block, pass 3 writes.
+// Ascending list of rows in [lo, hi) with flag != 0 or in [f0, f1) (flags are
+// cleared).  Pass 1 counts per 1024-row block, pass 2 scans the block counts
+// in one block, pass 3 writes.
 __global__ void k_flag_count(const uint8_t *__restrict__ flag, uint64_t lo,
-                             uint64_t hi, uint32_t *__restrict__ bcount) {
+                             uint64_t hi, uint32_t *__restrict__ bcount,
+                             uint64_t f0, uint64_t f1) {
   __shared__ uint32_t s_c;
   if (threadIdx.x == 0)
     s_c = 0;
@@ -729,7 +730,7 @@ __global__ void k_flag_count(const uint8_t *__restrict__ flag, uint64_t lo,
   const uint64_t r0 = lo + (uint64_t)blockIdx.x * 1024;
   for (int i = threadIdx.x; i < 1024; i += blockDim.x) {
     const uint64_t r = r0 + i;
-    c += (r < hi && flag[r]) ? 1u : 0u;
+    c += (r < hi && (flag[r] || (r >= f0 && r < f1))) ? 1u : 0u;
   }
   atomicAdd(&s_c, c);
   __syncthreads();
@@ -765,7 +766,8 @@ __global__ void k_scan_small(uint32_t *__restrict__ v, uint64_t n,
 
 __global__ void k_flag_write(uint8_t *__restrict__ flag, uint64_t lo,
                              uint64_t hi, const uint32_t *__restrict__ boff,
-                             uint32_t *__restrict__ out) {
+                             uint32_t *__restrict__ out, uint64_t f0,
+                             uint64_t f1) {
   // one wave walks the block's 1024 rows in order, 64 at a time
   const uint64_t r0 = lo + (uint64_t)blockIdx.x * 1024;
   if (threadIdx.x >= 64)
@@ -773,7 +775,7 @@ __global__ void k_flag_write(uint8_t *__restrict__ flag, uint64_t lo,
   uint32_t at = boff[blockIdx.x];
   for (int i = 0; i < 1024; i += 64) {
     const uint64_t r = r0 + i + threadIdx.x;
-    const bool f = r < hi && flag[r];
+    const bool f = r < hi && (flag[r] || (r >= f0 && r < f1));
     const uint64_t m = __ballot(f);
     if (f) {
       out[at + __popcll(m & ((1ull << threadIdx.x) - 1ull))] = (uint32_t)r;

@@ -1,0 +1,293 @@
+"""Serving form of the progressive engine: many sessions on one device.
+
+No counterpart in the reference, where one ``GPUMAP`` object runs at a time
+(python/gpumap/gpumap.py:33-162).  A :class:`Server` admits sessions while
+their device footprint fits its budget and runs their requests by continuous
+batching: a loop thread takes every pending request (at most one per session)
+and runs them as one batched cycle (``_core.run_batch``, the C ABI's
+``gm_run_batch``).  In a batched cycle the sessions' insertion work overlaps
+on the device and their per-cycle SGDs share one cooperative launch; a
+request that arrives while a cycle runs joins the next one, and a session
+that closes frees its footprint for the sessions waiting to be admitted.
+
+Each session's results are those of ``GPUMAP.run`` on its own (the same
+graph bit for bit, the same op counts and queue states); only the phase
+times are the batch's, and the Hogwild SGD makes embeddings agree only in
+distribution, as two runs alone do.
+"""
+
+from __future__ import annotations
+
+import threading
+from concurrent.futures import Future
+
+from .gpumap import GPUMAP
+from ._native import _core
+
+_POOL_W = 32          # pool slots per row
+_SURVIVOR_CAP = 2048  # BF16 screening survivors per batch row
+_LB_STORE_MAX = 8 << 30
+
+
+def session_footprint(n_instances: int, n_features: int, n_neighbors: int = 15,
+                      n_components: int = 2, knn_mode: int = 2,
+                      max_batch_rows: int = 1000) -> int:
+    """Device bytes a session of this shape allocates (the engine's session
+    arrays plus the scratch of insertions of up to `max_batch_rows` rows)."""
+    n, d, k, c = n_instances, n_features, n_neighbors, n_components
+    dp = (d + 3) // 4 * 4
+    per_row = (4 * dp                    # rows
+               + 8 * _POOL_W * 4         # pool, reverse lists (2), flagged ids
+               + 8 * _POOL_W             # candidate lists (2 x u32)
+               + 8 * k                   # public rows
+               + 5 * k + 48 * k          # weights, flags, COO and schedule
+               + 8 * c                   # embedding and its snapshot
+               + 16 + 48)                # rho, sigma, per-row counters
+    total = n * per_row
+    b = max_batch_rows
+    if knn_mode == 2:
+        db = (d + 15) // 16 * 16
+        total += (n + 127) // 128 * 128 * (4 * db + 4)   # split rows, norms
+        total += (b + 127) // 128 * 128 * 4 * db          # batch operands
+        total += b * _SURVIVOR_CAP * 12 + b * 128 * _POOL_W * 8
+        if n * b * 4 <= _LB_STORE_MAX:
+            total += n * b * 4                            # lower-bound store
+    return int(total)
+
+
+def _device_free_bytes() -> int | None:
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            return int(torch.cuda.mem_get_info()[0])
+    except Exception:  # pragma: no cover - torch is part of the image
+        pass
+    return None
+
+
+class Admission:
+    """Footprint bookkeeping of a server: `acquire` waits until a footprint
+    fits the budget (and a session slot is free), `release` frees it."""
+
+    def __init__(self, budget_bytes: int, max_sessions: int):
+        self.budget = int(budget_bytes)
+        self.max_sessions = int(max_sessions)
+        self.used = 0
+        self.sessions = 0
+        self._cv = threading.Condition()
+
+    def acquire(self, nbytes: int, timeout: float | None = None) -> None:
+        if nbytes > self.budget:
+            raise MemoryError(f"session needs {nbytes} bytes, more than the "
+                              f"server's budget of {self.budget}")
+        with self._cv:
+            ok = self._cv.wait_for(
+                lambda: (self.used + nbytes <= self.budget
+                         and self.sessions < self.max_sessions), timeout)
+            if not ok:
+                raise TimeoutError("no room for the session before the "
+                                   "admission timeout")
+            self.used += nbytes
+            self.sessions += 1
+
+    def release(self, nbytes: int) -> None:
+        with self._cv:
+            self.used -= nbytes
+            self.sessions -= 1
+            self._cv.notify_all()
+
+
+class ServedSession:
+    """A session admitted by a :class:`Server`: the ``GPUMAP`` surface, with
+    run() and run_with_n_ops() served in batched cycles.  The getters return
+    views of the session's device arrays; read them between the session's
+    own requests (a cycle in flight writes them)."""
+
+    def __init__(self, server: "Server", model: GPUMAP, footprint: int):
+        self._server = server
+        self._model = model
+        self.footprint = footprint
+        self._closed = False
+
+    @property
+    def params(self):
+        return self._model.params
+
+    @property
+    def strategy(self):
+        return self._model.strategy
+
+    @property
+    def n_inserted_instances(self) -> int:
+        return self._model.n_inserted_instances
+
+    def submit(self, target_latency: float | None = None,
+               n_ops: tuple[int, int, int] | None = None) -> Future:
+        """Queue one cycle: by the strategy for `target_latency`, or the
+        given (insertion, update, embedding) op counts."""
+        if self._closed:
+            raise RuntimeError("session is closed")
+        return self._server._submit(self, target_latency, n_ops)
+
+    def run(self, target_latency: float = 10.0):
+        return self.submit(target_latency=target_latency).result()
+
+    def run_with_n_ops(self, n_insertion_ops: int, n_update_ops: int,
+                       n_embedding_ops: int):
+        return self.submit(n_ops=(n_insertion_ops, n_update_ops,
+                                  n_embedding_ops)).result()
+
+    def get_embedding(self):
+        return self._model.get_embedding()
+
+    def get_neighbor_indices(self):
+        return self._model.get_neighbor_indices()
+
+    def get_neighbor_dists(self):
+        return self._model.get_neighbor_dists()
+
+    def close(self):
+        if not self._closed:
+            self._closed = True
+            self._server._close_session(self)
+
+
+class Server:
+    """Continuous batching of many sessions on the current device.
+
+    `hbm_budget_bytes` defaults to 90% of the device memory free when the
+    server starts; `max_sessions` bounds the sessions admitted at once and
+    `max_batch_rows` the insertion batch used in the footprint estimate."""
+
+    def __init__(self, hbm_budget_bytes: int | None = None,
+                 max_sessions: int = 64, max_batch_rows: int = 1000):
+        if hbm_budget_bytes is None:
+            free = _device_free_bytes()
+            if free is None:
+                raise RuntimeError("no device to serve on")
+            hbm_budget_bytes = int(0.9 * free)
+        self.admission = Admission(hbm_budget_bytes, max_sessions)
+        self.max_batch_rows = max_batch_rows
+        self._cv = threading.Condition()
+        self._pending: dict[ServedSession, tuple] = {}
+        self._queued: list[tuple] = []
+        self._stop = False
+        self.cycles = 0
+        self._thread = threading.Thread(target=self._loop, daemon=True,
+                                        name="gpumap-serve")
+        self._thread.start()
+
+    # -- sessions --------------------------------------------------------
+    def open(self, data_loader, params, strategy=None,
+             insertion_estimator=None, update_estimator=None,
+             embedding_estimator=None, default_n_ops: int = 100,
+             admission_timeout: float | None = None) -> ServedSession:
+        """Admit a session (waiting until its footprint fits) and create it
+        with the ``GPUMAP`` constructor's arguments."""
+        n = params.n_instances or data_loader.n_instances
+        d = params.n_features or data_loader.n_features
+        mode = getattr(params, "knn_mode", "exact_bf16")
+        mode = {"nndescent": 0, "exact_fp32": 1, "exact_bf16": 2}.get(mode, mode)
+        fp = session_footprint(n, d, params.n_neighbors, params.n_components,
+                               int(mode), self.max_batch_rows)
+        self.admission.acquire(fp, admission_timeout)
+        try:
+            model = GPUMAP(data_loader, params, strategy, insertion_estimator,
+                           update_estimator, embedding_estimator,
+                           default_n_ops)
+        except BaseException:
+            self.admission.release(fp)
+            raise
+        return ServedSession(self, model, fp)
+
+    def _close_session(self, s: ServedSession) -> None:
+        with self._cv:
+            # a queued request of a closing session is cancelled
+            req = self._pending.pop(s, None)
+            self._queued = [q for q in self._queued if q[0] is not s]
+        if req is not None:
+            req[2].cancel()
+        s._model.close()
+        self.admission.release(s.footprint)
+
+    # -- requests --------------------------------------------------------
+    def _submit(self, s, target_latency, n_ops) -> Future:
+        fut: Future = Future()
+        with self._cv:
+            if self._stop:
+                raise RuntimeError("server is closed")
+            if s in self._pending:
+                # one request per session per cycle; later ones queue
+                self._queued.append((s, (target_latency, n_ops), fut))
+            else:
+                self._pending[s] = ((target_latency, n_ops), None, fut)
+            self._cv.notify_all()
+        return fut
+
+    def _loop(self) -> None:
+        while True:
+            with self._cv:
+                self._cv.wait_for(lambda: self._pending or self._stop)
+                if self._stop and not self._pending:
+                    return
+                batch = list(self._pending.items())
+                self._pending.clear()
+                # the next queued request of each session joins the
+                # following cycle
+                rest = []
+                for q in self._queued:
+                    if q[0] not in self._pending:
+                        self._pending[q[0]] = (q[1], None, q[2])
+                    else:
+                        rest.append(q)
+                self._queued = rest
+            self._run_cycle(batch)
+
+    def _run_cycle(self, batch) -> None:
+        live, ops = [], []
+        for s, ((target_latency, n_ops), _, fut) in batch:
+            if not fut.set_running_or_notify_cancel():
+                continue
+            m = s._model
+            try:
+                if n_ops is None:
+                    a = m.strategy.allocate(target_latency)
+                    ni = min(a[0], m.data_loader.n_instances
+                             - m.n_inserted_instances)
+                    o = (ni, a[1], a[2])
+                else:
+                    o = tuple(int(v) for v in n_ops)
+                m._feed(o[0])
+            except BaseException as e:
+                fut.set_exception(e)
+                continue
+            live.append((s, fut))
+            ops.append(o)
+        if not live:
+            return
+        try:
+            results = _core.run_batch([s._model._impl for s, _ in live], ops)
+        except BaseException as e:
+            for _, fut in live:
+                fut.set_exception(e)
+            return
+        self.cycles += 1
+        for (s, fut), r in zip(live, results):
+            try:
+                fut.set_result(s._model._finish(r))
+            except BaseException as e:  # pragma: no cover - strategy errors
+                fut.set_exception(e)
+
+    def close(self) -> None:
+        """Finish the queued requests and stop the loop."""
+        with self._cv:
+            self._stop = True
+            self._cv.notify_all()
+        self._thread.join()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

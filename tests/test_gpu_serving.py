@@ -103,3 +103,106 @@ def test_batch_rejects_duplicate_sessions():
     e.append(x)
     with pytest.raises(Exception):
         capi.Engine.run_batch([e, e], [(100, 100, 100), (100, 100, 100)])
+
+
+def _params(n, d, k=15):
+    from gpumap_amd import GPUMAPParams
+
+    p = GPUMAPParams()
+    p.n_instances, p.n_features, p.n_neighbors = n, d, k
+    return p
+
+
+def _drive(session, n, chunk=400):
+    """A client: run_with_n_ops per chunk, then until the update completes,
+    then one full sweep; returns the result log."""
+    log, pos = [], 0
+    while True:
+        ins = min(chunk, n - pos)
+        r = session.run_with_n_ops(ins, chunk if ins else n,
+                                   chunk if ins else n)
+        pos += ins
+        log.append([getattr(r, f) for f in RESULT_FIELDS])
+        if pos == n and r.update_completed and not ins:
+            return log
+
+
+def test_server_clients_in_threads_equal_sessions_alone():
+    import threading
+
+    import torch
+
+    from gpumap_amd import GPUMAP
+    from gpumap_amd.data_loader import ArrayDataLoader
+    from gpumap_amd.serving import Server
+
+    xs = [ev.gaussian10(n=1600, d=24, index=7),
+          ev.image_like(n=1200, d=96, index=8),
+          ev.gaussian10(n=2000, d=48, index=9)]
+    alone = []
+    for x in xs:
+        m = GPUMAP(ArrayDataLoader(x), _params(*x.shape))
+        log = _drive(m, len(x))
+        alone.append((log, m.get_neighbor_indices().cpu().numpy(),
+                      m.get_embedding().cpu().numpy()))
+        m.close()
+    with Server(max_sessions=8) as srv:
+        sess = [srv.open(ArrayDataLoader(x), _params(*x.shape)) for x in xs]
+        logs = [None] * len(xs)
+
+        def client(i):
+            logs[i] = _drive(sess[i], len(xs[i]))
+
+        th = [threading.Thread(target=client, args=(i,))
+              for i in range(len(xs))]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join(timeout=120)
+        assert all(log is not None for log in logs)
+        # the three clients shared cycles
+        assert srv.cycles < sum(len(log) for log in logs)
+        for i, x in enumerate(xs):
+            assert logs[i] == alone[i][0]
+            idx = sess[i].get_neighbor_indices().cpu().numpy()
+            assert np.array_equal(idx, alone[i][1])
+            emb = sess[i].get_embedding().cpu().numpy()
+            ta = ev.trustworthiness(x, alone[i][2], sample=1000)
+            tm = ev.trustworthiness(x, emb, sample=1000)
+            assert abs(ta - tm) < 0.01, (i, ta, tm)
+        for s in sess:
+            s.close()
+        assert srv.admission.sessions == 0 and srv.admission.used == 0
+    torch.cuda.synchronize()
+
+
+def test_server_admits_a_waiting_session_when_one_closes():
+    import threading
+
+    from gpumap_amd.data_loader import ArrayDataLoader
+    from gpumap_amd.serving import Server, session_footprint
+
+    x = ev.gaussian10(n=800, d=16, index=11)
+    fp = session_footprint(800, 16, max_batch_rows=1000)
+    with Server(hbm_budget_bytes=fp * 2 + fp // 2, max_sessions=8) as srv:
+        a = srv.open(ArrayDataLoader(x), _params(800, 16))
+        b = srv.open(ArrayDataLoader(x), _params(800, 16))
+        admitted = []
+
+        def late():
+            admitted.append(srv.open(ArrayDataLoader(x), _params(800, 16)))
+
+        t = threading.Thread(target=late)
+        t.start()
+        a.run_with_n_ops(400, 400, 400)
+        t.join(timeout=0.2)
+        assert not admitted            # no room for a third session yet
+        a.close()
+        t.join(timeout=60)
+        assert admitted
+        c = admitted[0]
+        r1 = b.run_with_n_ops(400, 400, 400)
+        r2 = c.run_with_n_ops(400, 400, 400)
+        assert r1.n_inserted_instances == r2.n_inserted_instances == 400
+        b.close()
+        c.close()
