@@ -183,17 +183,16 @@ __global__ void k_membership(const uint32_t *__restrict__ pidx,
 
 // ------------------------------------------------------ COO compaction
 // Keep the live edges with w >= max_w / n_epochs (reference trim_graph
-// :348-400), directed entries first then the reverse entries, each in
-// (batch, slot) order: a deterministic COO.
+// :348-400) in (batch, slot) order, each kept directed edge (i, j) as a
+// twin pair of slots 2p, 2p + 1: (i, j), then its reverse (j, i) where the
+// union holds it as an edge of its own (i absent from j's list; the same
+// weight, so kept with it), else an empty slot (head = tail = -1, w = 0).
+// The SGD kernels skip empty slots and use the pairing to add both edges'
+// contributions to a row with one atomic.  A deterministic COO with the
+// reference's edge set.
 __device__ inline bool coo_keep(const float *w, const uint8_t *flags,
-                                uint64_t nd, uint64_t t, float thr) {
-  const uint64_t s = t < nd ? t : t - nd;
-  const uint8_t f = flags[s];
-  if (!(f & 1))
-    return false;
-  if (t >= nd && !(f & 2))
-    return false;
-  return w[s] >= thr;
+                                uint64_t s, float thr) {
+  return (flags[s] & 1) && w[s] >= thr;
 }
 
 __device__ inline float coo_thr(const uint32_t *maxw_bits, int n_epochs) {
@@ -214,7 +213,7 @@ __global__ void k_coo_count(const float *__restrict__ w,
   uint32_t c = 0;
   for (int i = threadIdx.x; i < 1024; i += blockDim.x) {
     const uint64_t t = t0 + i;
-    c += (t < 2 * nd && coo_keep(w, flags, nd, t, thr)) ? 1u : 0u;
+    c += (t < nd && coo_keep(w, flags, t, thr)) ? 2u : 0u;
   }
   atomicAdd(&s_c, c);
   __syncthreads();
@@ -237,20 +236,25 @@ __global__ void k_coo_write(const uint32_t *__restrict__ pidx, int k,
   const uint64_t t0 = (uint64_t)blockIdx.x * 1024;
   uint32_t at = boff[blockIdx.x];
   for (int i = 0; i < 1024; i += 64) {
-    const uint64_t t = t0 + i + threadIdx.x;
-    const bool f = t < 2 * nd && coo_keep(w, flags, nd, t, thr);
+    const uint64_t s = t0 + i + threadIdx.x;
+    const bool f = s < nd && coo_keep(w, flags, s, thr);
     const uint64_t m = __ballot(f);
     if (f) {
-      const uint64_t s = t < nd ? t : t - nd;
       const int32_t hi = (int32_t)batch[s / k];
       const int c = (int)(s % k);
       const int32_t hj = c == 0 ? hi : (int32_t)pidx[(uint64_t)hi * k + c - 1];
-      const uint32_t pos = at + __popcll(m & ((1ull << threadIdx.x) - 1ull));
-      head[pos] = t < nd ? hi : hj;
-      tail[pos] = t < nd ? hj : hi;
-      wout[pos] = w[s];
+      const bool rev = flags[s] & 2;
+      const float ws = w[s];
+      const uint32_t pos =
+          at + 2u * __popcll(m & ((1ull << threadIdx.x) - 1ull));
+      head[pos] = hi;
+      tail[pos] = hj;
+      wout[pos] = ws;
+      head[pos + 1] = rev ? hj : -1;
+      tail[pos + 1] = rev ? hi : -1;
+      wout[pos + 1] = rev ? ws : 0.0f;
     }
-    at += __popcll(m);
+    at += 2u * __popcll(m);
   }
 }
 
@@ -284,7 +288,9 @@ __global__ void k_sgd_init(const float *__restrict__ w,
   eps[e] = ep;
   nxt[e] = ep;
   nneg[e] = ep * nsr_inv;
-  atomicAdd(deg + head[e], 1);
+  const int32_t h = head[e];
+  if (h >= 0) // not an empty slot
+    atomicAdd(deg + h, 1);
 }
 
 __device__ inline float rounding_of(int md, float alpha) {
@@ -605,6 +611,105 @@ __device__ inline void seg_add(float *__restrict__ emb, int32_t row, Seg sg,
   }
 }
 
+// Runs of equal keys over the even lanes of a wave (stride 2, cut at the
+// 16-lane DPP rows); odd lanes are runs of their own.  m bit s (s = 1..3):
+// lane + 2^s is in the same run.
+__device__ inline Seg seg2_of(int32_t key) {
+  const int lane = threadIdx.x & 63;
+  const bool even = !(lane & 1);
+  const int32_t prev = __shfl_up(key, 2, 64);
+  const bool start = !even || (lane & 15) == 0 || prev != key;
+  const uint64_t sm = __ballot(even && start);
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  const int run = __popcll(sm & upto);
+  uint32_t m = 0;
+#pragma unroll
+  for (int s = 1; s < 4; ++s) {
+    const int ro = __shfl_down(run, 1 << s, 64);
+    if (even && (lane & 15) + (1 << s) < 16 && ro == run)
+      m |= 1u << s;
+  }
+  return Seg{m, start};
+}
+
+// seg_add over seg2_of's runs: DPP row shifts of 2, 4 and 8 lanes.
+template <int NC>
+__device__ inline void seg2_add(float *__restrict__ emb, int32_t row, Seg sg,
+                                float v[NC]) {
+#pragma unroll
+  for (int d = 0; d < NC; ++d) {
+    float o = dpp_from_above<1>(v[d]);
+    if (sg.m & 2u)
+      v[d] += o;
+    o = dpp_from_above<2>(v[d]);
+    if (sg.m & 4u)
+      v[d] += o;
+    o = dpp_from_above<3>(v[d]);
+    if (sg.m & 8u)
+      v[d] += o;
+  }
+  bool any = false;
+#pragma unroll
+  for (int d = 0; d < NC; ++d)
+    any |= v[d] != 0.0f;
+  if (sg.start && any) {
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      unsafeAtomicAdd(emb + (uint64_t)row * NC + d, v[d]);
+  }
+}
+
+// How a wave adds its edges' contributions.  The embedding's COO places
+// each edge (i, j) and its reverse (j, i) (or an empty slot) in lanes 2p,
+// 2p + 1: then each lane adds its own head's contribution and its twin's
+// tail's (the same row), lane 2p for i and lane 2p + 1 for j, and runs of
+// equal i over the even lanes share an atomic: about half the atomics of
+// adding each edge's head and tail on their own.  Any other wave (a caller's
+// COO) adds heads and tails by their own runs.  Either way every
+// contribution is added once, exactly (all are multiples of the rounding
+// grid).  Lanes without an edge carry negative row ids and zero
+// contributions.
+struct Agg {
+  bool paired;  // wave-uniform
+  Seg sh, st;   // unpaired: runs of heads and of tails
+  Seg sp;       // paired: runs of kp
+  int32_t kp;   // paired: the lane's row (its head, else its twin's tail)
+};
+
+__device__ inline Agg agg_of(int32_t hh, int32_t tt) {
+  const int32_t ph = __shfl_xor(hh, 1, 64), pt = __shfl_xor(tt, 1, 64);
+  const bool twins = hh == pt && tt == ph;
+  const bool ok = twins || hh < 0 || ph < 0;
+  Agg a;
+  a.paired = __ballot(!ok) == 0;
+  a.kp = hh >= 0 ? hh : pt;
+  if (a.paired) {
+    a.sp = seg2_of(a.kp);
+    a.sh = a.st = Seg{0u, true};
+  } else {
+    a.sh = seg_of(hh);
+    a.st = seg_of(tt);
+    a.sp = Seg{0u, true};
+  }
+  return a;
+}
+
+template <int NC>
+__device__ inline void agg_add(float *__restrict__ emb, const Agg &a,
+                               int32_t hh, int32_t tt, float gh[NC],
+                               float gt[NC]) {
+  if (a.paired) {
+    float tot[NC];
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      tot[d] = gh[d] + __shfl_xor(gt[d], 1, 64);
+    seg2_add<NC>(emb, a.kp, a.sp, tot);
+  } else {
+    seg_add<NC>(emb, tt, a.st, gt);
+    seg_add<NC>(emb, hh, a.sh, gh);
+  }
+}
+
 __device__ inline void flush_sgd_stats(unsigned long long *stats,
                                        unsigned long long c_pos,
                                        unsigned long long c_neg) {
@@ -634,8 +739,9 @@ __device__ inline void sgd_step(float *__restrict__ emb,
                                 float nsr_inv, uint64_t seed, float rounding,
                                 unsigned long long &c_pos,
                                 unsigned long long &c_neg) {
-  const bool valid = e < nnz;
-  const int32_t hv = valid ? head[e] : -1 - (int32_t)(threadIdx.x & 63);
+  const int32_t h0 = e < nnz ? head[e] : -1;
+  const bool valid = h0 >= 0; // an empty slot of the COO has head -1
+  const int32_t hv = valid ? h0 : -1 - (int32_t)(threadIdx.x & 63);
   const int32_t tv = valid ? tail[e] : -1 - (int32_t)(threadIdx.x & 63);
   float gh[NC], gt[NC];
 #pragma unroll
@@ -657,8 +763,7 @@ __device__ inline void sgd_step(float *__restrict__ emb,
       nneg[e] = nn;
     }
   }
-  seg_add<NC>(emb, tv, seg_of(tv), gt);
-  seg_add<NC>(emb, hv, seg_of(hv), gh);
+  agg_add<NC>(emb, agg_of(hv, tv), hv, tv, gh, gt);
 }
 
 // One epoch of the Hogwild edge sampler, one launch per epoch (used when the
@@ -744,11 +849,11 @@ __global__ __launch_bounds__(256) void k_sgd_fused(
   int32_t hh[EPT], tt[EPT];
   float ep[EPT], nx[EPT], nn[EPT];
   uint32_t dr[EPT][8]; // this epoch's first 8 negative-sample draws
-  Seg sh[EPT], st[EPT];
+  Agg ag[EPT];
 #pragma unroll
   for (int q = 0; q < EPT; ++q) {
     const uint64_t e = t0 + (uint64_t)q * stride;
-    if (e < nnz) {
+    if (e < nnz && head[e] >= 0) {
       hh[q] = head[e];
       tt[q] = tail[e];
       ep[q] = eps[e];
@@ -760,8 +865,7 @@ __global__ __launch_bounds__(256) void k_sgd_fused(
       nx[q] = INFINITY;
       nn[q] = 0.0f;
     }
-    sh[q] = seg_of(hh[q]);
-    st[q] = seg_of(tt[q]);
+    ag[q] = agg_of(hh[q], tt[q]);
     neg_draws(seed, e, dr[q]);
   }
   unsigned long long c_pos = 0, c_neg = 0;
@@ -787,8 +891,7 @@ __global__ __launch_bounds__(256) void k_sgd_fused(
       if (act)
         sgd_update<NC>(emb, hh[q], e, n, ep[q], nx[q], nn[q], ep_neg, alpha,
                        gamma, a, b, es, rounding, L, gh, gt, c_pos, c_neg);
-      seg_add<NC>(emb, tt[q], st[q], gt);
-      seg_add<NC>(emb, hh[q], sh[q], gh);
+      agg_add<NC>(emb, ag[q], hh[q], tt[q], gh, gt);
     }
     alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
     epoch_end(bar, gridDim.x, epoch, lag, peeked, seen);
@@ -820,7 +923,7 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
   const uint64_t e = (uint64_t)w * 64 + lane;
   int32_t hh, tt;
   float ep, nx, nn;
-  if (e < nnz) {
+  if (e < nnz && head[e] >= 0) {
     hh = head[e];
     tt = tail[e];
     ep = eps[e];
@@ -832,7 +935,7 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
     nx = INFINITY;
     nn = 0.0f;
   }
-  const Seg sh = seg_of(hh), st = seg_of(tt);
+  const Agg ag = agg_of(hh, tt);
   uint32_t dr[8];
   neg_draws(seed, e, dr);
   unsigned long long c_pos = 0, c_neg = 0;
@@ -860,8 +963,7 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
     if (act)
       sgd_update<NC>(emb, hh, e, n, ep, nx, nn, ep_neg, alpha, gamma, a, b,
                      es, rounding, L, gh, gt, c_pos, c_neg);
-    seg_add<NC>(emb, tt, st, gt);
-    seg_add<NC>(emb, hh, sh, gh);
+    agg_add<NC>(emb, ag, hh, tt, gh, gt);
     alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
     // boundary
     if (lane == 0)

@@ -893,7 +893,7 @@ struct Engine {
         pidx, pdist, n, k, euclid ? 1 : 0, sigma, rho, mean, dbatch, nb, w,
         wflags, scal + S_MAXW);
     LAUNCH_CHECK();
-    const unsigned nblk = blocks_for(2 * nd, 1024);
+    const unsigned nblk = blocks_for(nd, 1024);
     k_coo_count<<<nblk, 256, 0, s>>>(w, wflags, nd, scal + S_MAXW,
                                      (int)p.max_epoch, bcount);
     LAUNCH_CHECK();
@@ -1753,12 +1753,16 @@ int gm_debug_fuzzy(gm_handle *h, const uint32_t *batch, uint64_t nb,
     HIPC(hipMemcpyAsync(&cnt, e->scal + gm::S_NNZ, 4, hipMemcpyDeviceToHost,
                         e->s));
     HIPC(hipStreamSynchronize(e->s));
-    if (head)
-      HIPC(hipMemcpyAsync(head, e->head, cnt * 4ull, hipMemcpyDeviceToHost, e->s));
-    if (tail)
-      HIPC(hipMemcpyAsync(tail, e->tail, cnt * 4ull, hipMemcpyDeviceToHost, e->s));
-    if (w)
-      HIPC(hipMemcpyAsync(w, e->wout, cnt * 4ull, hipMemcpyDeviceToHost, e->s));
+    // the device COO holds twin pairs with empty slots (head -1); the caller
+    // gets the edges alone, in their order
+    std::vector<int32_t> hv(cnt), tv(cnt);
+    std::vector<float> wv(cnt);
+    HIPC(hipMemcpyAsync(hv.data(), e->head, cnt * 4ull, hipMemcpyDeviceToHost,
+                        e->s));
+    HIPC(hipMemcpyAsync(tv.data(), e->tail, cnt * 4ull, hipMemcpyDeviceToHost,
+                        e->s));
+    HIPC(hipMemcpyAsync(wv.data(), e->wout, cnt * 4ull, hipMemcpyDeviceToHost,
+                        e->s));
     if (sigma)
       HIPC(hipMemcpyAsync(sigma, e->sigma, e->n * 8, hipMemcpyDeviceToHost, e->s));
     if (rho)
@@ -1771,7 +1775,19 @@ int gm_debug_fuzzy(gm_handle *h, const uint32_t *batch, uint64_t nb,
       for (uint64_t i = 0; i < e->n; ++i)
         if (sigma[i] < 0.0)
           sigma[i] = 1e-3 * mean;
-    *nnz = cnt;
+    int64_t m = 0;
+    for (uint32_t i = 0; i < cnt; ++i) {
+      if (hv[i] < 0)
+        continue;
+      if (head)
+        head[m] = hv[i];
+      if (tail)
+        tail[m] = tv[i];
+      if (w)
+        w[m] = wv[i];
+      ++m;
+    }
+    *nnz = m;
   });
 }
 
@@ -1783,10 +1799,13 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
   int rc = guarded([&] {
     if (n == 0 || c < 1 || c > 4 || nnz < 0 || nsr < 1)
       throw Fail(GM_ERR_INVALID, "invalid SGD arguments");
-    for (int64_t i = 0; i < nnz; ++i)
+    for (int64_t i = 0; i < nnz; ++i) {
+      if (head[i] == -1 && tail[i] == -1)
+        continue; // an empty slot
       if (head[i] < 0 || (uint64_t)head[i] >= n || tail[i] < 0 ||
           (uint64_t)tail[i] >= n)
         throw Fail(GM_ERR_INVALID, "COO index out of range");
+    }
     gm_params q;
     gm_default_params(&q);
     q.n_instances = std::max<uint64_t>(n, (uint64_t)(nnz + 1) / 2 + 1);
@@ -1799,7 +1818,8 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
     e->n = n;
     float mw = 0.0f;
     for (int64_t i = 0; i < nnz; ++i)
-      mw = std::max(mw, w[i]);
+      if (head[i] >= 0)
+        mw = std::max(mw, w[i]);
     uint32_t sc[2] = {(uint32_t)nnz, 0};
     std::memcpy(&sc[1], &mw, 4);
     HIPC(hipMemcpyAsync(e->emb, emb, n * c * 4, hipMemcpyHostToDevice, e->s));

@@ -183,7 +183,9 @@ int gm_debug_fuzzy(gm_handle *h, const uint32_t *batch, uint64_t n_batch,
 /* SGD epochs of the engine on a caller-given embedding (n x c, host) and COO
  * (host), with the engine's kernels: all epochs in one cooperative launch
  * when the edges fit the resident grid, or (per_epoch != 0, or when they do
- * not fit) one launch per epoch. */
+ * not fit) one launch per epoch.  A slot with head = tail = -1 is empty (the
+ * engine's own COO keeps each edge's reverse in the slot after it, or an
+ * empty slot). */
 int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
                  const int32_t *tail, const float *w, int64_t nnz,
                  int32_t n_epochs, int32_t nsr, float a, float b, float alpha0,
@@ -200,7 +202,9 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
  *                        512 waves of edges, else k_sgd_fused) or, when
  *                        cooperative launch is unavailable, one launch per
  *                        epoch; work = {positive samples, negative samples,
- *                        edge-epochs scanned, launches that ran k_sgd_waves}
+ *                        COO slot-epochs scanned (edges and the empty slots
+ *                        of the twin-paired COO), launches that ran
+ *                        k_sgd_waves}
  *   GM_KSTAT_INSERT      insertions (all kernels of one insertion); work =
  *                        {(batch row, row) pairs, BF16 screening survivors
  *                        re-ranked in FP32, insertions redone by the FP32

@@ -184,6 +184,55 @@ def test_sgd_disjoint_edges_match_oracle():
     np.testing.assert_allclose(out_g, out_o, rtol=1e-5, atol=4e-6)
 
 
+def _twin_paired_wave():
+    """64 COO slots (one wave) in the engine's layout: each directed edge
+    (i, j) in an even slot, its reverse (j, i) or an empty slot after it;
+    runs of equal i over the even slots, one of them across a 16-lane row."""
+    spec = [(0, [10, 11, 12, 13]), (1, [14, 15]), (2, [16]),
+            (3, [17, 18, 19, 20, 21, 22, 23, 24]), (4, [25, 26, 27, 10, 11]),
+            (5, [28, 29, 30]), (6, [31, 32, 33, 14, 2, 3, 4, 34, 35])]
+    g = np.random.default_rng(11)
+    head, tail = [], []
+    for i, js in spec:
+        for j in js:
+            head.append(i)
+            tail.append(j)
+            rev = g.random() < 0.6
+            head.append(j if rev else -1)
+            tail.append(i if rev else -1)
+    assert len(head) == 64
+    return np.array(head, np.int32), np.array(tail, np.int32)
+
+
+@pytest.mark.parametrize("per_epoch", [False, True])
+def test_sgd_twin_paired_aggregation_equals_per_edge_aggregation(per_epoch):
+    # one wave, every edge at the maximum weight with 2 epochs and nsr 1:
+    # each edge is sampled once (epoch 1) with no negative samples and the
+    # wave reads every row before it adds.  The contributions are multiples
+    # of the rounding grid (2^-17 here) and the coordinates multiples of
+    # 2^-10, so every sum is exact and the result independent of how the
+    # contributions are grouped.  The twin-paired slots take the paired
+    # aggregation (both edges' contributions to a row in one atomic); the
+    # same edges compacted and ordered by tail take the per-head / per-tail
+    # runs.
+    h, t = _twin_paired_wave()
+    w = np.where(h >= 0, 1.0, 0.0).astype(np.float32)
+    live = h >= 0
+    order = np.lexsort((h[live], t[live]))
+    h2, t2 = h[live][order], t[live][order]
+    assert not (h2[0] == t2[1] and t2[0] == h2[1])  # not twins: unpaired
+    g = np.random.default_rng(12)
+    emb = (np.round(g.standard_normal((40, 2)) * 1024) / 1024).astype(np.float32)
+    paired = capi.sgd(emb, h, t, w, 2, 1, 1.577, 0.895, per_epoch=per_epoch)
+    plain = capi.sgd(emb, h2, t2, np.ones(h2.size, np.float32), 2, 1, 1.577,
+                     0.895, per_epoch=per_epoch)
+    np.testing.assert_array_equal(paired, plain)
+    moved = np.unique(np.concatenate([h2, t2]))
+    assert (paired[moved] != emb[moved]).any(axis=1).all()
+    rest = np.setdiff1d(np.arange(40), moved)
+    np.testing.assert_array_equal(paired[rest], emb[rest])
+
+
 def test_bf16_queue_overflow_redoes_insertion_with_fp32_tiles():
     # 2 distinct points repeated 3000 times: all pairs within a group are at
     # distance 0, so every duplicate survives the screening; once a row has
