@@ -94,13 +94,20 @@ __global__ void k_mean_partial(const uint32_t *__restrict__ pidx,
   }
 }
 
+// Also marks the batch rows in `mark` (when given) for k_membership and
+// k_reposition; k_reposition clears the marks.
 __global__ __launch_bounds__(256) void k_mean_final(
     const double *__restrict__ psum, const unsigned long long *__restrict__ pcnt,
-    int nparts, double *__restrict__ mean, uint32_t *__restrict__ reset3) {
+    int nparts, double *__restrict__ mean, uint32_t *__restrict__ reset3,
+    const uint32_t *__restrict__ batch, uint64_t nb,
+    uint8_t *__restrict__ mark) {
   // nparts <= 256 partials, summed as a fixed tree
   __shared__ double s_s[256];
   __shared__ unsigned long long s_c[256];
   const int t = threadIdx.x;
+  if (mark)
+    for (uint64_t r = t; r < nb; r += blockDim.x)
+      mark[batch[r]] = 1;
   s_s[t] = t < nparts ? psum[t] : 0.0;
   s_c[t] = t < nparts ? pcnt[t] : 0ull;
   __syncthreads();
@@ -131,15 +138,22 @@ __global__ __launch_bounds__(256) void k_mean_final(
 // ------------------------------------------------------------ membership
 // Directed weights of the batch rows and the fuzzy union w = p + q - p q
 // (reference compute_membership_strength_kernel :214-250 and
-// symmetrize_and_expand_kernel :266-320, without the self-loop).  flags bit 0
-// = live edge, bit 1 = i is absent from j's list (the reverse edge must be
-// emitted).  Also folds the maximum weight into *maxw_bits.
+// symmetrize_and_expand_kernel :266-320, without the self-loop).  Also folds
+// the maximum weight into *maxw_bits.  flags bit 0: live edge (i, j).  The
+// union holds (j, i) too when i is absent from j's list (the reverse edge,
+// same weight) or when j is a batch row with i in its list (j's own entry
+// for i, whose weight p + q - p q is the same double expression with the
+// operands swapped, so the same float).  The COO places (j, i) in the slot
+// after (i, j): bit 1 says that slot holds (j, i); in the second case the
+// entry of the larger row is not placed on its own (bit 2) since its twin
+// holds it.  `inb` marks the batch rows (null: every row is one).
 __global__ void k_membership(const uint32_t *__restrict__ pidx,
                              const float *__restrict__ pdist, uint64_t n,
                              int k, int euclid, const double *__restrict__ sigma,
                              const double *__restrict__ rho,
                              const double *__restrict__ mean_p,
                              const uint32_t *__restrict__ batch, uint64_t nb,
+                             const uint8_t *__restrict__ inb,
                              float *__restrict__ w, uint8_t *__restrict__ flags,
                              uint32_t *__restrict__ maxw_bits) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -169,7 +183,8 @@ __global__ void k_membership(const uint32_t *__restrict__ pidx,
         }
       }
       wf = (float)(p + q - p * q);
-      f = 1 | (found ? 0 : 2);
+      const bool twin = found && (!inb || inb[j]);
+      f = 1 | (!found || (twin && i < j) ? 2 : 0) | (twin && i > j ? 4 : 0);
     }
     w[t] = wf;
     flags[t] = f;
@@ -183,16 +198,15 @@ __global__ void k_membership(const uint32_t *__restrict__ pidx,
 
 // ------------------------------------------------------ COO compaction
 // Keep the live edges with w >= max_w / n_epochs (reference trim_graph
-// :348-400) in (batch, slot) order, each kept directed edge (i, j) as a
-// twin pair of slots 2p, 2p + 1: (i, j), then its reverse (j, i) where the
-// union holds it as an edge of its own (i absent from j's list; the same
-// weight, so kept with it), else an empty slot (head = tail = -1, w = 0).
-// The SGD kernels skip empty slots and use the pairing to add both edges'
-// contributions to a row with one atomic.  A deterministic COO with the
-// reference's edge set.
+// :348-400) in (batch, slot) order, each placed directed edge (i, j) as a
+// twin pair of slots 2p, 2p + 1: (i, j), then (j, i) where k_membership
+// found the union to hold it (the same weight, so kept with (i, j)), else an
+// empty slot (head = tail = -1, w = 0).  The SGD kernels skip empty slots
+// and use the pairing to add both edges' contributions to a row with one
+// atomic.  A deterministic COO with the reference's edge set.
 __device__ inline bool coo_keep(const float *w, const uint8_t *flags,
                                 uint64_t s, float thr) {
-  return (flags[s] & 1) && w[s] >= thr;
+  return (flags[s] & 5) == 1 && w[s] >= thr;
 }
 
 __device__ inline float coo_thr(const uint32_t *maxw_bits, int n_epochs) {
@@ -381,7 +395,7 @@ __global__ __launch_bounds__(256) void k_noise(float *__restrict__ emb, int c,
                         const uint32_t *__restrict__ batch, uint64_t nb,
                         const uint32_t *__restrict__ mn,
                         const uint32_t *__restrict__ mx, float noise_scale,
-                        uint64_t seed, uint8_t *__restrict__ in_batch) {
+                        uint64_t seed) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nb * (uint64_t)c)
     return;
@@ -389,9 +403,6 @@ __global__ __launch_bounds__(256) void k_noise(float *__restrict__ emb, int c,
   Rng rng(seed, (uint32_t)t, (uint32_t)(t >> 32), TAG_NOISE);
   const uint32_t row = batch[t / c];
   emb[(uint64_t)row * c + t % c] += rng.normal() * scale;
-  // batch rows are marked for k_reposition, which clears the marks
-  if (in_batch && t % c == 0)
-    in_batch[row] = 1;
 }
 
 // --------------------------------------------------------------- SGD

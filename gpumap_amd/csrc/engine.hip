@@ -879,19 +879,22 @@ struct Engine {
 
   // --------------------------------------------------------- fuzzy set
   // Builds the trimmed COO of the batch rows in dbatch[0, nb); leaves nnz in
-  // scal[S_NNZ] and the maximum weight in scal[S_MAXW].
+  // scal[S_NNZ] and the maximum weight in scal[S_MAXW].  A batch of fewer
+  // than n rows is marked in bmark (k_reposition, or the caller, clears it).
   void build_fuzzy(uint64_t nb) {
+    uint8_t *mark = nb < n ? bmark : nullptr;
     k_mean_partial<<<NPART, 256, 0, s>>>(pidx, pdist, n, k, euclid ? 1 : 0,
                                          psum, pcnt);
     LAUNCH_CHECK();
     static_assert(S_MIN == S_MAXW + 1 && S_MAX == S_MAXW + 2,
                   "k_mean_final resets three consecutive scalars");
-    k_mean_final<<<1, 256, 0, s>>>(psum, pcnt, NPART, mean, scal + S_MAXW);
+    k_mean_final<<<1, 256, 0, s>>>(psum, pcnt, NPART, mean, scal + S_MAXW,
+                                   dbatch, nb, mark);
     LAUNCH_CHECK();
     const uint64_t nd = nb * (uint64_t)k;
     k_membership<<<blocks_for(nd, 256), 256, 0, s>>>(
-        pidx, pdist, n, k, euclid ? 1 : 0, sigma, rho, mean, dbatch, nb, w,
-        wflags, scal + S_MAXW);
+        pidx, pdist, n, k, euclid ? 1 : 0, sigma, rho, mean, dbatch, nb, mark,
+        w, wflags, scal + S_MAXW);
     LAUNCH_CHECK();
     const unsigned nblk = blocks_for(nd, 1024);
     k_coo_count<<<nblk, 256, 0, s>>>(w, wflags, nd, scal + S_MAXW,
@@ -1064,14 +1067,13 @@ struct Engine {
 
     // noise (reference add_embedding_noise, embedding_kernel.cuh:953-1007);
     // the column-0 range accumulators were reset by k_mean_final, and the
-    // range pass also clears the SGD's head degrees; k_noise marks the batch
-    // rows for the reposition below when there is one
+    // range pass also clears the SGD's head degrees
     k_minmax_col0<<<std::min<unsigned>(blocks_for(n, 256), 64), 256, 0, s>>>(
         emb, n, c, scal + S_MIN, scal + S_MAX, deg);
     LAUNCH_CHECK();
     k_noise<<<blocks_for(nb * c, 256), 256, 0, s>>>(
         emb, c, dbatch, nb, scal + S_MIN, scal + S_MAX, 0.001f,
-        42ull ^ (uint64_t)embedding_sweeps_done, nb < n ? bmark : nullptr);
+        42ull ^ (uint64_t)embedding_sweeps_done);
     LAUNCH_CHECK();
     // the launch grids are sized for the most edges the batch can give (the
     // union keeps at most both directions of each of its nb k entries); the
@@ -1099,7 +1101,7 @@ struct Engine {
   // reposition of the rows outside the op's batch (reference
   // heuristic_neighbor_avg_kernel :811-894, impl.cuh:422-449), from a
   // snapshot so that no row reads a neighbour's new value; the batch rows
-  // were marked by k_noise and k_reposition clears the marks
+  // were marked by build_fuzzy and k_reposition clears the marks
   void embed_end() {
     const uint64_t nb = pend_nb;
     pend_nb = 0;
@@ -1744,11 +1746,18 @@ int gm_debug_fuzzy(gm_handle *h, const uint32_t *batch, uint64_t nb,
     gm::Engine *e = h->e;
     if (nb > e->n || nb == 0)
       throw Fail(GM_ERR_INVALID, "batch size out of range");
-    for (uint64_t i = 0; i < nb; ++i)
+    std::vector<bool> seen(e->n, false);
+    for (uint64_t i = 0; i < nb; ++i) {
       if (batch[i] >= e->n)
         throw Fail(GM_ERR_INVALID, "batch row out of range");
+      if (seen[batch[i]])
+        throw Fail(GM_ERR_INVALID, "duplicate batch row");
+      seen[batch[i]] = true;
+    }
     HIPC(hipMemcpyAsync(e->dbatch, batch, nb * 4, hipMemcpyHostToDevice, e->s));
     e->build_fuzzy(nb);
+    if (nb < e->n) // no reposition follows to clear the batch marks
+      HIPC(hipMemsetAsync(e->bmark, 0, e->n, e->s));
     uint32_t cnt = 0;
     HIPC(hipMemcpyAsync(&cnt, e->scal + gm::S_NNZ, 4, hipMemcpyDeviceToHost,
                         e->s));

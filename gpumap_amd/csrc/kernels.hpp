@@ -101,13 +101,14 @@ __global__ void k_mean_partial(const uint32_t *pidx, const float *pdist,
                                uint64_t n, int k, int euclid, double *psum,
                                unsigned long long *pcnt);
 __global__ void k_mean_final(const double *psum, const unsigned long long *pcnt,
-                             int nparts, double *mean, uint32_t *reset3);
+                             int nparts, double *mean, uint32_t *reset3,
+                             const uint32_t *batch, uint64_t nb, uint8_t *mark);
 __global__ void k_membership(const uint32_t *pidx, const float *pdist,
                              uint64_t n, int k, int euclid,
                              const double *sigma, const double *rho,
                              const double *mean, const uint32_t *batch,
-                             uint64_t nb, float *w, uint8_t *flags,
-                             uint32_t *maxw_bits);
+                             uint64_t nb, const uint8_t *inb, float *w,
+                             uint8_t *flags, uint32_t *maxw_bits);
 __global__ void k_coo_count(const float *w, const uint8_t *flags, uint64_t nd,
                             const uint32_t *maxw_bits, int n_epochs,
                             uint32_t *bcount);
@@ -127,7 +128,7 @@ __global__ void k_minmax_col0(const float *emb, uint64_t n, int c,
                               uint32_t *mn, uint32_t *mx, int *deg_zero);
 __global__ void k_noise(float *emb, int c, const uint32_t *batch, uint64_t nb,
                         const uint32_t *mn, const uint32_t *mx,
-                        float noise_scale, uint64_t seed, uint8_t *in_batch);
+                        float noise_scale, uint64_t seed);
 template <int NC>
 __global__ void k_sgd_epoch(float *emb, const int32_t *head,
                             const int32_t *tail, const uint32_t *nnz_p,

@@ -173,9 +173,10 @@ const char *gm_last_error(void);
 /* Raw 32-slot neighbour pools of the inserted rows, packed keys. */
 int gm_debug_pool(gm_handle *h, uint64_t *keys);
 
-/* Fuzzy set + trim for the given batch rows on the handle's current graph
- * (what one embed call builds before the SGD).  head/tail/w need room for
- * 2 * n_batch * n_neighbors entries; sigma/rho (may be NULL) n_inserted. */
+/* Fuzzy set + trim for the given batch rows (distinct) on the handle's
+ * current graph (what one embed call builds before the SGD).  head/tail/w
+ * need room for 2 * n_batch * n_neighbors entries; sigma/rho (may be NULL)
+ * n_inserted. */
 int gm_debug_fuzzy(gm_handle *h, const uint32_t *batch, uint64_t n_batch,
                    int32_t *head, int32_t *tail, float *w, double *sigma,
                    double *rho, int64_t *nnz);
