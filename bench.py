@@ -162,6 +162,181 @@ def measure(capi, xdev, n, d, mode, steps, warmup):
     return last, totals, lats
 
 
+def served(xdev, n, d, sessions, window=0.001):
+    """`sessions` sessions of the workload served together by the Server:
+    one client thread per session runs the headline's op plan through
+    run_with_n_ops, and the Server batches the pending requests into cycles.
+    Returns the wall time from the first request to the last result, every
+    request's latency, the cycle count and one session's model."""
+    import threading
+
+    import gpumap_amd as gm
+    from gpumap_amd.data_loader import ArrayDataLoader
+    from gpumap_amd.serving import Server
+
+    def client(sess, lat):
+        pos = 0
+        while True:
+            ins = min(CHUNK, n - pos)
+            plan = (ins, CHUNK, CHUNK) if ins else (0, n, n)
+            t0 = time.perf_counter()
+            r = sess.run_with_n_ops(*plan)
+            lat.append(time.perf_counter() - t0)
+            pos += ins
+            if pos == n and r.update_completed:
+                if plan[2] < n:
+                    t0 = time.perf_counter()
+                    sess.run_with_n_ops(0, 0, n)
+                    lat.append(time.perf_counter() - t0)
+                return
+
+    srv = Server(batch_window=window)
+    ss = []
+    for _ in range(sessions):
+        p = gm.GPUMAPParams()
+        p.knn_mode = "exact_bf16"
+        ss.append(srv.open(ArrayDataLoader(xdev), p))
+    lats = [[] for _ in ss]
+    th = [threading.Thread(target=client, args=(s, l)) for s, l in zip(ss, lats)]
+    c0 = srv.cycles
+    t0 = time.perf_counter()
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t0
+    cycles = srv.cycles - c0
+    keep = ss[0]
+    for s in ss[1:]:
+        s.close()
+    return srv, keep, wall, np.concatenate([np.array(l) for l in lats]), cycles
+
+
+def served_entry(ev, x, xdev, n, d, sessions, steps, fp32_value, no_quality):
+    srv, keep, wall, lat, cycles = served(xdev, n, d, sessions)  # warmup
+    keep.close()
+    srv.close()
+    walls, lats = [], []
+    for i in range(steps):
+        srv, keep, wall, lat, cycles = served(xdev, n, d, sessions)
+        walls.append(wall)
+        lats.append(lat)
+        if i < steps - 1:
+            keep.close()
+            srv.close()
+    lat = np.concatenate(lats)
+    value = sessions * n / float(np.mean(walls))
+    entry = {"sessions": sessions, "value": value, "unit": "points/s",
+             "wall_s": float(np.mean(walls)),
+             "request_latency_p50_ms": 1e3 * float(np.percentile(lat, 50)),
+             "request_latency_p99_ms": 1e3 * float(np.percentile(lat, 99)),
+             "cycles": cycles, "batch_window_ms": 1.0,
+             "vs_fp32_plain": value / fp32_value if fp32_value else None}
+    if not no_quality:
+        emb = keep.get_embedding().cpu().numpy()
+        entry["trustworthiness_at_15_session0"] = ev.trustworthiness(x, emb)
+    keep.close()
+    srv.close()
+    log(f"[served x{sessions}] {value:.0f} points/s"
+        + (f", {entry['vs_fp32_plain']:.2f}x the FP32 path" if fp32_value
+           else ""))
+    return entry
+
+
+def serving_mix(ev, n_sessions=64, rate=4.0, budget=0.5, seed=1003,
+                deadline=240.0, burst=False):
+    """The serving mix: `n_sessions` independent sessions arriving as a
+    Poisson process of `rate` per second, sizes log-uniform in [1e3, 1e5],
+    dims uniform in {32, 128, 784}, 10-cluster Gaussian mixtures; each
+    session runs run(target_latency=budget) until its insertion, update and
+    embedding have completed, served by one Server.  Sessions still running
+    `deadline` seconds after the start stop there and count as incomplete.
+    `burst` makes every session arrive at the start (the server's capacity
+    on the mix rather than the arrival rate)."""
+    import threading
+
+    import torch
+
+    import gpumap_amd as gm
+    from gpumap_amd.data_loader import ArrayDataLoader
+    from gpumap_amd.serving import Server
+
+    g = np.random.default_rng(seed)
+    sizes = np.exp(g.uniform(np.log(1e3), np.log(1e5), n_sessions)).astype(int)
+    dims = g.choice([32, 128, 784], n_sessions)
+    arrive = np.cumsum(g.exponential(1.0 / rate, n_sessions))
+    if burst:
+        arrive[:] = 0.0  # the same sessions, all arriving at once
+    data = [torch.from_numpy(ev.gaussian10(n=int(s), d=int(dd), index=100 + i))
+            .to("cuda:0") for i, (s, dd) in enumerate(zip(sizes, dims))]
+    torch.cuda.synchronize()
+    log(f"[serving mix] {n_sessions} sessions, {int(sizes.sum())} points "
+        f"staged in HBM")
+    srv = Server(batch_window=0.001)
+    lat = [[] for _ in range(n_sessions)]
+    span = [None] * n_sessions
+    err = []
+    complete = [False] * n_sessions
+
+    def session(i, t_start):
+        time.sleep(max(0.0, t_start + arrive[i] - time.perf_counter()))
+        t_arr = time.perf_counter()
+        try:
+            p = gm.GPUMAPParams()
+            p.knn_mode = "exact_bf16"
+            s = srv.open(ArrayDataLoader(data[i]), p)
+            while time.perf_counter() - t_start < deadline:
+                t0 = time.perf_counter()
+                r = s.run(target_latency=budget)
+                lat[i].append(time.perf_counter() - t0)
+                if (r.insertion_completed and r.update_completed
+                        and r.embedding_completed):
+                    complete[i] = True
+                    break
+            s.close()
+        except BaseException as e:  # reported in the JSON line
+            err.append(f"session {i}: {e!r}")
+        span[i] = (t_arr, time.perf_counter())
+        log(f"[serving mix] session {i} ({sizes[i]}x{dims[i]}) "
+            f"{'done' if complete[i] else 'stopped'} after {len(lat[i])} "
+            f"requests, {span[i][1] - span[i][0]:.2f} s; at "
+            f"{span[i][1] - t_start:.1f} s")
+
+    t_start = time.perf_counter()
+    th = [threading.Thread(target=session, args=(i, t_start))
+          for i in range(n_sessions)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    t_end = time.perf_counter()
+    srv.close()
+    allat = np.concatenate([np.array(v) for v in lat if v])
+    makespan = t_end - t_start
+    total = int(sizes.sum())
+    done = np.array([b - a for a, b in span])
+    return {
+        "sessions": n_sessions,
+        "arrival_rate_per_s": "all at once" if burst else rate,
+        "points": total, "dims": {str(k): int((dims == k).sum())
+                                  for k in (32, 128, 784)},
+        "size_min": int(sizes.min()), "size_max": int(sizes.max()),
+        "budget_ms": 1e3 * budget, "makespan_s": makespan,
+        "last_arrival_s": float(arrive[-1]),
+        "value": total / makespan, "unit": "points/s",
+        "request_latency_p50_ms": 1e3 * float(np.percentile(allat, 50)),
+        "request_latency_p99_ms": 1e3 * float(np.percentile(allat, 99)),
+        "requests": int(allat.size),
+        "requests_over_budget": int((allat > budget).sum()),
+        "session_time_p50_s": float(np.percentile(done, 50)),
+        "session_time_p99_s": float(np.percentile(done, 99)),
+        "incomplete": int(n_sessions - sum(complete)),
+        "cycles": srv.cycles, "max_sessions_per_cycle": srv.max_batch,
+        "mean_sessions_per_cycle": srv.requests_run / max(srv.cycles, 1),
+        "errors": err,
+    }
+
+
 def quality(ev, x, e, n):
     idx, _ = e.knn()
     q = np.random.Generator(np.random.PCG64(7)).choice(n, 2000, replace=False)
@@ -227,6 +402,11 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-quality", action="store_true")
+    ap.add_argument("--served", default="4,8",
+                    help="comma list of session counts served together on "
+                         "the workload ('' for none)")
+    ap.add_argument("--no-serving-mix", action="store_true",
+                    help="skip the 64-session serving mix")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device copy of MIB MiB (a kernel of "
                          "known traffic that scales rocprofv3 PMC counters)")
@@ -300,6 +480,21 @@ def main():
         log(f"[{m}] {entry['value']:.0f} points/s, headline is "
             f"{entry['headline_speedup']:.2f}x")
 
+    fp32 = compare.get("exact_fp32", {}).get("value")
+    served_out = {}
+    for k in [int(v) for v in args.served.split(",") if v]:
+        served_out[str(k)] = served_entry(ev, x, xdev, n, d, k, args.steps,
+                                          fp32, args.no_quality)
+    mix = None
+    if not args.no_serving_mix:
+        mix = serving_mix(ev)
+        mix["burst"] = serving_mix(ev, burst=True)
+        for m in (mix, mix["burst"]):
+            log(f"[serving mix] {m['value']:.0f} points/s over "
+                f"{m['makespan_s']:.1f} s, request p50 "
+                f"{m['request_latency_p50_ms']:.1f} ms p99 "
+                f"{m['request_latency_p99_ms']:.1f} ms")
+
     cpu = None
     if not args.no_cpu_baseline:
         # a sample of about 10-30 s of CPU work on 16 cores
@@ -336,6 +531,8 @@ def main():
         "kernels": kern,
         "roofline": roof,
         "compare": compare,
+        "served": served_out,
+        "serving_mix": mix,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -176,6 +176,16 @@ struct Model {
                          (uint64_t)info.shape[0], (uint64_t)info.shape[1]));
   }
 
+  // rows already in device memory (row-major float32 at `ptr`); the caller
+  // has finished writing them
+  void append_data_device(uintptr_t ptr, uint64_t rows, uint64_t cols) {
+    if (cols != params.p.n_features)
+      throw std::runtime_error(
+          "n_features mismatch with numpy array second dimension.");
+    check(gm_append_data_device(s->h, reinterpret_cast<const float *>(ptr),
+                                rows, cols));
+  }
+
   Result run(uint32_t ni, uint32_t nu, uint32_t ne) {
     Result r;
     int rc;
@@ -328,6 +338,8 @@ PYBIND11_MODULE(_core, m) {
       .def("initialize", &Model::initialize, py::arg("n_instances"),
            py::arg("n_features"))
       .def("append_data", &Model::append_data, py::arg("x"))
+      .def("append_data_device", &Model::append_data_device, py::arg("ptr"),
+           py::arg("rows"), py::arg("cols"))
       .def("run", &Model::run, py::arg("n_jobs_insertion"),
            py::arg("n_jobs_update"), py::arg("n_jobs_embedding"))
       .def("close", &Model::close)

@@ -810,8 +810,9 @@ __global__ __launch_bounds__(256) void k_sgd_epoch(
 // epoch's atomic updates stay in flight across the boundary.  Every thread
 // reads the counter at the start of the epoch (`peeked`, one request per
 // wave) and uses it only at the boundary, so the read's latency overlaps the
-// epoch's own memory accesses and the boundary usually waits for nothing.  The wait is
-// bounded: a boundary that times out sets bar[2] (the host reports it) and
+// epoch's own memory accesses and the boundary usually waits for nothing.
+// The wait is bounded: a boundary that times out sets bar[2] (the host
+// reports it), the later boundaries then stop waiting within 1024 polls, and
 // the kernel still runs to completion.
 __device__ inline unsigned bar_peek(unsigned *bar) {
   return __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -831,7 +832,8 @@ __device__ inline void epoch_end(unsigned *bar, unsigned nblocks, int epoch,
         if (seen >= target)
           break;
         __builtin_amdgcn_s_sleep(1);
-        if (++spins > (1u << 22)) {
+        if ((++spins & 1023u) == 0 &&
+            (spins > (1u << 22) || bar_peek(bar + 2))) {
           __hip_atomic_store(bar + 2, 1u, __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_AGENT);
           break;
@@ -993,7 +995,12 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
           m = min(m, (unsigned)__shfl_xor((int)m, off, 64));
         if (m >= target)
           break;
-        if (++spins > (1u << 20)) {
+        // bounded: a timed-out wait anywhere in the launch ends the others
+        // within 1024 polls
+        if ((++spins & 1023u) == 0 &&
+            (spins > (1u << 20) ||
+             __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                 timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))) {
           if (lane == 0)
             __hip_atomic_store(timeout, 1u, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);

@@ -1270,6 +1270,20 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
   std::vector<clk::time_point> te(cnt);
   int nchunks = 0;
   unsigned *const bar = bc.bar;
+  // completion of the last batched launch so far: an SGD that a session runs
+  // on its own stream (too large for the batched kernel) waits for it, since
+  // two grid-wide SGD kernels on the device at once could each hold CUs the
+  // other's waiting blocks need
+  hipEvent_t last_done = nullptr;
+  // the launches' completion events go back to their pools when the cycle
+  // ends (also when it throws), so none is re-recorded while waited on
+  struct DoneEvents {
+    std::vector<std::pair<Engine *, hipEvent_t>> v;
+    ~DoneEvents() {
+      for (auto &p : v)
+        p.first->ev_free.push_back(p.second);
+    }
+  } dones;
   // One batched SGD launch over `members` (one component count, at most
   // SGD_MAX_JOBS jobs and BATCH_MAX_WAVES waves), after their set-up; their
   // streams continue after it.  Launches run in order on the batch stream.
@@ -1319,6 +1333,7 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
                         hipMemcpyDeviceToHost, bc.s));
     hipEvent_t done = es[members[0]]->ev_get();
     HIPC(hipEventRecord(done, bc.s));
+    last_done = done;
     for (int i : members) {
       HIPC(hipStreamWaitEvent(es[i]->s, done, 0));
       ++es[i]->sgd_waves_launches;
@@ -1329,7 +1344,7 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
         LAUNCH_CHECK();
       }
     }
-    es[members[0]]->ev_free.push_back(done);
+    dones.v.emplace_back(es[members[0]], done);
     ++nchunks;
   };
   // One launch for all prepared sessions of a component count (a launch is
@@ -1368,6 +1383,8 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
     if (!emb[i])
       continue;
     te[i] = clk::now();
+    if (last_done)
+      HIPC(hipStreamWaitEvent(E.s, last_done, 0));
     rs[i].n_embedding_ops = E.embed_begin(ops[3 * i + 2], &jobs[i]);
     if (!jobs[i].nwaves) {
       HIPC(hipStreamSynchronize(E.s));

@@ -47,8 +47,11 @@ class NumpyDataLoader(DataLoader):
 
 
 class ArrayDataLoader(DataLoader):
+    """Chunks of an in-memory array: a host array, or a torch tensor on the
+    GPU whose chunks the facade appends by a device copy."""
+
     def __init__(self, X: np.ndarray, y: np.ndarray | None = None):
-        self.X = np.asarray(X)
+        self.X = X if getattr(X, "is_cuda", False) else np.asarray(X)
         self.y = y
         self.n_instances = int(self.X.shape[0])
         self.n_features = int(self.X.shape[1])

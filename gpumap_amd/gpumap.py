@@ -34,6 +34,26 @@ def find_ab_params(spread: float, min_dist: float) -> tuple[float, float]:
     return float(a), float(b)
 
 
+def _device_rows(chunk):
+    """A torch tensor on a GPU as (pointer, rows, cols) of C-contiguous
+    float32 rows, or None for anything else (host arrays take the reference's
+    path)."""
+    if not getattr(chunk, "is_cuda", False):
+        return None
+    import torch
+
+    if chunk.dim() != 2:
+        raise ValueError("data_chunk must be 2D array")
+    chunk = chunk.to(torch.float32).contiguous()
+    # the rows' producer is torch's current stream.  The session's stream is
+    # a blocking stream, so it already waits for work on the null stream; a
+    # side stream must finish first.
+    producer = torch.cuda.current_stream(chunk.device)
+    if producer.cuda_stream != 0:
+        producer.synchronize()
+    return chunk, chunk.data_ptr(), int(chunk.shape[0]), int(chunk.shape[1])
+
+
 def _to_device_tensor(arr):
     try:
         import torch
@@ -99,6 +119,14 @@ class GPUMAP:
         if n_insertion_ops <= 0:
             return
         chunk = self.data_loader.get_next_chunk(n_insertion_ops)
+        dev = _device_rows(chunk)
+        if dev is not None:
+            # rows resident in HBM (a loader over a torch GPU tensor): appended
+            # by a device copy, no host round trip
+            _, ptr, rows, cols = dev
+            if rows:
+                self._impl.append_data_device(ptr, rows, cols)
+            return
         try:
             validate_data_chunk(chunk, self._params.n_features)
             chunk = as_f32_c_contig_2d(chunk)

@@ -158,10 +158,19 @@ class Server:
 
     `hbm_budget_bytes` defaults to 90% of the device memory free when the
     server starts; `max_sessions` bounds the sessions admitted at once and
-    `max_batch_rows` the insertion batch used in the footprint estimate."""
+    `max_batch_rows` the rows a session inserts per cycle when its strategy
+    allocates the ops (run(target_latency)), which bounds both the footprint
+    estimate and the time one session's insertion adds to a shared cycle;
+    explicit op counts (run_with_n_ops) are run as given.  A
+    cycle starts once every session of the previous cycle that is still open
+    has its next request pending, or `batch_window` seconds after the first
+    pending request, whichever comes first (0 starts it at once with
+    whatever is pending): clients that answer each result with their next
+    request share every cycle, and idle sessions delay nothing."""
 
     def __init__(self, hbm_budget_bytes: int | None = None,
-                 max_sessions: int = 64, max_batch_rows: int = 1000):
+                 max_sessions: int = 64, max_batch_rows: int = 1000,
+                 batch_window: float = 0.001):
         if hbm_budget_bytes is None:
             free = _device_free_bytes()
             if free is None:
@@ -169,11 +178,15 @@ class Server:
             hbm_budget_bytes = int(0.9 * free)
         self.admission = Admission(hbm_budget_bytes, max_sessions)
         self.max_batch_rows = max_batch_rows
+        self.batch_window = float(batch_window)
+        self._last: list[ServedSession] = []
         self._cv = threading.Condition()
         self._pending: dict[ServedSession, tuple] = {}
         self._queued: list[tuple] = []
         self._stop = False
-        self.cycles = 0
+        self.cycles = 0          # batched cycles run
+        self.requests_run = 0    # requests over those cycles
+        self.max_batch = 0       # most sessions in one cycle
         self._thread = threading.Thread(target=self._loop, daemon=True,
                                         name="gpumap-serve")
         self._thread.start()
@@ -206,6 +219,7 @@ class Server:
             # a queued request of a closing session is cancelled
             req = self._pending.pop(s, None)
             self._queued = [q for q in self._queued if q[0] is not s]
+            self._cv.notify_all()
         if req is not None:
             req[2].cancel()
         s._model.close()
@@ -231,8 +245,16 @@ class Server:
                 self._cv.wait_for(lambda: self._pending or self._stop)
                 if self._stop and not self._pending:
                     return
+                if self.batch_window > 0:
+                    # let the other sessions' next requests join this cycle
+                    self._cv.wait_for(
+                        lambda: self._stop or all(
+                            s in self._pending or s._closed
+                            for s in self._last),
+                        self.batch_window)
                 batch = list(self._pending.items())
                 self._pending.clear()
+                self._last = [s for s, _ in batch]
                 # the next queued request of each session joins the
                 # following cycle
                 rest = []
@@ -253,7 +275,8 @@ class Server:
             try:
                 if n_ops is None:
                     a = m.strategy.allocate(target_latency)
-                    ni = min(a[0], m.data_loader.n_instances
+                    ni = min(a[0], self.max_batch_rows,
+                             m.data_loader.n_instances
                              - m.n_inserted_instances)
                     o = (ni, a[1], a[2])
                 else:
@@ -273,6 +296,8 @@ class Server:
                 fut.set_exception(e)
             return
         self.cycles += 1
+        self.requests_run += len(live)
+        self.max_batch = max(self.max_batch, len(live))
         for (s, fut), r in zip(live, results):
             try:
                 fut.set_result(s._model._finish(r))

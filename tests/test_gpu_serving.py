@@ -97,6 +97,31 @@ def test_batched_cycles_equal_each_session_alone():
         assert np.isfinite(em.embedding()).all()
 
 
+def test_own_stream_sgd_after_a_batched_launch_in_the_same_cycle():
+    # 16 sessions fill batched SGD launches (by job and wave count), which
+    # are issued before the 17th session's op: a full sweep too large for
+    # the batched kernel, run on the session's own stream.  That grid-wide
+    # kernel must start after the batched ones (two at once could each hold
+    # CUs the other's blocks wait for); the cycle ends without a barrier
+    # time-out.
+    small = [ev.gaussian10(n=600, d=8, index=10 + i) for i in range(17)]
+    big = ev.gaussian10(n=3000, d=16, index=40)
+    es = [capi.Engine(*x.shape, knn_mode=2) for x in small]
+    for e, x in zip(es, small):
+        e.append(x)
+    eb = capi.Engine(*big.shape, knn_mode=2)
+    eb.append(big)
+    r = eb.run(3000, 3000, 0)
+    assert r.update_completed
+    order = es[:16] + [eb] + es[16:]
+    ops = [(600, 0, 500)] * 16 + [(0, 0, 3000)] + [(600, 0, 500)]
+    rs = capi.Engine.run_batch(order, ops)
+    # the new rows' queue is drained whole (600 rows for 500 ops)
+    assert [q.n_embedding_ops for q in rs] == [600] * 16 + [3000] + [600]
+    for e in order:
+        assert np.isfinite(e.embedding()).all()
+
+
 def test_batch_rejects_duplicate_sessions():
     x = ev.gaussian10(n=600, d=8)
     e = capi.Engine(*x.shape)
