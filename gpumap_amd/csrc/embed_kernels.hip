@@ -491,15 +491,15 @@ __device__ inline void neg_step(float cur[NC], float g[NC], const float neg[NC],
   }
 }
 
+// the arithmetic of sgd_update; the caller advances the edge's schedule
 template <int NC>
-__device__ inline void sgd_update(const float *__restrict__ emb, int32_t j,
-                                  uint64_t e, uint32_t n, float ep, float &nx,
-                                  float &nn, float ep_neg, float alpha,
-                                  float gamma, float a, float b, uint64_t seed,
-                                  float rounding, EdgeLoads<NC> &L,
-                                  float gh[NC], float gt[NC],
-                                  unsigned long long &c_pos,
-                                  unsigned long long &c_neg) {
+__device__ inline void sgd_apply(const float *__restrict__ emb, int32_t j,
+                                 uint64_t e, uint32_t n, float ep_neg,
+                                 float alpha, float gamma, float a, float b,
+                                 uint64_t seed, float rounding,
+                                 EdgeLoads<NC> &L, float gh[NC], float gt[NC],
+                                 unsigned long long &c_pos,
+                                 unsigned long long &c_neg) {
   ++c_pos;
   float g[NC];
   float d2 = 0.0f;
@@ -520,7 +520,6 @@ __device__ inline void sgd_update(const float *__restrict__ emb, int32_t j,
     L.cur[d] += g[d];
     gt[d] = (rounding + -g[d]) - rounding;
   }
-  nx += ep;
   const int n_neg = L.n_neg;
   if (n_neg > 0) {
     const int cnt = min(8, n_neg);
@@ -547,11 +546,62 @@ __device__ inline void sgd_update(const float *__restrict__ emb, int32_t j,
         neg_step<NC>(L.cur, g, neg, alpha, gamma, a, b);
       }
     }
-    nn += (float)n_neg * ep_neg;
   }
 #pragma unroll
   for (int d = 0; d < NC; ++d)
     gh[d] = (rounding + g[d]) - rounding;
+}
+
+// the edge's schedule after a sample with L.n_neg negative samples
+__device__ inline void advance_schedule(float ep, float ep_neg, int n_neg,
+                                        float &nx, float &nn) {
+  nx += ep;
+  if (n_neg > 0)
+    nn += (float)n_neg * ep_neg;
+}
+
+template <int NC>
+__device__ inline void sgd_update(const float *__restrict__ emb, int32_t j,
+                                  uint64_t e, uint32_t n, float ep, float &nx,
+                                  float &nn, float ep_neg, float alpha,
+                                  float gamma, float a, float b, uint64_t seed,
+                                  float rounding, EdgeLoads<NC> &L,
+                                  float gh[NC], float gt[NC],
+                                  unsigned long long &c_pos,
+                                  unsigned long long &c_neg) {
+  sgd_apply<NC>(emb, j, e, n, ep_neg, alpha, gamma, a, b, seed, rounding, L,
+                gh, gt, c_pos, c_neg);
+  advance_schedule(ep, ep_neg, L.n_neg, nx, nn);
+}
+
+// sgd_gather without branches: every lane issues all ten loads (endpoints
+// and 8 negative samples); a lane without a sample this epoch, and samples
+// past its count, read row 0 (one cache line per wave).  With a fixed number
+// of loads the compiler's waits count exactly the operations issued after
+// them.
+template <int NC>
+__device__ inline void sgd_gather_all(const float *__restrict__ emb, bool act,
+                                      int32_t j, int32_t kk, uint32_t n,
+                                      float ep_neg, float nn, int epoch,
+                                      const uint32_t draw[8],
+                                      EdgeLoads<NC> &L) {
+  L.n_neg = act ? (int)(((float)epoch - nn) / ep_neg) : 0;
+  const int cnt = min(8, L.n_neg);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    L.r[q] = __umulhi(draw[q], n);
+    const uint32_t rq = q < cnt ? L.r[q] : 0u;
+    const float *np_ = emb + (uint64_t)rq * NC;
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      L.neg[q][d] = np_[d];
+  }
+  const uint64_t jj = act ? (uint64_t)j : 0, kq = act ? (uint64_t)kk : 0;
+#pragma unroll
+  for (int d = 0; d < NC; ++d) {
+    L.cur[d] = emb[jj * NC + d];
+    L.oth[d] = emb[kq * NC + d];
+  }
 }
 
 // Runs of equal row ids over the lanes of a wave (edges in COO order, so a
@@ -692,7 +742,8 @@ __device__ inline Agg agg_of(int32_t hh, int32_t tt) {
   const bool twins = hh == pt && tt == ph;
   const bool ok = twins || hh < 0 || ph < 0;
   Agg a;
-  a.paired = __ballot(!ok) == 0;
+  // uniform in a scalar register, so that its branches are scalar branches
+  a.paired = __builtin_amdgcn_readfirstlane(__ballot(!ok) == 0 ? 1 : 0) != 0;
   a.kp = hh >= 0 ? hh : pt;
   if (a.paired) {
     a.sp = seg2_of(a.kp);
@@ -719,6 +770,83 @@ __device__ inline void agg_add(float *__restrict__ emb, const Agg &a,
     seg_add<NC>(emb, tt, a.st, gt);
     seg_add<NC>(emb, hh, a.sh, gh);
   }
+}
+
+// ----------------------------------------------- branch-free row updates
+// The per-cycle SGD kernel issues the next epoch's loads before this epoch's
+// atomics, so that waiting for the loads does not wait for the atomics (the
+// vector memory counter completes in issue order).  That works only if the
+// operations issued after the loads do not depend on a branch (the compiler
+// then has to assume the fewest of them and waits for the rest): the row
+// updates go through a raw buffer over the embedding, and a lane that adds
+// nothing passes an offset past the buffer's end, which the hardware drops.
+constexpr uint32_t BUF_SKIP = 0x80000000u;
+
+template <int NC>
+__device__ inline __amdgpu_buffer_rsrc_t emb_rsrc(float *emb, uint32_t n) {
+  // num_records in bytes; rows stay below 2^31 / 16
+  return __builtin_amdgcn_make_buffer_rsrc(emb, 0, (int)(n * (NC * 4u)),
+                                           0x00020000);
+}
+
+// run sum into each run's first lane (seg_add's DPP steps from shift 2^S0)
+template <int NC, int S0>
+__device__ inline bool seg_sum(Seg sg, float v[NC]) {
+#pragma unroll
+  for (int d = 0; d < NC; ++d) {
+    if constexpr (S0 == 0) {
+      const float o = dpp_from_above<0>(v[d]);
+      if (sg.m & 1u)
+        v[d] += o;
+    }
+    float o = dpp_from_above<1>(v[d]);
+    if (sg.m & 2u)
+      v[d] += o;
+    o = dpp_from_above<2>(v[d]);
+    if (sg.m & 4u)
+      v[d] += o;
+    o = dpp_from_above<3>(v[d]);
+    if (sg.m & 8u)
+      v[d] += o;
+  }
+  bool any = false;
+#pragma unroll
+  for (int d = 0; d < NC; ++d)
+    any |= v[d] != 0.0f;
+  return sg.start && any;
+}
+
+template <int NC>
+__device__ inline void buf_row_add(__amdgpu_buffer_rsrc_t r, int32_t row,
+                                   bool wr, const float v[NC]) {
+  const uint32_t off =
+      (wr && row >= 0) ? (uint32_t)row * (NC * 4u) : BUF_SKIP;
+#pragma unroll
+  for (int d = 0; d < NC; ++d)
+    __builtin_amdgcn_raw_ptr_buffer_atomic_fadd_f32(v[d], r,
+                                                    (int)(off + 4u * d), 0, 0);
+}
+
+// agg_add without branches: always two sets of NC row updates (a paired
+// wave's second set is dropped), selected by the uniform `paired`.  seg2_of's
+// runs have no bit 0, so seg_sum's first step adds nothing for them.
+template <int NC>
+__device__ inline void agg_add_buf(__amdgpu_buffer_rsrc_t r, const Agg &a,
+                                   int32_t hh, int32_t tt, float gh[NC],
+                                   float gt[NC]) {
+  float v1[NC], v2[NC];
+#pragma unroll
+  for (int d = 0; d < NC; ++d) {
+    const float tot = gh[d] + __shfl_xor(gt[d], 1, 64);
+    v1[d] = a.paired ? tot : gt[d];
+    v2[d] = a.paired ? 0.0f : gh[d];
+  }
+  const Seg s1 = a.paired ? a.sp : a.st;
+  const int32_t k1 = a.paired ? a.kp : tt;
+  const bool w1 = seg_sum<NC, 0>(s1, v1);
+  const bool w2 = seg_sum<NC, 0>(a.sh, v2);
+  buf_row_add<NC>(r, k1, w1, v1);
+  buf_row_add<NC>(r, hh, w2 && !a.paired, v2);
 }
 
 __device__ inline void flush_sgd_stats(unsigned long long *stats,
@@ -912,17 +1040,45 @@ __global__ __launch_bounds__(256) void k_sgd_fused(
   flush_sgd_stats(stats, c_pos, c_neg);
 }
 
+// The wave's arrival slots, read without a branch: lanes past nw read the
+// last slot, and slot_min masks them out where the values are used (so that
+// the loads stay in flight until then).
+__device__ inline void poll_slots(const unsigned *__restrict__ slots,
+                                  unsigned nw, int lane, unsigned sv[8]) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const unsigned idx = (unsigned)lane + 64u * i;
+    sv[i] = __hip_atomic_load(slots + min(idx, nw - 1u), __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// the smallest slot value over the wave
+__device__ inline unsigned slot_min(const unsigned sv[8], unsigned nw,
+                                    int lane) {
+  unsigned m = ~0u;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const unsigned idx = (unsigned)lane + 64u * i;
+    m = min(m, sv[i] | (0u - (unsigned)(idx >= nw)));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    m = min(m, (unsigned)__shfl_xor((int)m, off, 64));
+  return m;
+}
+
 // All epochs in one cooperative launch for edge sets of at most 512 waves
 // (one edge per thread, one wave per block): the per-cycle embedding batches.
 // Epoch boundary without atomics or block barriers: wave w stores epoch + 1
 // into its arrival slot slots[w] when it finishes an epoch, and may start
 // epoch e + 1 once every slot of its edge set is >= e - lag + 1.  Each lane
-// reads up to 8 slots at the start of the epoch and the wave reduces them at
-// the boundary, so the read overlaps the epoch's own gathers and a wave that
-// is not more than `lag` epochs ahead waits for nothing.  The wait is bounded
-// (*timeout is set, as epoch_end's bar[2]).  sgd_waves_body is one wave's
-// work; k_sgd_waves runs one edge set, k_sgd_waves_multi several (one
-// session each) side by side, each paced only by its own waves.
+// reads up to 8 slots with the epoch's gathers and the wave reduces them at
+// the boundary, so a wave that is not more than `lag` epochs ahead waits for
+// nothing.  The wait is bounded (*timeout is set, as epoch_end's bar[2]).
+// sgd_waves_body is one wave's work; k_sgd_waves runs one edge set,
+// k_sgd_waves_multi several (one session each) side by side, each paced only
+// by its own waves.
 template <int NC>
 __device__ __attribute__((always_inline)) inline void sgd_waves_body(
     float *__restrict__ emb, const int32_t *__restrict__ head,
@@ -949,73 +1105,87 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
     nn = 0.0f;
   }
   const Agg ag = agg_of(hh, tt);
-  uint32_t dr[8];
-  neg_draws(seed, e, dr);
+  const __amdgpu_buffer_rsrc_t er = emb_rsrc<NC>(emb, n);
+  const float ep_neg = ep * nsr_inv;
   unsigned long long c_pos = 0, c_neg = 0;
   float alpha = alpha0;
-  for (int epoch = 0; epoch < n_epochs; ++epoch) {
-    unsigned sv[8];
+  // Two epochs in flight: epoch e + 1's gathers are issued before epoch e's
+  // arithmetic, so their memory latency overlaps it.  The schedule does not
+  // depend on the positions, so it is advanced when the gathers are issued;
+  // the positions that epoch e + 1 reads miss the updates of epoch e (the
+  // wave's own and others'), a Hogwild read as the lag already allows for
+  // other waves.  Per epoch, every lane issues, in this order: the next
+  // epoch's 10 gathers, its 8 slot reads, its row updates (2 x NC, dropped
+  // where there is nothing to add) and its arrival, so that every wait
+  // counts exactly the operations issued after what it waits for.
+  uint32_t dr[8];
+  neg_draws(seed, e, dr);
+  EdgeLoads<NC> L0, L1;
+  bool act0 = !(nx > 0.0f);
+  sgd_gather_all<NC>(emb, act0, hh, tt, n, ep_neg, nn, 0, dr, L0);
+  if (act0)
+    advance_schedule(ep, ep_neg, L0.n_neg, nx, nn);
+  neg_draws(seed + 1, e, dr);
+  unsigned sv[8];
+  poll_slots(slots, nw, lane, sv);
+  {
+    float z[NC];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const unsigned idx = (unsigned)lane + 64u * i;
-      sv[i] = idx < nw ? __hip_atomic_load(slots + idx, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT)
-                       : ~0u;
-    }
+    for (int d = 0; d < NC; ++d)
+      z[d] = 0.0f;
+    buf_row_add<NC>(er, -1, false, z);
+    buf_row_add<NC>(er, -1, false, z);
+    __hip_atomic_store(slots + w, 0u, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  for (int epoch = 0; epoch < n_epochs; ++epoch) {
     const uint64_t es = seed + (uint64_t)epoch;
-    const bool act = !(nx > (float)epoch);
-    const float ep_neg = ep * nsr_inv;
-    EdgeLoads<NC> L;
-    if (act)
-      sgd_gather<NC>(emb, hh, tt, n, ep_neg, nn, epoch, dr, L);
-    neg_draws(es + 1, e, dr);
+    // the next epoch's gathers (the last epoch issues them for nothing:
+    // every lane reads row 0)
+    const bool act1 = epoch + 1 < n_epochs && !(nx > (float)(epoch + 1));
+    sgd_gather_all<NC>(emb, act1, hh, tt, n, ep_neg, nn, epoch + 1, dr, L1);
+    if (act1)
+      advance_schedule(ep, ep_neg, L1.n_neg, nx, nn);
+    neg_draws(es + 2, e, dr);
     float gh[NC], gt[NC];
 #pragma unroll
     for (int d = 0; d < NC; ++d)
       gh[d] = gt[d] = 0.0f;
-    if (act)
-      sgd_update<NC>(emb, hh, e, n, ep, nx, nn, ep_neg, alpha, gamma, a, b,
-                     es, rounding, L, gh, gt, c_pos, c_neg);
-    agg_add<NC>(emb, ag, hh, tt, gh, gt);
+    if (act0)
+      sgd_apply<NC>(emb, hh, e, n, ep_neg, alpha, gamma, a, b, es, rounding,
+                    L0, gh, gt, c_pos, c_neg);
     alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
-    // boundary
-    if (lane == 0)
-      __hip_atomic_store(slots + w, (unsigned)epoch + 1u, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+    // boundary: epoch + 1 starts once every slot shows epoch - lag + 1 (the
+    // first check is outside the polling loop so that its wait is only for
+    // the slot reads)
     if (epoch >= lag && epoch + 1 < n_epochs) {
       const unsigned target = (unsigned)(epoch - lag + 1);
-      unsigned spins = 0;
-      for (;;) {
-        unsigned m = sv[0];
-#pragma unroll
-        for (int i = 1; i < 8; ++i)
-          m = min(m, sv[i]);
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
-          m = min(m, (unsigned)__shfl_xor((int)m, off, 64));
-        if (m >= target)
-          break;
-        // bounded: a timed-out wait anywhere in the launch ends the others
-        // within 1024 polls
-        if ((++spins & 1023u) == 0 &&
-            (spins > (1u << 20) ||
-             __builtin_amdgcn_readfirstlane(__hip_atomic_load(
-                 timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))) {
-          if (lane == 0)
-            __hip_atomic_store(timeout, 1u, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          const unsigned idx = (unsigned)lane + 64u * i;
-          sv[i] = idx < nw ? __hip_atomic_load(slots + idx, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT)
-                           : ~0u;
-        }
+      if (slot_min(sv, nw, lane) < target) {
+        unsigned spins = 0;
+        do {
+          // bounded: a timed-out wait anywhere in the launch ends the
+          // others within 1024 polls
+          if ((++spins & 1023u) == 0 &&
+              (spins > (1u << 20) ||
+               __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                   timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))) {
+            if (lane == 0)
+              __hip_atomic_store(timeout, 1u, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          poll_slots(slots, nw, lane, sv);
+        } while (slot_min(sv, nw, lane) < target);
       }
     }
+    poll_slots(slots, nw, lane, sv);
+    agg_add_buf<NC>(er, ag, hh, tt, gh, gt);
+    // arrival (every lane stores the same value: no branch)
+    __hip_atomic_store(slots + w, (unsigned)epoch + 1u, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    L0 = L1;
+    act0 = act1;
   }
   flush_sgd_stats(stats, c_pos, c_neg);
 }

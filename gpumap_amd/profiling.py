@@ -1,0 +1,65 @@
+"""Kernel summary of a rocprofv3 database (``--kernel-trace`` run, the
+``*_results.db`` file): calls, total, average, minimum and maximum duration
+per kernel, as a markdown table.
+
+    python -m gpumap_amd.profiling gpurun_out/prof/run_results.db [--top 30]
+        [--after NAME:N]
+
+``--after NAME:N`` keeps the kernels that start after the N-th launch (in
+time order) of the kernels whose name contains NAME has ended: the timed
+part of a run that first warms up.
+
+rocprofv3's own ``--stats`` CSV holds the same figures; this reads the
+database, which is complete even when the profiler ends abnormally after the
+program has finished.
+"""
+
+from __future__ import annotations
+
+import argparse
+import sqlite3
+
+
+def kernel_table(db_path: str, top: int = 30, after: str | None = None) -> str:
+    con = sqlite3.connect(db_path)
+    try:
+        t0 = 0
+        if after:
+            name, n = after.rsplit(":", 1)
+            ends = con.execute(
+                "SELECT end FROM kernels WHERE instr(name, ?) > 0 "
+                "ORDER BY start", (name,)).fetchall()
+            if len(ends) < int(n):
+                raise ValueError(f"fewer than {n} launches of {name!r}")
+            t0 = ends[int(n) - 1][0]
+        rows = con.execute(
+            "SELECT name, COUNT(*), SUM(end - start), AVG(end - start), "
+            "MIN(end - start), MAX(end - start) FROM kernels WHERE start > ? "
+            "GROUP BY name ORDER BY SUM(end - start) DESC", (t0,)).fetchall()
+    finally:
+        con.close()
+    total = sum(r[2] for r in rows) or 1
+    out = [f"Total kernel time {total / 1e6:.1f} ms.", "",
+           "| kernel | calls | total ms | avg us | min us | max us | % |",
+           "|---|---|---|---|---|---|---|"]
+    for name, calls, tot, avg, mn, mx in rows[:top]:
+        short = name.split("(")[0]
+        if len(short) > 64:
+            short = short[:61] + "..."
+        out.append(f"| `{short}` | {calls} | {tot / 1e6:.2f} | {avg / 1e3:.2f} "
+                   f"| {mn / 1e3:.2f} | {mx / 1e3:.2f} | "
+                   f"{100.0 * tot / total:.1f} |")
+    return "\n".join(out)
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("db")
+    ap.add_argument("--top", type=int, default=30)
+    ap.add_argument("--after", default=None, metavar="NAME:N")
+    a = ap.parse_args()
+    print(kernel_table(a.db, a.top, a.after))
+
+
+if __name__ == "__main__":
+    main()
