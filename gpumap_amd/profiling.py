@@ -7,7 +7,8 @@ per kernel, as a markdown table.
 
 ``--after NAME:N`` keeps the kernels that start after the N-th launch (in
 time order) of the kernels whose name contains NAME has ended: the timed
-part of a run that first warms up.
+part of a run that first warms up.  ``--pmc`` summarises a ``--pmc`` run
+instead: the mean of each counter per dispatch, per kernel.
 
 rocprofv3's own ``--stats`` CSV holds the same figures; this reads the
 database, which is complete even when the profiler ends abnormally after the
@@ -52,13 +53,43 @@ def kernel_table(db_path: str, top: int = 30, after: str | None = None) -> str:
     return "\n".join(out)
 
 
+def pmc_table(db_path: str, top: int = 30) -> str:
+    con = sqlite3.connect(db_path)
+    try:
+        rows = con.execute(
+            "SELECT kernel_name, counter_name, COUNT(*), AVG(value), "
+            "SUM(value) FROM counters_collection "
+            "GROUP BY kernel_name, counter_name").fetchall()
+    finally:
+        con.close()
+    per: dict[str, dict[str, tuple[int, float, float]]] = {}
+    for k, c, n, avg, tot in rows:
+        per.setdefault(k, {})[c] = (n, avg, tot)
+    counters = sorted({c for v in per.values() for c in v})
+    order = sorted(per, key=lambda k: -max(t for _, _, t in per[k].values()))
+    out = ["| kernel | dispatches | " + " | ".join(counters) + " |",
+           "|---|---|" + "---|" * len(counters)]
+    for k in order[:top]:
+        short = k.split("(")[0]
+        if len(short) > 64:
+            short = short[:61] + "..."
+        n = max(v[0] for v in per[k].values())
+        cells = [f"{per[k][c][1]:.4g}" if c in per[k] else "" for c in counters]
+        out.append(f"| `{short}` | {n} | " + " | ".join(cells) + " |")
+    return "\n".join(out)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("db")
     ap.add_argument("--top", type=int, default=30)
     ap.add_argument("--after", default=None, metavar="NAME:N")
+    ap.add_argument("--pmc", action="store_true")
     a = ap.parse_args()
-    print(kernel_table(a.db, a.top, a.after))
+    if a.pmc:
+        print(pmc_table(a.db, a.top))
+    else:
+        print(kernel_table(a.db, a.top, a.after))
 
 
 if __name__ == "__main__":
