@@ -25,12 +25,13 @@ RESULT_FIELDS = ["n_insertion_ops", "n_update_ops", "n_embedding_ops",
 
 
 def _lockstep(x, chunk, k=15, metric=0, n_ops=None, embed=0, max_cycles=80,
-              check_every=1, mode=0):
+              check_every=1, mode=0, c=2):
     n, d = x.shape
     n_ops = n_ops or chunk
-    e = capi.Engine(n, d, n_neighbors=k, metric=metric, knn_mode=mode)
-    o = oracle.Oracle(n, d, n_neighbors=k, metric=metric, n_threads=8,
-                      knn_mode=min(mode, 1))
+    e = capi.Engine(n, d, n_neighbors=k, n_components=c, metric=metric,
+                    knn_mode=mode)
+    o = oracle.Oracle(n, d, n_neighbors=k, n_components=c, metric=metric,
+                      n_threads=8, knn_mode=min(mode, 1))
     pos, cyc = 0, 0
     while True:
         ins = min(chunk, n - pos)
@@ -106,6 +107,33 @@ def test_exact_mode_is_exact_knn(mode):
     idx, _ = e.knn()
     ei, ed = ev.exact_knn(x, 15)
     assert ev.recall_at_k(idx, x, ei, ed) == 1.0
+
+
+def test_vmf_cosine_k30_shape_bf16():
+    # the 200,000 x 300 cosine config's shape at 4,000 rows: von Mises-Fisher
+    # clusters with Zipf sizes, k = 30, BF16-screened insertion; the graph
+    # equals the oracle's every cycle and the exact cosine kNN
+    x = ev.vmf50(n=4000, d=300)
+    e, o = _lockstep(x, chunk=500, k=30, metric=1, mode=2, check_every=2)
+    idx, _ = e.knn()
+    ei, ed = ev.exact_knn(x, 30, metric="cosine")
+    assert ev.recall_at_k(idx, x, ei, ed, metric="cosine") == 1.0
+
+
+def test_three_components_stream_shape():
+    # the 2,000,000 x 128 streaming config's shape at 20,000 rows: 100
+    # clusters, 2,000-row chunks, three components; the graph equals the
+    # oracle's, and the embedding after the op plan's embedding ops agrees
+    # with the oracle's in trustworthiness
+    x = ev.stream100(n=20_000, d=128)
+    e, o = _lockstep(x, chunk=2000, mode=2, c=3, embed=2000, check_every=4)
+    for eng in (e, o):
+        eng.run(0, 0, len(x))
+    ge, oe = e.embedding(), o.embedding()
+    assert ge.shape == (len(x), 3) and np.isfinite(ge).all()
+    tg = ev.trustworthiness(x, ge, sample=5000)
+    to = ev.trustworthiness(x, oe, sample=5000)
+    assert abs(tg - to) <= 0.01, (tg, to)
 
 
 def _fp64_union(idx, dist, sig, rho, batch, k):
