@@ -35,7 +35,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 BF16_PEAK = 2.5e15         # MI355X dense BF16 MFMA, flop/s
-CHUNK = 500
+CHUNK = 500  # the workload's insertion chunk (set from WORKLOAD_CFG)
 METRIC = ("points embedded/s @recall@15≥0.95, BF16-MFMA vs FP32; "
           "p50/p99 run() latency")
 MODES = {"nndescent": 0, "exact_fp32": 1, "exact_bf16": 2}
@@ -46,7 +46,21 @@ WORKLOAD_DESC = {
                   "2-D, 200 epochs",
     "gaussian10": "10,000x64 float32 mixture of 10 isotropic Gaussians; "
                   "euclidean, k=15, 2-D, 200 epochs",
+    "vmf50": "200,000x300 float32 unit vectors, 50 von Mises-Fisher clusters "
+             "(kappa 50, Zipf(1.2) sizes); cosine, k=30, 2-D, 200 epochs",
+    "stream100": "2,000,000x128 float32 100-cluster Gaussian mixture (sigma "
+                 "U[0.5,2], centres U[-20,20]); euclidean, k=15, 3-D, 200 "
+                 "epochs, 10,000-row chunks",
 }
+# per workload: n_neighbors, n_components, metric (0 euclidean, 1 cosine) and
+# the insertion chunk of the op plan
+WORKLOAD_CFG = {
+    "image_like": (15, 2, 0, 500),
+    "gaussian10": (15, 2, 0, 500),
+    "vmf50": (30, 2, 1, 500),
+    "stream100": (15, 3, 0, 10_000),
+}
+CFG = WORKLOAD_CFG["image_like"]
 
 
 def log(*a):
@@ -54,7 +68,9 @@ def log(*a):
 
 
 def one_session(capi, xdev, n, d, profile=False, knn_mode=2):
-    e = capi.Engine(n, d, knn_mode=knn_mode)
+    k, c, metric, _ = CFG
+    e = capi.Engine(n, d, n_neighbors=k, n_components=c, metric=metric,
+                    knn_mode=knn_mode)
     if profile:
         e.set_profiling(True)
     base = xdev.data_ptr()
@@ -192,9 +208,11 @@ def served(xdev, n, d, sessions, window=0.001):
 
     srv = Server(batch_window=window)
     ss = []
+    k, c, metric, _ = CFG
     for _ in range(sessions):
         p = gm.GPUMAPParams()
         p.knn_mode = "exact_bf16"
+        p.n_neighbors, p.n_components, p.metric = k, c, metric
         ss.append(srv.open(ArrayDataLoader(xdev), p))
     lats = [[] for _ in ss]
     th = [threading.Thread(target=client, args=(s, l)) for s, l in zip(ss, lats)]
@@ -234,7 +252,9 @@ def served_entry(ev, x, xdev, n, d, sessions, steps, fp32_value, no_quality):
              "vs_fp32_plain": value / fp32_value if fp32_value else None}
     if not no_quality:
         emb = keep.get_embedding().cpu().numpy()
-        entry["trustworthiness_at_15_session0"] = ev.trustworthiness(x, emb)
+        entry["trustworthiness_at_15_session0"] = ev.trustworthiness(
+            x, emb, sample=5000, metric="cosine" if CFG[2] == 1 else
+            "euclidean")
     keep.close()
     srv.close()
     log(f"[served x{sessions}] {value:.0f} points/s"
@@ -338,12 +358,15 @@ def serving_mix(ev, n_sessions=64, rate=4.0, budget=0.5, seed=1003,
 
 
 def quality(ev, x, e, n):
+    k, _, metric, _ = CFG
+    mname = "cosine" if metric == 1 else "euclidean"
     idx, _ = e.knn()
     q = np.random.Generator(np.random.PCG64(7)).choice(n, 2000, replace=False)
-    ei, ed = ev.exact_knn(x, 15, queries=q, device="cuda:0")
-    return {"recall_at_15": ev.recall_at_k(idx, x, ei, ed, queries=q),
-            "trustworthiness_at_15": ev.trustworthiness(x, e.embedding(),
-                                                         sample=5000)}
+    ei, ed = ev.exact_knn(x, k, queries=q, metric=mname, device="cuda:0")
+    return {f"recall_at_{k}": ev.recall_at_k(idx, x, ei, ed, queries=q,
+                                             metric=mname),
+            "trustworthiness_at_15": ev.trustworthiness(
+                x, e.embedding(), sample=5000, metric=mname)}
 
 
 def cpu_baseline(x, budget_rows, knn_mode=1):
@@ -352,7 +375,9 @@ def cpu_baseline(x, budget_rows, knn_mode=1):
     xs = np.ascontiguousarray(x[:budget_rows])
     n, d = xs.shape
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    o = oracle.Oracle(n, d, n_threads=threads, knn_mode=min(knn_mode, 1))
+    k, c, metric, _ = CFG
+    o = oracle.Oracle(n, d, n_neighbors=k, n_components=c, metric=metric,
+                      n_threads=threads, knn_mode=min(knn_mode, 1))
     t_run, pos = 0.0, 0
     while True:
         ins = min(CHUNK, n - pos)
@@ -422,6 +447,9 @@ def main():
     from gpumap_amd import capi
     from gpumap_amd import evaluation as ev
 
+    global CFG, CHUNK
+    CFG = WORKLOAD_CFG[args.workload]
+    CHUNK = CFG[3]
     x = ev.WORKLOADS[args.workload]()
     n, d = x.shape
     xdev = torch.from_numpy(x).to("cuda:0")
@@ -433,7 +461,7 @@ def main():
                                  args.warmup)
     ms_per_step = 1e3 * float(np.mean(totals))
     value = n / (ms_per_step / 1e3)
-    kern = kernel_numbers(last, capi, d, 2)
+    kern = kernel_numbers(last, capi, d, CFG[1])
     qual = {} if args.no_quality else quality(ev, x, last, n)
     if qual:
         log(f"[{args.knn_mode}] quality {qual}")
@@ -471,7 +499,7 @@ def main():
         entry = {"value": n / (ms2 / 1e3), "ms_per_step": ms2,
                  "latency_p50_ms": 1e3 * float(np.percentile(lat2, 50)),
                  "latency_p99_ms": 1e3 * float(np.percentile(lat2, 99)),
-                 "kernels": kernel_numbers(e2, capi, d, 2)}
+                 "kernels": kernel_numbers(e2, capi, d, CFG[1])}
         if not args.no_quality:
             entry["quality"] = quality(ev, x, e2, n)
         e2.close()
@@ -522,9 +550,10 @@ def main():
                                  "bound and re-ranked in exact FP32; sigma/rho "
                                  "FP64; SGD FP32")
                    if mode == MODES["exact_bf16"] else "FP32",
-                   "op_plan": "run(500,500,500) per 500-row chunk, then "
-                              "run(0,N,N) until update_completed, then one "
-                              "full embedding sweep"},
+                   "op_plan": f"run({CHUNK},{CHUNK},{CHUNK}) per "
+                              f"{CHUNK}-row chunk, then run(0,N,N) until "
+                              "update_completed, then one full embedding "
+                              "sweep"},
         "latency_p50_ms": 1e3 * float(np.percentile(lats, 50)),
         "latency_p99_ms": 1e3 * float(np.percentile(lats, 99)),
         "quality": qual,

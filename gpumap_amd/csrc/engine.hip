@@ -155,7 +155,11 @@ struct Engine {
   float *lbm = nullptr;
   uint64_t lbm_max_bytes = 8ull << 30;
   uint64_t xb_cap = 0;
-  static constexpr unsigned RCAP = 2048; // survivors per batch row
+  // survivors per batch row: 2048, or up to 16384 for sessions above 2^18
+  // rows, where a new row can lie within many earlier rows' pool maxima (in
+  // concentrated clusters nearly every row of its cluster); bounded by the
+  // select kernel's LDS (8 bytes per key)
+  unsigned rcap = 2048;
   uint64_t ins_pairs = 0; // (batch row, column) pairs screened or computed
   uint32_t *bcount = nullptr;
   uint32_t *scal = nullptr;
@@ -273,6 +277,15 @@ struct Engine {
       HIPC(hipMemsetAsync(Xs, 0, rows_ * Db * 2 * sizeof(__bf16), s));
       nrm = dalloc<float>(rows_);
       HIPC(hipMemsetAsync(nrm, 0, rows_ * sizeof(float), s));
+      rcap = 2048;
+      if (N > (1u << 18)) {
+        int smem = 0;
+        HIPC(hipDeviceGetAttribute(
+            &smem, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+        rcap = 16384;
+        while (rcap > 2048 && (uint64_t)rcap * 8 > (uint64_t)smem)
+          rcap /= 2;
+      }
       qctl = dalloc<unsigned>(4);
       HIPC(hipMemsetAsync(qctl, 0, 4 * sizeof(unsigned), s));
     }
@@ -723,7 +736,7 @@ struct Engine {
       LAUNCH_CHECK();
       k_bf16_screen<1><<<blocks, 256, 0, s>>>(
           Xs, Xb, nsteps, nrm, pmax, tau, b0, B, n, bsplit, (int)brt, part_f,
-          rlist, rcnt, RCAP, qctl, coef, cfac, lbm);
+          rlist, rcnt, rcap, qctl, coef, cfac, lbm);
       LAUNCH_CHECK();
       k_tau_select<<<(unsigned)B, 256, 0, s>>>(part_f, bsplit, tau, B);
       LAUNCH_CHECK();
@@ -735,19 +748,20 @@ struct Engine {
         const uint64_t cpw = std::max<uint64_t>(8, (n + ranges - 1) / ranges);
         const uint64_t waves = groups * ((n + cpw - 1) / cpw);
         k_bf16_lbscan<<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(
-            lbm, B, n, b0, tau, pmax, rlist, rcnt, RCAP, qctl, cpw);
+            lbm, B, n, b0, tau, pmax, rlist, rcnt, rcap, qctl, cpw);
       } else {
         k_bf16_screen<2><<<blocks, 256, 0, s>>>(
             Xs, Xb, nsteps, nrm, pmax, tau, b0, B, n, bsplit, (int)brt,
-            part_f, rlist, rcnt, RCAP, qctl, coef, cfac, nullptr);
+            part_f, rlist, rcnt, rcap, qctl, coef, cfac, nullptr);
       }
       LAUNCH_CHECK();
-      k_bf16_rerank<<<dim3((unsigned)B, RCAP / 64), 64, 0, s>>>(
-          X, Dp, rlist, rcnt, RCAP, qctl, fkeys, pool, pmax, flag, b0, tau, B,
+      k_bf16_rerank<<<dim3((unsigned)B, std::min(rcap, 2048u) / 64), 64, 0,
+                      s>>>(
+          X, Dp, rlist, rcnt, rcap, qctl, fkeys, pool, pmax, flag, b0, tau, B,
           prof ? kstats : nullptr);
       LAUNCH_CHECK();
-      k_bf16_select<<<(unsigned)B, 64, RCAP * sizeof(uint64_t), s>>>(
-          fkeys, rcnt + B, RCAP, qctl, pool, b0, prof ? kstats : nullptr);
+      k_bf16_select<<<(unsigned)B, 64, rcap * sizeof(uint64_t), s>>>(
+          fkeys, rcnt + B, rcap, qctl, pool, b0, prof ? kstats : nullptr);
       LAUNCH_CHECK();
       // the FP32 tiles below run only if the survivor queue overflowed
       guard = qctl;
@@ -780,9 +794,9 @@ struct Engine {
     Xb = dalloc<__bf16>(rows_ * Db * 2);
     HIPC(hipMemsetAsync(Xb, 0, rows_ * Db * 2 * sizeof(__bf16), s));
     tau = dalloc<float>(B);
-    rlist = dalloc<uint32_t>(B * (uint64_t)RCAP);
+    rlist = dalloc<uint32_t>(B * (uint64_t)rcap);
     rcnt = dalloc<unsigned>(2 * B); // survivors, then forward keys in tau
-    fkeys = dalloc<uint64_t>(B * (uint64_t)RCAP);
+    fkeys = dalloc<uint64_t>(B * (uint64_t)rcap);
     if (N * B * sizeof(float) <= lbm_max_bytes) {
       // optional: without it the survivors come from a second tile pass
       if (hipMalloc((void **)&lbm, N * B * sizeof(float)) != hipSuccess) {

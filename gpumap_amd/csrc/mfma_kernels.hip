@@ -443,8 +443,9 @@ __global__ __launch_bounds__(256) void k_tau_select(
 
 // Canonical distances of the listed pairs and the reverse offers.  Block
 // (r, g) is one wave and takes entries [64 g, 64 g + 64) of batch row
-// u = b0 + r's list (blocks past the list end return at once), so the
-// work spreads evenly however uneven the lists are.  The wave walks the
+// u = b0 + r's list, then every (gridDim.y)-th group after it (blocks past
+// the list end return at once), so the work spreads evenly however uneven
+// the lists are.  The wave walks the
 // features in chunks of 32: the 64 column rows' chunk is staged in LDS from
 // coalesced 128-byte row reads (the next chunk's reads are in flight
 // meanwhile), then every lane continues its own pair's FMA chain over the
@@ -463,108 +464,110 @@ __global__ __launch_bounds__(64) void k_bf16_rerank(
     unsigned long long *__restrict__ stats) {
   __shared__ float sl[64 * RS];
   const uint64_t r = blockIdx.x;
-  const unsigned b = blockIdx.y * 64u;
   const unsigned cnt = min(rcnt[r], rcap);
-  if (b >= cnt || qctl[1])
+  if (qctl[1] || blockIdx.y * 64u >= cnt)
     return;
   const int lane = threadIdx.x;
-  if (stats && b == 0 && lane == 0)
+  if (stats && blockIdx.y == 0 && lane == 0)
     atomicAdd(&stats[5], (unsigned long long)cnt);
   const uint64_t u = b0 + r;
   const float *__restrict__ xu = X + u * dp;
   const int c4 = (lane & 7) * 4; // this lane's float4 of a staged row chunk
-  const bool valid = b + lane < cnt;
-  const uint32_t ent = valid ? rlist[r * (uint64_t)rcap + b + lane] : 0u;
-  const uint64_t j = ent & LID;
-  // staged rows lane / 8 + 8 q: loads are branch-free (a row past the list
-  // reads row 0 and a float4 past dp re-reads the row's last one; both are
-  // then replaced by zeros)
-  uint32_t jr[8];
-  uint32_t live = 0;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int row = (lane >> 3) + 8 * q;
-    const uint32_t jq = (uint32_t)__shfl((uint32_t)j, row, 64);
-    const bool ok = b + row < cnt;
-    jr[q] = ok ? jq : 0u;
-    live |= ok ? 1u << q : 0u;
-  }
-  // (the zeroing is a bit mask rather than a select: a select lets the
-  // compiler move each load under a branch and wait for it there, which
-  // serialises the eight loads)
-  auto stage = [&](uint64_t f, float4 out[8]) {
-    const bool fin = f + c4 < dp;
-    const uint64_t col = fin ? f + c4 : dp - 4;
+  // groups of 64 entries, grid-strided over the block rows of the grid's y
+  for (unsigned b = blockIdx.y * 64u; b < cnt; b += gridDim.y * 64u) {
+    const bool valid = b + lane < cnt;
+    const uint32_t ent = valid ? rlist[r * (uint64_t)rcap + b + lane] : 0u;
+    const uint64_t j = ent & LID;
+    // staged rows lane / 8 + 8 q: loads are branch-free (a row past the list
+    // reads row 0 and a float4 past dp re-reads the row's last one; both are
+    // then replaced by zeros)
+    uint32_t jr[8];
+    uint32_t live = 0;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const float4 v =
-          *reinterpret_cast<const float4 *>(X + (uint64_t)jr[q] * dp + col);
-      const uint32_t m = (fin && ((live >> q) & 1u)) ? ~0u : 0u;
-      out[q] = make_float4(__uint_as_float(__float_as_uint(v.x) & m),
-                           __uint_as_float(__float_as_uint(v.y) & m),
-                           __uint_as_float(__float_as_uint(v.z) & m),
-                           __uint_as_float(__float_as_uint(v.w) & m));
+      const int row = (lane >> 3) + 8 * q;
+      const uint32_t jq = (uint32_t)__shfl((uint32_t)j, row, 64);
+      const bool ok = b + row < cnt;
+      jr[q] = ok ? jq : 0u;
+      live |= ok ? 1u << q : 0u;
     }
-  };
-  float4 nv[8];
-  stage(0, nv);
-  float acc = 0.0f;
-  for (uint64_t f0 = 0; f0 < dp; f0 += RC) {
+    // (the zeroing is a bit mask rather than a select: a select lets the
+    // compiler move each load under a branch and wait for it there, which
+    // serialises the eight loads)
+    auto stage = [&](uint64_t f, float4 out[8]) {
+      const bool fin = f + c4 < dp;
+      const uint64_t col = fin ? f + c4 : dp - 4;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      float *d = sl + ((lane >> 3) + 8 * q) * RS + c4;
-      d[0] = nv[q].x;
-      d[1] = nv[q].y;
-      d[2] = nv[q].z;
-      d[3] = nv[q].w;
+      for (int q = 0; q < 8; ++q) {
+        const float4 v =
+            *reinterpret_cast<const float4 *>(X + (uint64_t)jr[q] * dp + col);
+        const uint32_t m = (fin && ((live >> q) & 1u)) ? ~0u : 0u;
+        out[q] = make_float4(__uint_as_float(__float_as_uint(v.x) & m),
+                             __uint_as_float(__float_as_uint(v.y) & m),
+                             __uint_as_float(__float_as_uint(v.z) & m),
+                             __uint_as_float(__float_as_uint(v.w) & m));
+      }
+    };
+    float4 nv[8];
+    stage(0, nv);
+    float acc = 0.0f;
+    for (uint64_t f0 = 0; f0 < dp; f0 += RC) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        float *d = sl + ((lane >> 3) + 8 * q) * RS + c4;
+        d[0] = nv[q].x;
+        d[1] = nv[q].y;
+        d[2] = nv[q].z;
+        d[3] = nv[q].w;
+      }
+      __syncthreads();
+      // next chunk's rows, in flight during this chunk's chain
+      stage(f0 + RC, nv);
+      __builtin_amdgcn_sched_barrier(0);
+      // u's chunk (wave-uniform); features past dp are zero on both sides and
+      // fma(0, 0, acc) == acc
+      float xv[RC];
+      if (f0 + RC <= dp) {
+#pragma unroll
+        for (int i = 0; i < RC; ++i)
+          xv[i] = xu[f0 + i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < RC; ++i)
+          xv[i] = f0 + i < dp ? xu[f0 + i] : 0.0f;
+      }
+#pragma unroll
+      for (int i = 0; i < RC; ++i) {
+        const float t = xv[i] - sl[lane * RS + i];
+        acc = __builtin_fmaf(t, t, acc);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      // keep the prefetched values as this iteration's (without this the
+      // loads are folded into the next iteration and no longer overlap)
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        asm volatile(""
+                     : "+v"(nv[q].x), "+v"(nv[q].y), "+v"(nv[q].z),
+                       "+v"(nv[q].w));
+      __syncthreads();
     }
-    __syncthreads();
-    // next chunk's rows, in flight during this chunk's chain
-    stage(f0 + RC, nv);
-    __builtin_amdgcn_sched_barrier(0);
-    // u's chunk (wave-uniform); features past dp are zero on both sides and
-    // fma(0, 0, acc) == acc
-    float xv[RC];
-    if (f0 + RC <= dp) {
-#pragma unroll
-      for (int i = 0; i < RC; ++i)
-        xv[i] = xu[f0 + i];
-    } else {
-#pragma unroll
-      for (int i = 0; i < RC; ++i)
-        xv[i] = f0 + i < dp ? xu[f0 + i] : 0.0f;
-    }
-#pragma unroll
-    for (int i = 0; i < RC; ++i) {
-      const float t = xv[i] - sl[lane * RS + i];
-      acc = __builtin_fmaf(t, t, acc);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    // keep the prefetched values as this iteration's (without this the
-    // loads are folded into the next iteration and no longer overlap)
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-      asm volatile(""
-                   : "+v"(nv[q].x), "+v"(nv[q].y), "+v"(nv[q].z),
-                     "+v"(nv[q].w));
-    __syncthreads();
+    // forward keys within tau, appended to the row's key list (rcnt[nb + r]
+    // counts them): the row's 32 nearest are among them, since the 32 columns
+    // with the smallest upper bounds all have exact distances <= tau
+    const bool fw = valid && (ent & LFWD) && acc <= tau[r];
+    const uint64_t fm = __ballot(fw);
+    unsigned base = 0;
+    if (lane == 0 && fm)
+      base = atomicAdd(&rcnt[nb + r], (unsigned)__popcll(fm));
+    base = (unsigned)__shfl((int)base, 0, 64);
+    if (fw)
+      fkeys[r * (uint64_t)rcap + base +
+            __popcll(fm & ((1ull << lane) - 1ull))] =
+          make_key(acc, (uint32_t)j, false);
+    if (valid && (ent & LREV) && acc <= pmax[j] &&
+        pool_offer_lane(pool + j * W, make_key(acc, (uint32_t)u, false)))
+      flag[j] = 1;
   }
-  // forward keys within tau, appended to the row's key list (rcnt[nb + r]
-  // counts them): the row's 32 nearest are among them, since the 32 columns
-  // with the smallest upper bounds all have exact distances <= tau
-  const bool fw = valid && (ent & LFWD) && acc <= tau[r];
-  const uint64_t fm = __ballot(fw);
-  unsigned base = 0;
-  if (lane == 0 && fm)
-    base = atomicAdd(&rcnt[nb + r], (unsigned)__popcll(fm));
-  base = (unsigned)__shfl((int)base, 0, 64);
-  if (fw)
-    fkeys[r * (uint64_t)rcap + base +
-          __popcll(fm & ((1ull << lane) - 1ull))] =
-        make_key(acc, (uint32_t)j, false);
-  if (valid && (ent & LREV) && acc <= pmax[j] &&
-      pool_offer_lane(pool + j * W, make_key(acc, (uint32_t)u, false)))
-    flag[j] = 1;
 }
 
 // Batch row u = b0 + r's pool: the 32 smallest of its forward keys (those

@@ -25,7 +25,8 @@ from .gpumap import GPUMAP
 from ._native import _core
 
 _POOL_W = 32          # pool slots per row
-_SURVIVOR_CAP = 2048  # BF16 screening survivors per batch row
+_SURVIVOR_CAP = 2048  # BF16 screening survivors per batch row (x8 above
+                      # 2^18 rows)
 _LB_STORE_MAX = 8 << 30
 
 
@@ -49,7 +50,8 @@ def session_footprint(n_instances: int, n_features: int, n_neighbors: int = 15,
         db = (d + 15) // 16 * 16
         total += (n + 127) // 128 * 128 * (4 * db + 4)   # split rows, norms
         total += (b + 127) // 128 * 128 * 4 * db          # batch operands
-        total += b * _SURVIVOR_CAP * 12 + b * 128 * _POOL_W * 8
+        cap = _SURVIVOR_CAP if n <= 1 << 18 else 8 * _SURVIVOR_CAP
+        total += b * cap * 12 + b * 128 * _POOL_W * 8
         if n * b * 4 <= _LB_STORE_MAX:
             total += n * b * 4                            # lower-bound store
     return int(total)
