@@ -1190,6 +1190,112 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
   flush_sgd_stats(stats, c_pos, c_neg);
 }
 
+// sgd_waves_body for the batched kernel (several sessions' edge sets in one
+// launch, thousands of waves): there the kernel is bound by the memory
+// pipeline's throughput rather than by one wave's latency, and the
+// latency-hiding form above costs more than it hides (every lane's ten
+// gathers and the dropped row updates are issued whether or not the lane
+// samples), so each epoch issues only the memory operations it needs:
+// the slot reads and the sampling lanes' gathers at the start of the epoch,
+// the row updates (global atomics, seg_add / seg2_add) at its end.
+template <int NC>
+__device__ __attribute__((always_inline)) inline void sgd_waves_body_batch(
+    float *__restrict__ emb, const int32_t *__restrict__ head,
+    const int32_t *__restrict__ tail, uint64_t nnz, uint32_t n,
+    const float *__restrict__ eps, int n_epochs, float alpha0, float gamma,
+    float a, float b, float nsr_inv, uint64_t seed, float rounding,
+    unsigned *__restrict__ slots, unsigned nw, unsigned w,
+    unsigned *__restrict__ timeout, unsigned long long *__restrict__ stats,
+    int lag) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t e = (uint64_t)w * 64 + lane;
+  int32_t hh, tt;
+  float ep, nx, nn;
+  if (e < nnz && head[e] >= 0) {
+    hh = head[e];
+    tt = tail[e];
+    ep = eps[e];
+    nx = ep;
+    nn = ep * nsr_inv;
+  } else {
+    hh = tt = -1 - lane;
+    ep = -1.0f;
+    nx = INFINITY;
+    nn = 0.0f;
+  }
+  const Agg ag = agg_of(hh, tt);
+  uint32_t dr[8];
+  neg_draws(seed, e, dr);
+  unsigned long long c_pos = 0, c_neg = 0;
+  float alpha = alpha0;
+  for (int epoch = 0; epoch < n_epochs; ++epoch) {
+    unsigned sv[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const unsigned idx = (unsigned)lane + 64u * i;
+      sv[i] = idx < nw ? __hip_atomic_load(slots + idx, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT)
+                       : ~0u;
+    }
+    const uint64_t es = seed + (uint64_t)epoch;
+    const bool act = !(nx > (float)epoch);
+    const float ep_neg = ep * nsr_inv;
+    EdgeLoads<NC> L;
+    if (act)
+      sgd_gather<NC>(emb, hh, tt, n, ep_neg, nn, epoch, dr, L);
+    neg_draws(es + 1, e, dr);
+    float gh[NC], gt[NC];
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      gh[d] = gt[d] = 0.0f;
+    if (act)
+      sgd_update<NC>(emb, hh, e, n, ep, nx, nn, ep_neg, alpha, gamma, a, b,
+                     es, rounding, L, gh, gt, c_pos, c_neg);
+    agg_add<NC>(emb, ag, hh, tt, gh, gt);
+    alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
+    // boundary
+    if (lane == 0)
+      __hip_atomic_store(slots + w, (unsigned)epoch + 1u, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    if (epoch >= lag && epoch + 1 < n_epochs) {
+      const unsigned target = (unsigned)(epoch - lag + 1);
+      unsigned spins = 0;
+      for (;;) {
+        unsigned m = sv[0];
+#pragma unroll
+        for (int i = 1; i < 8; ++i)
+          m = min(m, sv[i]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+          m = min(m, (unsigned)__shfl_xor((int)m, off, 64));
+        if (m >= target)
+          break;
+        // bounded: a timed-out wait anywhere in the launch ends the others
+        // within 1024 polls
+        if ((++spins & 1023u) == 0 &&
+            (spins > (1u << 20) ||
+             __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                 timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))) {
+          if (lane == 0)
+            __hip_atomic_store(timeout, 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const unsigned idx = (unsigned)lane + 64u * i;
+          sv[i] = idx < nw ? __hip_atomic_load(slots + idx, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)
+                           : ~0u;
+        }
+      }
+    }
+  }
+  flush_sgd_stats(stats, c_pos, c_neg);
+}
+
+
 template <int NC>
 __global__ __launch_bounds__(64) void k_sgd_waves(
     float *__restrict__ emb, const int32_t *__restrict__ head,
@@ -1217,7 +1323,7 @@ __global__ __launch_bounds__(64) void k_sgd_waves_multi(SgdBatch batch,
   const SgdJob &J = batch.job[j];
   if (b >= J.wave0 + J.nwaves)
     return; // not part of any job (never launched so)
-  sgd_waves_body<NC>(J.emb, J.head, J.tail, *J.nnz_p, J.n, J.eps, J.n_epochs,
+  sgd_waves_body_batch<NC>(J.emb, J.head, J.tail, *J.nnz_p, J.n, J.eps, J.n_epochs,
                      J.alpha0, J.gamma, J.a, J.b, J.nsr_inv, J.seed,
                      *J.rounding_p, bar + 4 + J.wave0, J.nwaves,
                      b - J.wave0, bar + 2, J.stats, lag);
