@@ -28,6 +28,7 @@
 #include <stdexcept>
 #include <string>
 #include <unordered_set>
+#include <functional>
 #include <mutex>
 #include <vector>
 
@@ -1035,7 +1036,10 @@ struct Engine {
   // batched launch (job->nwaves = 0 when the SGD ran here after all).  The
   // reposition follows in embed_end, once the SGD is done.
   uint64_t pend_nb = 0;
-  uint32_t embed_begin(uint32_t n_ops, SgdJob *job) {
+  // `before_own_sgd` (may be empty) runs right before an SGD launched on the
+  // session's own stream, the one place it is known that one will be
+  uint32_t embed_begin(uint32_t n_ops, SgdJob *job,
+                       const std::function<void()> &before_own_sgd = {}) {
     if (job)
       job->nwaves = 0;
     pend_nb = 0;
@@ -1105,6 +1109,8 @@ struct Engine {
                     (unsigned)waves};
       return (uint32_t)nb;
     }
+    if (before_own_sgd)
+      before_own_sgd();
     run_sgd(emb, n, cap, (int)p.max_epoch, (int)p.negative_sample_rate, p.a,
             p.b, p.alpha, 1.0f, 42ull, false, true);
     if (fused_used)
@@ -1397,9 +1403,25 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
     if (!emb[i])
       continue;
     te[i] = clk::now();
-    if (last_done)
-      HIPC(hipStreamWaitEvent(E.s, last_done, 0));
-    rs[i].n_embedding_ops = E.embed_begin(ops[3 * i + 2], &jobs[i]);
+    // An SGD on the session's own stream is a grid-wide kernel that needs
+    // every CU: it starts once everything the cycle has enqueued so far, on
+    // every session's stream and the batch stream, has finished.  Other work
+    // still running would hold CUs while its resident blocks wait at the
+    // epoch boundaries for the rest (and a batched SGD could wait for them
+    // in turn).
+    auto quiesce = [&]() {
+      if (last_done)
+        HIPC(hipStreamWaitEvent(E.s, last_done, 0));
+      for (int j = 0; j < cnt; ++j) {
+        if (j == i)
+          continue;
+        hipEvent_t ev = es[j]->ev_get();
+        HIPC(hipEventRecord(ev, es[j]->s));
+        HIPC(hipStreamWaitEvent(E.s, ev, 0));
+        dones.v.emplace_back(es[j], ev);
+      }
+    };
+    rs[i].n_embedding_ops = E.embed_begin(ops[3 * i + 2], &jobs[i], quiesce);
     if (!jobs[i].nwaves) {
       HIPC(hipStreamSynchronize(E.s));
       continue;
