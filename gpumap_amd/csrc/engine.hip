@@ -30,6 +30,8 @@
 #include <unordered_set>
 #include <functional>
 #include <mutex>
+#include <shared_mutex>
+#include <condition_variable>
 #include <vector>
 
 namespace gm {
@@ -119,6 +121,44 @@ enum Scalar {
 // that no two such launches (of any sessions in this process) are in flight
 // on the device at once: each relies on all its blocks being resident.
 std::mutex g_coop_mu;
+
+// The sessions' device work in this process: an insertion or update phase of
+// run() holds it shared (the phase ends with its stream synchronised); an
+// embedding phase of run() (a grid-wide SGD kernel) and a batched cycle hold
+// it exclusively, so that a grid-wide kernel never shares the device with
+// other sessions' kernels, whose CUs its resident blocks would wait for.
+// Writers are preferred: a waiting exclusive holder blocks new shared ones.
+class DevLock {
+  std::mutex m_;
+  std::condition_variable cv_;
+  int readers_ = 0, waiting_ = 0;
+  bool writer_ = false;
+
+public:
+  void lock_shared() {
+    std::unique_lock<std::mutex> l(m_);
+    cv_.wait(l, [&] { return !writer_ && waiting_ == 0; });
+    ++readers_;
+  }
+  void unlock_shared() {
+    std::lock_guard<std::mutex> l(m_);
+    if (--readers_ == 0)
+      cv_.notify_all();
+  }
+  void lock() {
+    std::unique_lock<std::mutex> l(m_);
+    ++waiting_;
+    cv_.wait(l, [&] { return !writer_ && readers_ == 0; });
+    --waiting_;
+    writer_ = true;
+  }
+  void unlock() {
+    std::lock_guard<std::mutex> l(m_);
+    writer_ = false;
+    cv_.notify_all();
+  }
+};
+DevLock g_dev_mu;
 
 struct Engine {
   gm_params p;
@@ -1140,6 +1180,7 @@ struct Engine {
     using clk = std::chrono::steady_clock;
     std::memset(r, 0, sizeof(*r));
     if (!insertion_completed) {
+      std::shared_lock<DevLock> dl(g_dev_mu);
       auto t0 = clk::now();
       r->n_insertion_ops = insert(ni);
       HIPC(hipStreamSynchronize(s));
@@ -1148,6 +1189,7 @@ struct Engine {
       insertion_completed = insertion_queue.size() == 0 && n == N;
     }
     if (!(nu == 0 || update_completed)) {
+      std::shared_lock<DevLock> dl(g_dev_mu);
       uint64_t ch = 0;
       auto t0 = clk::now();
       r->n_update_ops = update(nu, ch);
@@ -1157,6 +1199,7 @@ struct Engine {
     }
     r->embedding_pending_edges = n;
     if (embedding_due(ne)) {
+      std::unique_lock<DevLock> dl(g_dev_mu);
       std::lock_guard<std::mutex> lk(g_coop_mu);
       auto t0 = clk::now();
       r->n_embedding_ops = embed(ne);
@@ -1264,6 +1307,7 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
     return std::chrono::duration<float>(clk::now() - t0).count();
   };
   std::lock_guard<std::mutex> lk(g_batch_mu);
+  std::unique_lock<DevLock> dl(g_dev_mu);
   BatchCtx &bc = batch_ctx(es[0]->device);
   for (int i = 0; i < cnt; ++i)
     std::memset(&rs[i], 0, sizeof(gm_result));
@@ -1807,6 +1851,7 @@ int gm_debug_fuzzy(gm_handle *h, const uint32_t *batch, uint64_t nb,
         throw Fail(GM_ERR_INVALID, "duplicate batch row");
       seen[batch[i]] = true;
     }
+    std::shared_lock<gm::DevLock> dl(gm::g_dev_mu);
     HIPC(hipMemcpyAsync(e->dbatch, batch, nb * 4, hipMemcpyHostToDevice, e->s));
     e->build_fuzzy(nb);
     if (nb < e->n) // no reposition follows to clear the batch marks
@@ -1892,6 +1937,7 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
                         e->s));
     HIPC(hipMemcpyAsync(e->scal + gm::S_MAXW, &sc[1], 4, hipMemcpyHostToDevice,
                         e->s));
+    std::unique_lock<gm::DevLock> dl(gm::g_dev_mu);
     std::lock_guard<std::mutex> lk(gm::g_coop_mu);
     if (mw > 0.0f && nnz > 0)
       e->run_sgd(e->emb, n, (uint64_t)nnz, n_epochs, nsr, a, b, alpha0, gamma,

@@ -122,6 +122,43 @@ def test_own_stream_sgd_after_a_batched_launch_in_the_same_cycle():
         assert np.isfinite(e.embedding()).all()
 
 
+def test_sessions_run_from_threads_equal_each_session_alone():
+    # run() on several sessions from several threads at once (no Server):
+    # the phases interleave on the device under the process-wide device
+    # lock and every session's results equal its run alone
+    import threading
+
+    specs = [(ev.gaussian10(n=2000, d=24, index=20 + i), 2, 15)
+             for i in range(3)]
+    alone = _run_alone(specs, 400)
+    sessions = _sessions(specs)
+    logs = [None] * len(sessions)
+
+    def drive(i):
+        e, x = sessions[i]
+        p, log = Plan(e, x, 400), []
+        while not p.done:
+            ops = p.next_ops()
+            r = e.run(*ops)
+            log.append([getattr(r, f) for f in RESULT_FIELDS])
+            p.record(ops, r)
+        logs[i] = log
+
+    th = [threading.Thread(target=drive, args=(i,))
+          for i in range(len(sessions))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for (ea, la), (em, _), lm in zip(alone, sessions, logs):
+        assert la == lm
+        ia, da = ea.knn()
+        im, dm = em.knn()
+        assert np.array_equal(ia, im)
+        assert np.array_equal(da.view(np.uint32), dm.view(np.uint32))
+        assert np.isfinite(em.embedding()).all()
+
+
 def test_batch_rejects_duplicate_sessions():
     x = ev.gaussian10(n=600, d=8)
     e = capi.Engine(*x.shape)
