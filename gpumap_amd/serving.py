@@ -238,8 +238,17 @@ class Server:
                 self._queued.append((s, (target_latency, n_ops), fut))
             else:
                 self._pending[s] = ((target_latency, n_ops), None, fut)
-            self._cv.notify_all()
+                # wake the loop for the cycle's first request (it then waits
+                # for the batch window) and when the batch is complete, not
+                # for every request: each wake-up contends for the GIL with
+                # the clients still answering
+                if len(self._pending) == 1 or self._complete():
+                    self._cv.notify_all()
         return fut
+
+    def _complete(self) -> bool:
+        # every session of the previous cycle has its next request pending
+        return all(q in self._pending or q._closed for q in self._last)
 
     def _loop(self) -> None:
         while True:
@@ -249,11 +258,8 @@ class Server:
                     return
                 if self.batch_window > 0:
                     # let the other sessions' next requests join this cycle
-                    self._cv.wait_for(
-                        lambda: self._stop or all(
-                            s in self._pending or s._closed
-                            for s in self._last),
-                        self.batch_window)
+                    self._cv.wait_for(lambda: self._stop or self._complete(),
+                                      self.batch_window)
                 batch = list(self._pending.items())
                 self._pending.clear()
                 self._last = [s for s, _ in batch]
