@@ -1593,7 +1593,10 @@ template __global__ void k_sgd_epoch<4>(float *, const int32_t *,
 // ----------------------------------------------------------- reposition
 // Unselected rows move to the membership-weighted mean of their embed-time
 // neighbours, read from a snapshot (reference heuristic_neighbor_avg_kernel
-// :811-894), any n_components <= 4.
+// :811-894), any n_components <= 4.  A thread reads its k - 1 neighbour
+// entries four at a time, each group's entries and snapshot rows as
+// independent loads (entries past k - 1 read entry 0 and are skipped), and
+// sums them in neighbour order.
 __global__ void k_reposition(const uint32_t *__restrict__ pidx,
                              const float *__restrict__ pdist, uint64_t n,
                              int k, int euclid, const double *__restrict__ sigma,
@@ -1610,19 +1613,40 @@ __global__ void k_reposition(const uint32_t *__restrict__ pidx,
     return;
   }
   const double sig_i = sigma_of(sigma, i, *mean_p);
+  const double rho_i = rho[i];
   double wsum = 0.0, acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int q = 0; q < k; ++q) {
-    uint32_t j;
-    double d;
-    embed_entry(pidx, pdist, i, q, k, euclid != 0, &j, &d);
-    if (j == i || j >= n || !valid_d(d))
-      continue;
-    const double wq = membership(d, rho[i], sig_i);
-    wsum += wq;
+  for (int q0 = 0; q0 < k - 1; q0 += 4) {
+    uint32_t js[4];
+    float vs[4], sv[4][4];
 #pragma unroll
-    for (int dd = 0; dd < 4; ++dd)
-      if (dd < c)
-        acc[dd] += wq * snap[(uint64_t)j * c + dd];
+    for (int t = 0; t < 4; ++t) {
+      const uint64_t at = i * k + (q0 + t < k - 1 ? q0 + t : 0);
+      js[t] = pidx[at];
+      vs[t] = pdist[at];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const uint64_t jr = js[t] < n ? js[t] : 0;
+#pragma unroll
+      for (int dd = 0; dd < 4; ++dd)
+        sv[t][dd] = dd < c ? snap[jr * c + dd] : 0.0f;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const uint32_t j = js[t];
+      const float v = vs[t];
+      const double d = !valid_d((double)v) ? (double)v
+                       : euclid ? (double)sqrt_rn(fmaxf(v, 0.0f))
+                                : (double)v;
+      if (q0 + t < k - 1 && j != i && j < n && valid_d(d)) {
+        const double wq = membership(d, rho_i, sig_i);
+        wsum += wq;
+#pragma unroll
+        for (int dd = 0; dd < 4; ++dd)
+          if (dd < c)
+            acc[dd] += wq * sv[t][dd];
+      }
+    }
   }
   if (wsum > 0.0) {
 #pragma unroll

@@ -798,30 +798,65 @@ __global__ __launch_bounds__(64) void k_init_embedding(float *__restrict__ emb, 
     return;
   const uint64_t u = b0 + bi;
   Rng rng(42, (uint32_t)u, 0, TAG_EMB_INIT);
-  float e[4];
-  for (int d = 0; d < c; ++d)
-    e[d] = rng.normal();
+  float e[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int d = 0; d < 4; ++d)
+    if (d < c)
+      e[d] = rng.normal();
+  // the row's k entries, read four at a time as independent loads (entries
+  // past k read entry 0 and are skipped); pre-batch neighbours only
   float dmin = 3.402823466e38f;
-  for (int j = 0; j < k; ++j)
-    if (pidx[u * k + j] < b0)
-      dmin = fminf(dmin, pdist[u * k + j]);
+  for (int j0 = 0; j0 < k; j0 += 4) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const uint64_t at = u * k + (j0 + t < k ? j0 + t : 0);
+      const uint32_t id = pidx[at];
+      const float dv = pdist[at];
+      if (j0 + t < k && id < b0)
+        dmin = fminf(dmin, dv);
+    }
+  }
   if (dmin != 3.402823466e38f) {
     float wsum = 0.0f, acc[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int j = 0; j < k; ++j) {
-      const uint32_t id = pidx[u * k + j];
-      if (id >= b0)
-        continue;
-      const float wj = expf(-(pdist[u * k + j] - dmin));
-      wsum += wj;
-      for (int d = 0; d < c; ++d)
-        acc[d] += wj * emb[(uint64_t)id * c + d];
+    for (int j0 = 0; j0 < k; j0 += 4) {
+      uint32_t ids[4];
+      float dv[4], ev[4][4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const uint64_t at = u * k + (j0 + t < k ? j0 + t : 0);
+        ids[t] = pidx[at];
+        dv[t] = pdist[at];
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const uint64_t r = ids[t] < b0 ? ids[t] : 0;
+#pragma unroll
+        for (int d = 0; d < 4; ++d)
+          ev[t][d] = d < c ? emb[r * c + d] : 0.0f;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        if (j0 + t >= k || ids[t] >= b0)
+          continue;
+        const float wj = expf(-(dv[t] - dmin));
+        wsum += wj;
+#pragma unroll
+        for (int d = 0; d < 4; ++d)
+          if (d < c)
+            acc[d] += wj * ev[t][d];
+      }
     }
-    if (wsum > 0.0f)
-      for (int d = 0; d < c; ++d)
-        e[d] = acc[d] / wsum;
+    if (wsum > 0.0f) {
+#pragma unroll
+      for (int d = 0; d < 4; ++d)
+        if (d < c)
+          e[d] = acc[d] / wsum;
+    }
   }
-  for (int d = 0; d < c; ++d)
-    emb[u * c + d] = e[d];
+#pragma unroll
+  for (int d = 0; d < 4; ++d)
+    if (d < c)
+      emb[u * c + d] = e[d];
 }
 
 } // namespace gm
