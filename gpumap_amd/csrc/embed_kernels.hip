@@ -170,16 +170,26 @@ __global__ void k_membership(const uint32_t *__restrict__ pidx,
       const double p = membership(d, rho[i], sigma_of(sigma, i, mean_dist));
       double q = 0.0;
       bool found = false;
-      for (int cj = 0; cj < k; ++cj) {
+      // i's first entry in j's list (j's self entry aside), its ids read
+      // four at a time as independent loads
+      int hit = -1;
+      for (int e0 = 0; e0 < k - 1 && hit < 0; e0 += 4) {
+        uint32_t ids[4];
+#pragma unroll
+        for (int t4 = 0; t4 < 4; ++t4)
+          ids[t4] = pidx[(uint64_t)j * k + (e0 + t4 < k - 1 ? e0 + t4 : 0)];
+#pragma unroll
+        for (int t4 = 3; t4 >= 0; --t4)
+          if (e0 + t4 < k - 1 && ids[t4] == (uint32_t)i)
+            hit = e0 + t4;
+      }
+      if (hit >= 0) {
         uint32_t jj;
         double dj;
-        embed_entry(pidx, pdist, j, cj, k, euclid != 0, &jj, &dj);
-        if (jj == i) {
-          if (valid_d(dj)) {
-            q = membership(dj, rho[j], sigma_of(sigma, j, mean_dist));
-            found = true;
-          }
-          break;
+        embed_entry(pidx, pdist, j, hit + 1, k, euclid != 0, &jj, &dj);
+        if (valid_d(dj)) {
+          q = membership(dj, rho[j], sigma_of(sigma, j, mean_dist));
+          found = true;
         }
       }
       wf = (float)(p + q - p * q);
