@@ -178,13 +178,17 @@ def measure(capi, xdev, n, d, mode, steps, warmup):
     return last, totals, lats
 
 
-def served(xdev, n, d, sessions, window=0.001):
-    """`sessions` sessions of the workload served together by the Server:
-    one client thread per session runs the headline's op plan through
-    run_with_n_ops, and the Server batches the pending requests into cycles.
+def served(xdev, n, d, sessions, window=0.001, clients="loop"):
+    """`sessions` sessions of the workload served together by the Server,
+    each running the headline's op plan through requests of
+    run_with_n_ops; the Server batches the pending requests into cycles.
+    clients="loop": one front-end thread submits every session's next
+    request as soon as its previous result is back (the event-loop form of a
+    serving front end); "threads": one blocking client thread per session.
     Returns the wall time from the first request to the last result, every
     request's latency, the cycle count and one session's model."""
     import threading
+    from concurrent.futures import FIRST_COMPLETED, wait
 
     import gpumap_amd as gm
     from gpumap_amd.data_loader import ArrayDataLoader
@@ -215,13 +219,44 @@ def served(xdev, n, d, sessions, window=0.001):
         p.n_neighbors, p.n_components, p.metric = k, c, metric
         ss.append(srv.open(ArrayDataLoader(xdev), p))
     lats = [[] for _ in ss]
-    th = [threading.Thread(target=client, args=(s, l)) for s, l in zip(ss, lats)]
     c0 = srv.cycles
     t0 = time.perf_counter()
-    for t in th:
-        t.start()
-    for t in th:
-        t.join()
+    if clients == "threads":
+        th = [threading.Thread(target=client, args=(s, l))
+              for s, l in zip(ss, lats)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+    else:
+        pos = [0] * len(ss)
+        swept = [False] * len(ss)
+        inflight = {}
+
+        def nxt(i):
+            ins = min(CHUNK, n - pos[i])
+            plan = (ins, CHUNK, CHUNK) if ins else (0, n, n)
+            pos[i] += ins
+            f = ss[i].submit(n_ops=plan)
+            inflight[f] = (i, plan, time.perf_counter())
+
+        for i in range(len(ss)):
+            nxt(i)
+        while inflight:
+            done, _ = wait(list(inflight), return_when=FIRST_COMPLETED)
+            for f in done:
+                i, plan, ts = inflight.pop(f)
+                r = f.result()
+                lats[i].append(time.perf_counter() - ts)
+                if pos[i] == n and r.update_completed:
+                    if plan[2] >= n or swept[i]:
+                        continue
+                    swept[i] = True
+                    ts2 = time.perf_counter()
+                    f2 = ss[i].submit(n_ops=(0, 0, n))
+                    inflight[f2] = (i, (0, 0, n), ts2)
+                    continue
+                nxt(i)
     wall = time.perf_counter() - t0
     cycles = srv.cycles - c0
     keep = ss[0]
@@ -232,6 +267,11 @@ def served(xdev, n, d, sessions, window=0.001):
 
 def served_entry(ev, x, xdev, n, d, sessions, steps, fp32_value, no_quality):
     srv, keep, wall, lat, cycles = served(xdev, n, d, sessions)  # warmup
+    keep.close()
+    srv.close()
+    # the same sessions with one blocking client thread each
+    srv, keep, t_wall, t_lat, _ = served(xdev, n, d, sessions,
+                                         clients="threads")
     keep.close()
     srv.close()
     walls, lats = [], []
@@ -249,7 +289,15 @@ def served_entry(ev, x, xdev, n, d, sessions, steps, fp32_value, no_quality):
              "request_latency_p50_ms": 1e3 * float(np.percentile(lat, 50)),
              "request_latency_p99_ms": 1e3 * float(np.percentile(lat, 99)),
              "cycles": cycles, "batch_window_ms": 1.0,
-             "vs_fp32_plain": value / fp32_value if fp32_value else None}
+             "clients": "one front-end thread submitting each session's next "
+                        "request when its result is back",
+             "vs_fp32_plain": value / fp32_value if fp32_value else None,
+             "threads_clients": {
+                 "value": sessions * n / t_wall,
+                 "request_latency_p50_ms": 1e3 * float(np.percentile(t_lat, 50)),
+                 "request_latency_p99_ms": 1e3 * float(np.percentile(t_lat, 99)),
+                 "vs_fp32_plain": (sessions * n / t_wall / fp32_value
+                                   if fp32_value else None)}}
     if not no_quality:
         emb = keep.get_embedding().cpu().numpy()
         entry["trustworthiness_at_15_session0"] = ev.trustworthiness(
