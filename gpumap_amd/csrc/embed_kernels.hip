@@ -851,7 +851,8 @@ __device__ inline void agg_add_buf(__amdgpu_buffer_rsrc_t r, const Agg &a,
     v1[d] = a.paired ? tot : gt[d];
     v2[d] = a.paired ? 0.0f : gh[d];
   }
-  const Seg s1 = a.paired ? a.sp : a.st;
+  // field by field: a select of whole structs goes through scratch memory
+  const Seg s1{a.paired ? a.sp.m : a.st.m, a.paired ? a.sp.start : a.st.start};
   const int32_t k1 = a.paired ? a.kp : tt;
   const bool w1 = seg_sum<NC, 0>(s1, v1);
   const bool w2 = seg_sum<NC, 0>(a.sh, v2);
@@ -1339,9 +1340,41 @@ __global__ __launch_bounds__(64) void k_sgd_waves_multi(SgdBatch batch,
                      b - J.wave0, bar + 2, J.stats, lag);
 }
 
+// A slot's schedule as the streaming kernel reads it: all five fields in one
+// round trip (an index past the edge set reads the last slot and is marked
+// empty, head -1).
+struct SlotSched {
+  int32_t h, t;
+  float nx, nn, ep;
+};
+
+__device__ inline SlotSched slot_load(const int32_t *__restrict__ head,
+                                      const int32_t *__restrict__ tail,
+                                      const float *__restrict__ nxt,
+                                      const float *__restrict__ nneg,
+                                      const float *__restrict__ eps,
+                                      uint64_t e, uint64_t nnz) {
+  const uint64_t q = e < nnz ? e : nnz - 1;
+  SlotSched r;
+  const int32_t h = head[q];
+  r.t = tail[q];
+  r.nx = nxt[q];
+  r.nn = nneg[q];
+  r.ep = eps[q];
+  r.h = e < nnz ? h : -1;
+  return r;
+}
+
 // All epochs in one cooperative launch for edge sets larger than the
 // register-resident grid: the schedule stays in memory (eps, nxt, nneg as in
-// k_sgd_epoch) and each thread walks a grid-strided range of edges per epoch.
+// k_sgd_epoch) and each wave walks a grid-strided range of slots per epoch.
+// The walk is pipelined: the next slot's schedule is loaded before this
+// slot's work, and everything issued after it -- the sample's ten gathers
+// (sgd_gather_all), the schedule's write-back and the row updates (buffer
+// operations whose inactive lanes pass an offset past the buffer) -- is a
+// fixed set of operations, so that waiting for a slot's schedule waits for
+// nothing issued later.  Each slot then costs one memory round trip (its
+// gathers) instead of four dependent ones.
 template <int NC>
 __global__ __launch_bounds__(1024) void k_sgd_stream(
     float *__restrict__ emb, const int32_t *__restrict__ head,
@@ -1354,18 +1387,62 @@ __global__ __launch_bounds__(1024) void k_sgd_stream(
   const uint64_t nnz = *nnz_p;
   const float rounding = *rounding_p;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  // whole waves iterate together (the row updates are wave operations); the
+  // wave's first slot in a scalar register, so that the per-slot buffers
+  // below are uniform
+  const uint64_t w0 = (uint64_t)blockIdx.x * blockDim.x +
+                      (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x - lane);
+  const __amdgpu_buffer_rsrc_t er = emb_rsrc<NC>(emb, n);
   unsigned long long c_pos = 0, c_neg = 0;
   unsigned seen = 0;
   float alpha = alpha0;
   for (int epoch = 0; epoch < n_epochs; ++epoch) {
     const unsigned peeked = bar_peek(bar);
     const uint64_t es = seed + (uint64_t)epoch;
-    // whole waves iterate together (seg_add is a wave operation)
-    for (uint64_t base = t0 - (threadIdx.x & 63); base < nnz; base += stride)
-      sgd_step<NC>(emb, head, tail, base + (threadIdx.x & 63), nnz, n, eps,
-                   nxt, nneg, alpha, epoch, gamma, a, b, nsr_inv, es,
-                   rounding, c_pos, c_neg);
+    if (w0 < nnz) {
+      SlotSched cur = slot_load(head, tail, nxt, nneg, eps, w0 + lane, nnz);
+      // the first schedule complete before the loop: otherwise the loop
+      // head, reached with these loads in flight, would wait for every
+      // operation still in flight on each iteration
+      __builtin_amdgcn_s_waitcnt(0);
+      for (uint64_t base = w0; base < nnz; base += stride) {
+        const uint64_t e = base + lane;
+        const SlotSched nxs =
+            slot_load(head, tail, nxt, nneg, eps, e + stride, nnz);
+        const bool valid = cur.h >= 0; // an empty slot of the COO has head -1
+        const int32_t hv = valid ? cur.h : -1 - lane;
+        const int32_t tv = valid ? cur.t : -1 - lane;
+        const bool act = valid && !(cur.nx > (float)epoch);
+        const float ep_neg = cur.ep * nsr_inv;
+        uint32_t dr[8];
+        neg_draws(es, e, dr);
+        EdgeLoads<NC> L;
+        sgd_gather_all<NC>(emb, act, hv, tv, n, ep_neg, cur.nn, epoch, dr, L);
+        float gh[NC], gt[NC];
+#pragma unroll
+        for (int d = 0; d < NC; ++d)
+          gh[d] = gt[d] = 0.0f;
+        float nx = cur.nx, nn = cur.nn;
+        if (act) {
+          sgd_apply<NC>(emb, hv, e, n, ep_neg, alpha, gamma, a, b, es,
+                        rounding, L, gh, gt, c_pos, c_neg);
+          advance_schedule(cur.ep, ep_neg, L.n_neg, nx, nn);
+        }
+        // the sampled slots' schedules back, the others' stores dropped
+        const uint32_t so = act ? (uint32_t)lane * 4u : BUF_SKIP;
+        __builtin_amdgcn_raw_buffer_store_b32(
+            __float_as_uint(nx),
+            __builtin_amdgcn_make_buffer_rsrc(nxt + base, 0, 256, 0x00020000),
+            (int)so, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(
+            __float_as_uint(nn),
+            __builtin_amdgcn_make_buffer_rsrc(nneg + base, 0, 256, 0x00020000),
+            (int)so, 0, 0);
+        agg_add_buf<NC>(er, agg_of(hv, tv), hv, tv, gh, gt);
+        cur = nxs;
+      }
+    }
     alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
     epoch_end(bar, gridDim.x, epoch, lag, peeked, seen);
   }
@@ -1472,7 +1549,13 @@ static hipError_t try_fused_any(float *emb, const int32_t *head,
                                 uint64_t seed, const float *rounding_p,
                                 unsigned *bar, unsigned long long *stats,
                                 int lag, uint64_t nnz, int num_cu,
-                                hipStream_t s, int *variant) {
+                                hipStream_t s, int *variant,
+                                bool force_stream) {
+  *variant = 1;
+  if (force_stream)
+    return try_stream<NC>(emb, head, tail, nnz_p, n, eps, nxt, nneg, n_epochs,
+                          alpha0, gamma, a, b, nsr_inv, seed, rounding_p, bar,
+                          stats, lag, nnz, num_cu, s);
   *variant = 2;
   {
     // small edge sets: per-wave arrival slots, twice the counter kernels' lag
@@ -1540,24 +1623,28 @@ hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
                             uint64_t seed, const float *rounding_p,
                             unsigned *bar, unsigned long long *stats, int lag,
                             uint64_t nnz, int num_cu, hipStream_t s,
-                            int *variant) {
+                            int *variant, bool force_stream) {
   switch (nc) {
   case 1:
     return try_fused_any<1>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
                             n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
-                            rounding_p, bar, stats, lag, nnz, num_cu, s, variant);
+                            rounding_p, bar, stats, lag, nnz, num_cu, s, variant,
+                            force_stream);
   case 2:
     return try_fused_any<2>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
                             n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
-                            rounding_p, bar, stats, lag, nnz, num_cu, s, variant);
+                            rounding_p, bar, stats, lag, nnz, num_cu, s, variant,
+                            force_stream);
   case 3:
     return try_fused_any<3>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
                             n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
-                            rounding_p, bar, stats, lag, nnz, num_cu, s, variant);
+                            rounding_p, bar, stats, lag, nnz, num_cu, s, variant,
+                            force_stream);
   default:
     return try_fused_any<4>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
                             n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
-                            rounding_p, bar, stats, lag, nnz, num_cu, s, variant);
+                            rounding_p, bar, stats, lag, nnz, num_cu, s, variant,
+                            force_stream);
   }
 }
 

@@ -755,9 +755,11 @@ struct Engine {
       //   s: u;  d~ (one fma): 2.02 u (|s - 2P| <= 2.02 s);
       //   hi/lo split with round-to-nearest BF16: 3.01 2^-16 of sum |u_i j_i|
       //     (<= s / 2) in the dot, so 3.01 2^-16 s in d~;
-      //   summation: each step's 48 exact products summed on the matrix core
-      //     (at most 2u per addition over <= 48 partial sums), steps added in
-      //     round-to-nearest FP32: (96 + nsteps) u of sum |products|;
+      //   summation: each group of G = BF16_GROUP_STEPS feature steps' 48 G
+      //     exact products summed on the matrix core (at most 2u per
+      //     addition), the ceil(nsteps / G) groups and one final partial group
+      //     added in round-to-nearest FP32: (96 G + ceil(nsteps / G) + 1) u of
+      //     sum |products|;
       // so |d~ - d*| <= A s.  The canonical distance d (one FMA chain over D
       // rounded differences) is within (D + 2) u relative of d*.  coef = 1.25
       // A and cfac = 1.25 (D + 2) u; the factor 1.25 covers second-order
@@ -765,8 +767,9 @@ struct Engine {
       const double u24 = std::ldexp(1.0, -24);
       const double nsteps_d = (double)(Db / 16);
       const double gnorm = (8.0 * (double)((Db + 511) / 512) + 6.0) * u24;
+      const double G = (double)BF16_GROUP_STEPS;
       const double A = gnorm + 3.02 * u24 + 3.01 * std::ldexp(1.0, -16) +
-                       1.01 * (96.0 + nsteps_d) * u24;
+                       1.01 * (96.0 * G + std::ceil(nsteps_d / G) + 1.0) * u24;
       const float coef = (float)(1.25 * A);
       const float cfac = (float)(1.25 * (double)(D + 2) * u24);
       const int nsteps = (int)(Db / 16);
@@ -838,7 +841,9 @@ struct Engine {
     rlist = dalloc<uint32_t>(B * (uint64_t)rcap);
     rcnt = dalloc<unsigned>(2 * B); // survivors, then forward keys in tau
     fkeys = dalloc<uint64_t>(B * (uint64_t)rcap);
-    if (N * B * sizeof(float) <= lbm_max_bytes) {
+    // the screening kernel addresses each 128-column region of the store
+    // with 32-bit offsets: B < 2^22 rows keeps it below 2^31 bytes
+    if (N * B * sizeof(float) <= lbm_max_bytes && B < (1ull << 22)) {
       // optional: without it the survivors come from a second tile pass
       if (hipMalloc((void **)&lbm, N * B * sizeof(float)) != hipSuccess) {
         (void)hipGetLastError();
@@ -991,9 +996,12 @@ struct Engine {
     LAUNCH_CHECK();
   }
 
+  // sgd_mode: SGD_AUTO (the kernel that fits), SGD_PER_EPOCH (one launch per
+  // epoch) or SGD_STREAM (the streaming cooperative kernel at any size)
+  static constexpr int SGD_AUTO = 0, SGD_PER_EPOCH = 1, SGD_STREAM = 2;
   void run_sgd(float *e, uint64_t nrows, uint64_t cap, int n_epochs, int nsr,
                float a, float b, float alpha0, float gamma, uint64_t seed,
-               bool force_per_epoch = false, bool deg_zeroed = false) {
+               int sgd_mode = 0, bool deg_zeroed = false) {
     if (cap == 0 || n_epochs <= 0)
       return;
     const float nsr_inv = 1.0f / (float)nsr;
@@ -1002,11 +1010,11 @@ struct Engine {
     unsigned long long *st = prof ? kstats : nullptr;
     ev_start(GM_KSTAT_SGD);
     bool fused = false;
-    if (!force_per_epoch) {
+    if (sgd_mode != SGD_PER_EPOCH) {
       const hipError_t fe = launch_sgd_fused(
           c, e, head, tail, scal + S_NNZ, (uint32_t)nrows, eps, nxt, nneg,
           n_epochs, alpha0, gamma, a, b, nsr_inv, seed, rounding, gbar, st,
-          SGD_LAG, cap, num_cu, s, &sgd_variant);
+          SGD_LAG, cap, num_cu, s, &sgd_variant, sgd_mode == SGD_STREAM);
       if (fe == hipSuccess) {
         fused = true;
         fused_used = true;
@@ -1152,7 +1160,7 @@ struct Engine {
     if (before_own_sgd)
       before_own_sgd();
     run_sgd(emb, n, cap, (int)p.max_epoch, (int)p.negative_sample_rate, p.a,
-            p.b, p.alpha, 1.0f, 42ull, false, true);
+            p.b, p.alpha, 1.0f, 42ull, SGD_AUTO, true);
     if (fused_used)
       HIPC(hipMemcpyAsync(h_scal + 12, gbar + 2, 4, hipMemcpyDeviceToHost, s));
     return (uint32_t)nb;
@@ -1904,7 +1912,8 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
                  float gamma, uint64_t seed, int32_t per_epoch) {
   gm_handle *h = nullptr;
   int rc = guarded([&] {
-    if (n == 0 || c < 1 || c > 4 || nnz < 0 || nsr < 1)
+    if (n == 0 || c < 1 || c > 4 || nnz < 0 || nsr < 1 || per_epoch < 0 ||
+        per_epoch > 2)
       throw Fail(GM_ERR_INVALID, "invalid SGD arguments");
     for (int64_t i = 0; i < nnz; ++i) {
       if (head[i] == -1 && tail[i] == -1)
@@ -1941,7 +1950,10 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
     std::lock_guard<std::mutex> lk(gm::g_coop_mu);
     if (mw > 0.0f && nnz > 0)
       e->run_sgd(e->emb, n, (uint64_t)nnz, n_epochs, nsr, a, b, alpha0, gamma,
-                 seed, per_epoch != 0);
+                 seed,
+                 per_epoch == 1   ? gm::Engine::SGD_PER_EPOCH
+                 : per_epoch == 2 ? gm::Engine::SGD_STREAM
+                                  : gm::Engine::SGD_AUTO);
     HIPC(hipMemcpyAsync(emb, e->emb, n * c * 4, hipMemcpyDeviceToHost, e->s));
     HIPC(hipStreamSynchronize(e->s));
     e->check_barrier();

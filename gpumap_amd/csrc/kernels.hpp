@@ -66,6 +66,10 @@ __global__ void k_exact_merge(const uint64_t *part, int nsplit, uint64_t *pool,
                               const unsigned *guard);
 
 // mfma_kernels.hip
+// feature steps (16 features each) whose products k_bf16_screen sums on the
+// matrix core before one round-to-nearest FP32 add into the running dot
+// product; the screening bound's summation term depends on it
+constexpr int BF16_GROUP_STEPS = 4;
 __global__ void k_bf16_prep(const float *X, uint64_t dp, uint64_t db,
                             __bf16 *Xs, __bf16 *Xb, float *nrm, uint8_t *flag,
                             unsigned *rcnt, unsigned *qctl, uint64_t b0,
@@ -174,9 +178,9 @@ hipError_t launch_sgd_batch(int nc, const SgdBatch &batch, unsigned waves,
                             unsigned *bar, int lag, int num_cu,
                             hipStream_t s);
 // cooperative launch of k_sgd_waves (*variant 2), k_sgd_fused (0) or, for
-// edge sets larger than its resident grid, k_sgd_stream (1, counting its work
-// at stats + 8) (embed_kernels.hip); returns the launch error when none can
-// be launched
+// edge sets larger than its resident grid or with force_stream,
+// k_sgd_stream (1, counting its work at stats + 8) (embed_kernels.hip);
+// returns the launch error when none can be launched
 hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
                             const int32_t *tail, const uint32_t *nnz_p,
                             uint32_t n, const float *eps, float *nxt,
@@ -185,7 +189,7 @@ hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
                             uint64_t seed, const float *rounding_p,
                             unsigned *bar, unsigned long long *stats, int lag,
                             uint64_t nnz, int num_cu, hipStream_t s,
-                            int *variant);
+                            int *variant, bool force_stream = false);
 __global__ void k_reposition(const uint32_t *pidx, const float *pdist,
                              uint64_t n, int k, int euclid, const double *sigma,
                              const double *rho, const double *mean,

@@ -13,8 +13,9 @@
 //     accumulation);
 //   * d~ - coef (|u|^2 + |j|^2) <= d / (1 - cfac) and d / (1 + cfac) <=
 //     d~ + coef (|u|^2 + |j|^2), where d is the canonical distance: coef
-//     covers the BF16 split, the summation of each feature step's 48 products
-//     on the matrix core and the round-to-nearest adds across steps, the norm
+//     covers the BF16 split, the summation of each group of
+//     BF16_GROUP_STEPS feature steps' products on the matrix core and the
+//     round-to-nearest adds across groups, the norm
 //     and d~ roundings; cfac covers the canonical distance's own rounding
 //     (derived on the host, engine.hip);
 //   * pass 1 keeps, per lane and batch row, the TOPL smallest upper bounds
@@ -58,6 +59,8 @@ constexpr int PARTW = 4 * TOPL; // pass-1 values per (batch row, split)
 constexpr uint32_t LFWD = 0x80000000u; // survivor list entry: forward pair
 constexpr uint32_t LREV = 0x40000000u; // survivor list entry: reverse pair
 constexpr uint32_t LID = 0x3fffffffu;  // survivor list entry: column row
+// a buffer offset past every buffer's end: the access is dropped
+constexpr uint32_t BUF_OFF = 0x80000000u;
 constexpr int RC = 32;                 // features per re-rank chunk
 constexpr int RS = RC + 1;             // padded LDS row of a chunk
 
@@ -168,12 +171,14 @@ __global__ __launch_bounds__(256, 2) void k_bf16_screen(
     nu[sb] = rok[sb] ? nrm[b0 + rl] : 0.0f;
     tr[sb] = (PASS == 2 && rok[sb]) ? tau[rl] : 0.0f;
   }
-  float top[2][TOPL];
+  // pass 1: the TOPL smallest upper bounds per batch row, ascending, as
+  // bit patterns (the bounds are >= 0, so their bits order as their values)
+  uint32_t top[2][TOPL];
 #pragma unroll
   for (int sb = 0; sb < 2; ++sb)
 #pragma unroll
     for (int q = 0; q < TOPL; ++q)
-      top[sb][q] = INFINITY;
+      top[sb][q] = 0x7f800000u; // +inf
 
   const uint64_t stepw = 1024; // elements per (group, step): hi + lo blocks
   const __bf16 *pb0 =
@@ -193,7 +198,9 @@ __global__ __launch_bounds__(256, 2) void k_bf16_screen(
         Xs + ((c0 / 32 + wa * 2 + 0) * (uint64_t)nsteps) * stepw + lane * 8;
     const __bf16 *pa1 =
         Xs + ((c0 / 32 + wa * 2 + 1) * (uint64_t)nsteps) * stepw + lane * 8;
-    floatx16 acc[2][2];
+    // acc: the running dot products; grp: the current group's products,
+    // summed on the matrix core
+    floatx16 acc[2][2], grp[2][2];
 #pragma unroll
     for (int sa = 0; sa < 2; ++sa)
 #pragma unroll
@@ -202,55 +209,94 @@ __global__ __launch_bounds__(256, 2) void k_bf16_screen(
     bf16x8 ah0 = ld8(pa0), al0 = ld8(pa0 + 512), ah1 = ld8(pa1),
            al1 = ld8(pa1 + 512), bh0 = ld8(pb0), bl0 = ld8(pb0 + 512),
            bh1 = ld8(pb1), bl1 = ld8(pb1 + 512);
-    for (int st = 0; st < nsteps; ++st) {
-      // prefetch the next step while this one runs on the matrix cores
-      const uint64_t o = (uint64_t)(st + 1 < nsteps ? st + 1 : st) * stepw;
-      bf16x8 nah0 = ld8(pa0 + o), nal0 = ld8(pa0 + o + 512),
-             nah1 = ld8(pa1 + o), nal1 = ld8(pa1 + o + 512),
-             nbh0 = ld8(pb0 + o), nbl0 = ld8(pb0 + o + 512),
-             nbh1 = ld8(pb1 + o), nbl1 = ld8(pb1 + o + 512);
-      __builtin_amdgcn_sched_barrier(0); // issue the loads first
-      // this step's 48 products per pair in a fresh accumulator, then one
-      // round-to-nearest FP32 add per step into the running dot product
-      // (bounds the summation error by the step structure, not by D)
-      const floatx16 z = {};
-      {
-        floatx16 ta = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bh0, z, 0, 0, 0);
-        floatx16 tb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bh1, z, 0, 0, 0);
-        ta = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bl0, ta, 0, 0, 0);
-        tb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bl1, tb, 0, 0, 0);
-        ta = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0, bh0, ta, 0, 0, 0);
-        tb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0, bh1, tb, 0, 0, 0);
-        acc[0][0] += ta;
-        acc[0][1] += tb;
-      }
-      {
-        floatx16 ta = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bh0, z, 0, 0, 0);
-        floatx16 tb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bh1, z, 0, 0, 0);
-        ta = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bl0, ta, 0, 0, 0);
-        tb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bl1, tb, 0, 0, 0);
-        ta = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1, bh0, ta, 0, 0, 0);
-        tb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1, bh1, tb, 0, 0, 0);
-        acc[1][0] += ta;
-        acc[1][1] += tb;
-      }
-      __builtin_amdgcn_sched_barrier(0);
-      // keep the prefetched operands as values of this iteration (without
-      // this the loads are folded into the loop header and no longer
-      // overlap the matrix work)
-      asm volatile("" : "+v"(nah0), "+v"(nal0), "+v"(nah1), "+v"(nal1));
-      asm volatile("" : "+v"(nbh0), "+v"(nbl0), "+v"(nbh1), "+v"(nbl1));
-      ah0 = nah0;
-      al0 = nal0;
-      ah1 = nah1;
-      al1 = nal1;
-      bh0 = nbh0;
-      bl0 = nbl0;
-      bh1 = nbh1;
-      bl1 = nbl1;
+    const floatx16 z = {};
+    // One feature step: prefetch the next step's operands, then this step's
+    // 48 products per pair into the group's accumulators (C = 0 on the
+    // group's first step).  Every BF16_GROUP_STEPS steps the group is added
+    // to the running dot product in round-to-nearest FP32, so the summation
+    // error grows with the group, not with D.
+#define GM_SCREEN_STEP(FRESH)                                                  \
+  {                                                                            \
+    const uint64_t o = (uint64_t)(st + 1 < nsteps ? st + 1 : st) * stepw;      \
+    bf16x8 nah0 = ld8(pa0 + o), nal0 = ld8(pa0 + o + 512),                     \
+           nah1 = ld8(pa1 + o), nal1 = ld8(pa1 + o + 512),                     \
+           nbh0 = ld8(pb0 + o), nbl0 = ld8(pb0 + o + 512),                     \
+           nbh1 = ld8(pb1 + o), nbl1 = ld8(pb1 + o + 512);                     \
+    __builtin_amdgcn_sched_barrier(0); /* issue the loads first */             \
+    grp[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(                       \
+        ah0, bh0, FRESH ? z : grp[0][0], 0, 0, 0);                             \
+    grp[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(                       \
+        ah0, bh1, FRESH ? z : grp[0][1], 0, 0, 0);                             \
+    grp[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(                       \
+        ah1, bh0, FRESH ? z : grp[1][0], 0, 0, 0);                             \
+    grp[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(                       \
+        ah1, bh1, FRESH ? z : grp[1][1], 0, 0, 0);                             \
+    grp[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bl0, grp[0][0],   \
+                                                        0, 0, 0);              \
+    grp[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bl1, grp[0][1],   \
+                                                        0, 0, 0);              \
+    grp[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bl0, grp[1][0],   \
+                                                        0, 0, 0);              \
+    grp[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bl1, grp[1][1],   \
+                                                        0, 0, 0);              \
+    grp[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0, bh0, grp[0][0],   \
+                                                        0, 0, 0);              \
+    grp[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0, bh1, grp[0][1],   \
+                                                        0, 0, 0);              \
+    grp[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1, bh0, grp[1][0],   \
+                                                        0, 0, 0);              \
+    grp[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1, bh1, grp[1][1],   \
+                                                        0, 0, 0);              \
+    __builtin_amdgcn_sched_barrier(0);                                         \
+    /* keep the prefetched operands as values of this step (without this   */ \
+    /* the loads are folded into the loop header and no longer overlap the */ \
+    /* matrix work)                                                          */ \
+    asm volatile("" : "+v"(nah0), "+v"(nal0), "+v"(nah1), "+v"(nal1));        \
+    asm volatile("" : "+v"(nbh0), "+v"(nbl0), "+v"(nbh1), "+v"(nbl1));        \
+    ah0 = nah0;                                                                \
+    al0 = nal0;                                                                \
+    ah1 = nah1;                                                                \
+    al1 = nal1;                                                                \
+    bh0 = nbh0;                                                                \
+    bl0 = nbl0;                                                                \
+    bh1 = nbh1;                                                                \
+    bl1 = nbl1;                                                                \
+  }
+    for (int st = 0; st < nsteps;) {
+      const int gend = min(st + BF16_GROUP_STEPS, nsteps);
+      GM_SCREEN_STEP(true)
+      for (++st; st < gend; ++st)
+        GM_SCREEN_STEP(false)
+#pragma unroll
+      for (int sa = 0; sa < 2; ++sa)
+#pragma unroll
+        for (int sb = 0; sb < 2; ++sb)
+          acc[sa][sb] += grp[sa][sb];
     }
+#undef GM_SCREEN_STEP
     __syncthreads(); // s_nj / s_pm written
 
+    // pass 1 stores every pair's lower bound, column-major, for
+    // k_bf16_lbscan: through a buffer over this tile's columns, so that a
+    // lane past the batch (offset BUF_OFF) or a column past n (beyond
+    // num_records) is dropped by the hardware, without a branch per store
+    // (num_records 0 when there is no store; the host passes lbm only for
+    // batches whose 128-column region stays below 2^31 bytes)
+    const uint64_t cols = min((uint64_t)TC, n - c0);
+    const __amdgpu_buffer_rsrc_t lr = __builtin_amdgcn_make_buffer_rsrc(
+        lbm ? lbm + c0 * nb : nullptr, 0,
+        lbm ? (int)(cols * nb * sizeof(float)) : 0, 0x00020000);
+    // a column's stride, taken as a new value per tile so that the 64
+    // per-pair offsets below are formed here and not hoisted out of the tile
+    // loop (where they would not fit in registers)
+    uint32_t cstride = (uint32_t)nb * 4u;
+    asm volatile("" : "+s"(cstride));
+    uint32_t lbase[2];
+#pragma unroll
+    for (int sb = 0; sb < 2; ++sb)
+      lbase[sb] = (rok[sb] ? (uint32_t)(r0 + wb * 64 + sb * 32 + (lane & 31)) * 4u
+                           : BUF_OFF) +
+                  (uint32_t)(wa * 64 + (lane >> 5) * 4) * cstride;
 #pragma unroll
     for (int sa = 0; sa < 2; ++sa)
 #pragma unroll
@@ -267,16 +313,21 @@ __global__ __launch_bounds__(256, 2) void k_bf16_screen(
           const float dt = __builtin_fmaf(-2.0f, acc[sa][sb][i], s);
           const float bnd = __builtin_fmaf(coef, s, 1e-30f);
           if (PASS == 1) {
-            // the lower bound of every pair, column-major, for k_bf16_lbscan
-            if (lbm && rok[sb] && j < n)
-              lbm[j * nb + rl] = (dt - bnd) * (1.0f - cfac);
+            __builtin_amdgcn_raw_buffer_store_b32(
+                __float_as_uint((dt - bnd) * (1.0f - cfac)), lr,
+                (int)(lbase[sb] +
+                      (uint32_t)(sa * 32 + (i >> 2) * 8 + (i & 3)) * cstride),
+                0, 0);
             const float ub = fmaxf((dt + bnd) * (1.0f + cfac), 0.0f);
-            if (ok && ub < top[sb][TOPL - 1]) {
+            // branch-free insertion: the value is carried down the sorted
+            // list, each slot keeps the smaller, the largest drops out; a
+            // pair that may not enter carries ~0u (above every bound)
+            uint32_t x = ok ? __float_as_uint(ub) & 0x7fffffffu : 0xffffffffu;
 #pragma unroll
-              for (int q = TOPL - 1; q > 0; --q)
-                top[sb][q] = ub < top[sb][q - 1] ? top[sb][q - 1]
-                                                 : fminf(top[sb][q], ub);
-              top[sb][0] = fminf(top[sb][0], ub);
+            for (int q = 0; q < TOPL; ++q) {
+              const uint32_t lo = min(top[sb][q], x);
+              x = max(top[sb][q], x);
+              top[sb][q] = lo;
             }
           } else {
             const float lb = (dt - bnd) * (1.0f - cfac);
@@ -306,7 +357,7 @@ __global__ __launch_bounds__(256, 2) void k_bf16_screen(
                    (wa * 2 + (lane >> 5)) * TOPL;
 #pragma unroll
       for (int q = 0; q < TOPL; ++q)
-        dst[q] = top[sb][q];
+        dst[q] = __uint_as_float(top[sb][q]);
     }
   }
 }

@@ -182,9 +182,11 @@ int gm_debug_fuzzy(gm_handle *h, const uint32_t *batch, uint64_t n_batch,
                    double *rho, int64_t *nnz);
 
 /* SGD epochs of the engine on a caller-given embedding (n x c, host) and COO
- * (host), with the engine's kernels: all epochs in one cooperative launch
- * when the edges fit the resident grid, or (per_epoch != 0, or when they do
- * not fit) one launch per epoch.  A slot with head = tail = -1 is empty (the
+ * (host), with the engine's kernels: per_epoch 0 runs all epochs in one
+ * cooperative launch of the kernel that fits the edge set (one launch per
+ * epoch when none can be launched), 1 one launch per epoch, 2 the streaming
+ * cooperative kernel the engine uses for edge sets beyond the resident grid,
+ * at any size.  A slot with head = tail = -1 is empty (the
  * engine's own COO keeps each edge's reverse in the slot after it, or an
  * empty slot). */
 int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,

@@ -232,8 +232,10 @@ def _twin_paired_wave():
     return np.array(head, np.int32), np.array(tail, np.int32)
 
 
-@pytest.mark.parametrize("per_epoch", [False, True])
-def test_sgd_twin_paired_aggregation_equals_per_edge_aggregation(per_epoch):
+@pytest.mark.parametrize("per_epoch,stream",
+                         [(False, False), (True, False), (False, True)])
+def test_sgd_twin_paired_aggregation_equals_per_edge_aggregation(per_epoch,
+                                                                 stream):
     # one wave, every edge at the maximum weight with 2 epochs and nsr 1:
     # each edge is sampled once (epoch 1) with no negative samples and the
     # wave reads every row before it adds.  The contributions are multiples
@@ -251,9 +253,10 @@ def test_sgd_twin_paired_aggregation_equals_per_edge_aggregation(per_epoch):
     assert not (h2[0] == t2[1] and t2[0] == h2[1])  # not twins: unpaired
     g = np.random.default_rng(12)
     emb = (np.round(g.standard_normal((40, 2)) * 1024) / 1024).astype(np.float32)
-    paired = capi.sgd(emb, h, t, w, 2, 1, 1.577, 0.895, per_epoch=per_epoch)
+    paired = capi.sgd(emb, h, t, w, 2, 1, 1.577, 0.895, per_epoch=per_epoch,
+                      stream=stream)
     plain = capi.sgd(emb, h2, t2, np.ones(h2.size, np.float32), 2, 1, 1.577,
-                     0.895, per_epoch=per_epoch)
+                     0.895, per_epoch=per_epoch, stream=stream)
     np.testing.assert_array_equal(paired, plain)
     moved = np.unique(np.concatenate([h2, t2]))
     assert (paired[moved] != emb[moved]).any(axis=1).all()
@@ -342,6 +345,40 @@ def test_sgd_fused_equals_per_epoch_launches():
     per = capi.sgd(emb, h, t, w, 2, 1, 1.577, 0.895, per_epoch=True)
     np.testing.assert_array_equal(fused, per)
     assert not np.array_equal(fused, emb)
+
+
+def test_sgd_stream_equals_per_epoch_launches():
+    # the streaming kernel (the engine's choice for edge sets beyond the
+    # resident grid, forced here at a small size) computes the same epochs as
+    # one launch per epoch: conflict-free edges of equal weight with 2 epochs
+    # and nsr 1, so every edge takes exactly one attraction step (epoch 1)
+    # and no negative sample; more slots than the grid has threads, so that
+    # each wave walks several
+    g = np.random.default_rng(16)
+    n = 1 << 20
+    emb = g.standard_normal((n, 2)).astype(np.float32)
+    h = np.arange(0, n, 2, dtype=np.int32)
+    t = h + 1
+    w = np.ones(h.size, np.float32)
+    stream = capi.sgd(emb, h, t, w, 2, 1, 1.577, 0.895, stream=True)
+    per = capi.sgd(emb, h, t, w, 2, 1, 1.577, 0.895, per_epoch=True)
+    np.testing.assert_array_equal(stream, per)
+    assert (stream != emb).any(axis=1).mean() > 0.99
+
+
+def test_sgd_stream_statistics_match_oracle():
+    # the streaming kernel on a real fuzzy graph with negative sampling
+    x = ev.gaussian10(n=2000, d=16)
+    e, o = _lockstep(x, chunk=500, mode=1, check_every=10)
+    batch = np.arange(2000, dtype=np.uint32)
+    h, t, w, _, _ = e.fuzzy(batch)
+    g = np.random.default_rng(8)
+    emb0 = g.uniform(-10, 10, size=(2000, 2)).astype(np.float32)
+    out_g = capi.sgd(emb0, h, t, w, 200, 5, 1.577, 0.895, stream=True)
+    out_o = oracle.sgd(emb0, h, t, w, 200, 5, 1.577, 0.895)
+    tg = ev.trustworthiness(x, out_g)
+    to = ev.trustworthiness(x, out_o)
+    assert abs(tg - to) <= 0.01, (tg, to)
 
 
 def test_sgd_many_epochs_statistics_match_oracle():
