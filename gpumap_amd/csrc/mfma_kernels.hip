@@ -76,6 +76,46 @@ __device__ inline bf16x8 ld8(const __bf16 *p) {
   return *reinterpret_cast<const bf16x8 *>(p);
 }
 
+// stages of the screening kernel's operand ring (16 KB each)
+constexpr int NSTG = 4;
+// s_waitcnt for the ring: vector memory count (NSTG - 2) * 4 (a wave's loads
+// of the NSTG - 2 stages issued after the awaited one), no other wait
+constexpr int RING_WAIT = ((NSTG - 2) * 4) | (7 << 4) | (15 << 8);
+
+// A step's eight operands of one wave from the ring: four blocks from the
+// column half at a, four from the batch-row half at b (LDS byte addresses of
+// this lane's 16 bytes in the first block).  Written as one asm block that
+// waits for its own reads: the caller has already waited for the stage's
+// direct-to-LDS loads (its own, and every wave's at the barrier), which the
+// compiler cannot tell apart from the later stages' loads still in flight
+// and would otherwise wait for as well.
+__device__ inline void ring_read(uint32_t a, uint32_t b, bf16x8 &ah0,
+                                 bf16x8 &al0, bf16x8 &ah1, bf16x8 &al1,
+                                 bf16x8 &bh0, bf16x8 &bl0, bf16x8 &bh1,
+                                 bf16x8 &bl1) {
+  asm volatile("ds_read_b128 %0, %8\n\t"
+               "ds_read_b128 %1, %8 offset:1024\n\t"
+               "ds_read_b128 %2, %8 offset:2048\n\t"
+               "ds_read_b128 %3, %8 offset:3072\n\t"
+               "ds_read_b128 %4, %9\n\t"
+               "ds_read_b128 %5, %9 offset:1024\n\t"
+               "ds_read_b128 %6, %9 offset:2048\n\t"
+               "ds_read_b128 %7, %9 offset:3072\n\t"
+               "s_waitcnt lgkmcnt(0)"
+               : "=&v"(ah0), "=&v"(al0), "=&v"(ah1), "=&v"(al1), "=&v"(bh0),
+                 "=&v"(bl0), "=&v"(bh1), "=&v"(bl1)
+               : "v"(a), "v"(b)
+               : "memory");
+}
+
+// block-wide barrier for LDS data only: completes this wave's LDS accesses,
+// not its global loads (__syncthreads would also wait for the operand ring's
+// loads in flight)
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_s_waitcnt((15 | (3 << 14)) | (7 << 4)); // lgkmcnt(0)
+  __builtin_amdgcn_s_barrier();
+}
+
 } // namespace
 
 // hi/lo BF16 split of the batch rows into the column operand store Xs (global
@@ -143,6 +183,7 @@ __global__ __launch_bounds__(256, 2) void k_bf16_screen(
     unsigned *__restrict__ qctl, float coef, float cfac,
     float *__restrict__ lbm) {
   __shared__ float s_nj[TC], s_pm[TC];
+  __shared__ __attribute__((aligned(16))) __bf16 ring[NSTG][16 * 512];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int wa = wave & 1, wb = wave >> 1;
@@ -181,23 +222,71 @@ __global__ __launch_bounds__(256, 2) void k_bf16_screen(
       top[sb][q] = 0x7f800000u; // +inf
 
   const uint64_t stepw = 1024; // elements per (group, step): hi + lo blocks
-  const __bf16 *pb0 =
-      Xb + ((r0 / 32 + wb * 2 + 0) * (uint64_t)nsteps) * stepw + lane * 8;
-  const __bf16 *pb1 =
-      Xb + ((r0 / 32 + wb * 2 + 1) * (uint64_t)nsteps) * stepw + lane * 8;
+  const uint64_t gstride = (uint64_t)nsteps * stepw; // one group's steps
+  // Operand ring in LDS: NSTG stages of one feature step each, 16 blocks of
+  // 1 KB per stage (blocks 0-7: the tile's 4 column groups, hi then lo;
+  // blocks 8-15: the 4 batch-row groups).  Wave w loads blocks 4w .. 4w + 3
+  // of each stage with direct-to-LDS loads (16 bytes a lane, MFMA lane
+  // order), NSTG - 1 steps ahead of the step on the matrix cores; the steps
+  // run over this block's tiles in sequence, so the next tile's first steps
+  // load during this tile's last.  Each operand block is fetched once per
+  // workgroup (the waves that share it read it from LDS).
+  const bool aw = wave < 2; // this wave loads column blocks, else batch rows
+  const __bf16 *const bsrc = // waves 2, 3: batch-row groups 0-1, 2-3
+      Xb + ((r0 / 32 + (uint64_t)(wave & 1) * 2) * gstride) + lane * 8;
+  const uint32_t my_tiles =
+      split < ntile ? (uint32_t)((ntile - 1 - (uint64_t)split) / nsplit + 1)
+                    : 0u;
+  uint32_t pti = 0, pst = 0; // the next step to load: tile index, step
+  // issue the next step's 4 blocks of this wave into stage `stg` (past the
+  // last step a repeat of it, so that every step issues the same loads)
+#define GM_RING_ISSUE(stg)                                                      \
+  {                                                                            \
+    const bool live = pti < my_tiles;                                          \
+    const uint32_t qt = live ? pti : my_tiles - 1;                             \
+    const uint32_t qs = live ? pst : (uint32_t)nsteps - 1;                     \
+    const __bf16 *src =                                                        \
+        (aw ? Xs + (((split + (uint64_t)qt * nsplit) * TC / 32 +               \
+                     (uint64_t)wave * 2) * gstride) + lane * 8                 \
+            : bsrc) +                                                          \
+        (uint64_t)qs * stepw;                                                  \
+    __bf16 *dst = &ring[(stg)][wave * 4 * 512];                                \
+    __builtin_amdgcn_global_load_lds(                                          \
+        src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);         \
+    __builtin_amdgcn_global_load_lds(                                          \
+        src + 512, (__attribute__((address_space(3))) void *)(dst + 512), 16,  \
+        0, 0);                                                                 \
+    __builtin_amdgcn_global_load_lds(                                          \
+        src + gstride, (__attribute__((address_space(3))) void *)(dst + 1024), \
+        16, 0, 0);                                                             \
+    __builtin_amdgcn_global_load_lds(                                          \
+        src + gstride + 512,                                                   \
+        (__attribute__((address_space(3))) void *)(dst + 1536), 16, 0, 0);     \
+    if (++pst == (uint32_t)nsteps) {                                           \
+      pst = 0;                                                                 \
+      ++pti;                                                                   \
+    }                                                                          \
+  }
+  // this lane's LDS byte address in stage 0 of the ring
+  const uint32_t ring_base =
+      (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const char *)(
+          (const char *)&ring[0][0]) +
+      (uint32_t)lane * 16u;
+  uint32_t gs = 0; // the step on the matrix cores, over the block's tiles
+  if (my_tiles > 0) {
+#pragma unroll
+    for (int q = 0; q < NSTG - 1; ++q)
+      GM_RING_ISSUE(q)
+  }
 
-  for (uint64_t t = split; t < ntile; t += nsplit) {
-    const uint64_t c0 = t * TC;
-    __syncthreads(); // previous tile's readers of s_nj / s_pm are done
+  for (uint32_t ti = 0; ti < my_tiles; ++ti) {
+    const uint64_t c0 = (split + (uint64_t)ti * nsplit) * TC;
+    lds_barrier(); // previous tile's readers of s_nj / s_pm are done
     if (tid < TC) {
       const uint64_t j = c0 + tid;
       s_nj[tid] = j < n ? nrm[j] : 0.0f;
       s_pm[tid] = (PASS == 2 && j < b0) ? pmax[j] : -INFINITY;
     }
-    const __bf16 *pa0 =
-        Xs + ((c0 / 32 + wa * 2 + 0) * (uint64_t)nsteps) * stepw + lane * 8;
-    const __bf16 *pa1 =
-        Xs + ((c0 / 32 + wa * 2 + 1) * (uint64_t)nsteps) * stepw + lane * 8;
     // acc: the running dot products; grp: the current group's products,
     // summed on the matrix core
     floatx16 acc[2][2], grp[2][2];
@@ -206,23 +295,25 @@ __global__ __launch_bounds__(256, 2) void k_bf16_screen(
 #pragma unroll
       for (int sb = 0; sb < 2; ++sb)
         acc[sa][sb] = floatx16{};
-    bf16x8 ah0 = ld8(pa0), al0 = ld8(pa0 + 512), ah1 = ld8(pa1),
-           al1 = ld8(pa1 + 512), bh0 = ld8(pb0), bl0 = ld8(pb0 + 512),
-           bh1 = ld8(pb1), bl1 = ld8(pb1 + 512);
     const floatx16 z = {};
-    // One feature step: prefetch the next step's operands, then this step's
-    // 48 products per pair into the group's accumulators (C = 0 on the
-    // group's first step).  Every BF16_GROUP_STEPS steps the group is added
-    // to the running dot product in round-to-nearest FP32, so the summation
-    // error grows with the group, not with D.
+    // One feature step: wait until this step's stage has landed (this
+    // wave's loads of it, then every wave's at the barrier, which also
+    // frees the stage read by the previous step), load the step NSTG - 1
+    // ahead into that stage, then this step's 48 products per pair into the
+    // group's accumulators (C = 0 on the group's first step).  Every
+    // BF16_GROUP_STEPS steps the group is added to the running dot product
+    // in round-to-nearest FP32, so the summation error grows with the
+    // group, not with D.
 #define GM_SCREEN_STEP(FRESH)                                                  \
   {                                                                            \
-    const uint64_t o = (uint64_t)(st + 1 < nsteps ? st + 1 : st) * stepw;      \
-    bf16x8 nah0 = ld8(pa0 + o), nal0 = ld8(pa0 + o + 512),                     \
-           nah1 = ld8(pa1 + o), nal1 = ld8(pa1 + o + 512),                     \
-           nbh0 = ld8(pb0 + o), nbl0 = ld8(pb0 + o + 512),                     \
-           nbh1 = ld8(pb1 + o), nbl1 = ld8(pb1 + o + 512);                     \
-    __builtin_amdgcn_sched_barrier(0); /* issue the loads first */             \
+    __builtin_amdgcn_s_waitcnt(RING_WAIT);                                     \
+    __builtin_amdgcn_s_barrier();                                              \
+    GM_RING_ISSUE((gs + NSTG - 1) % NSTG)                                      \
+    bf16x8 ah0, al0, ah1, al1, bh0, bl0, bh1, bl1;                            \
+    ring_read(ring_base + (uint32_t)(gs % NSTG) * 16384u + (uint32_t)wa * 4096u, \
+              ring_base + (uint32_t)(gs % NSTG) * 16384u + 8192u +             \
+                  (uint32_t)wb * 4096u,                                        \
+              ah0, al0, ah1, al1, bh0, bl0, bh1, bl1);                         \
     grp[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(                       \
         ah0, bh0, FRESH ? z : grp[0][0], 0, 0, 0);                             \
     grp[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(                       \
@@ -247,20 +338,7 @@ __global__ __launch_bounds__(256, 2) void k_bf16_screen(
                                                         0, 0, 0);              \
     grp[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1, bh1, grp[1][1],   \
                                                         0, 0, 0);              \
-    __builtin_amdgcn_sched_barrier(0);                                         \
-    /* keep the prefetched operands as values of this step (without this   */ \
-    /* the loads are folded into the loop header and no longer overlap the */ \
-    /* matrix work)                                                          */ \
-    asm volatile("" : "+v"(nah0), "+v"(nal0), "+v"(nah1), "+v"(nal1));        \
-    asm volatile("" : "+v"(nbh0), "+v"(nbl0), "+v"(nbh1), "+v"(nbl1));        \
-    ah0 = nah0;                                                                \
-    al0 = nal0;                                                                \
-    ah1 = nah1;                                                                \
-    al1 = nal1;                                                                \
-    bh0 = nbh0;                                                                \
-    bl0 = nbl0;                                                                \
-    bh1 = nbh1;                                                                \
-    bl1 = nbl1;                                                                \
+    ++gs;                                                                      \
   }
     for (int st = 0; st < nsteps;) {
       const int gend = min(st + BF16_GROUP_STEPS, nsteps);
@@ -274,7 +352,7 @@ __global__ __launch_bounds__(256, 2) void k_bf16_screen(
           acc[sa][sb] += grp[sa][sb];
     }
 #undef GM_SCREEN_STEP
-    __syncthreads(); // s_nj / s_pm written
+    lds_barrier(); // s_nj / s_pm written
 
     // pass 1 stores every pair's lower bound, column-major, for
     // k_bf16_lbscan: through a buffer over this tile's columns, so that a
@@ -345,6 +423,8 @@ __global__ __launch_bounds__(256, 2) void k_bf16_screen(
         }
       }
   }
+
+#undef GM_RING_ISSUE
 
   if (PASS == 1) {
     // PARTW values per (batch row, split): 4 lanes x TOPL, each list sorted
