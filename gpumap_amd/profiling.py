@@ -7,7 +7,9 @@ per kernel, as a markdown table.
 
 ``--after NAME:N`` keeps the kernels that start after the N-th launch (in
 time order) of the kernels whose name contains NAME has ended: the timed
-part of a run that first warms up.  ``--pmc`` summarises a ``--pmc`` run
+part of a run that first warms up.  ``--until NAME:N`` keeps those that
+start before the N-th launch of NAME has ended: with ``--after`` a window
+such as one session of a run of several.  ``--pmc`` summarises a ``--pmc`` run
 instead: the mean of each counter per dispatch, per kernel.
 
 rocprofv3's own ``--stats`` CSV holds the same figures; this reads the
@@ -21,22 +23,29 @@ import argparse
 import sqlite3
 
 
-def kernel_table(db_path: str, top: int = 30, after: str | None = None) -> str:
+def _end_of(con, spec: str) -> int:
+    """End time of the N-th launch (in time order) of the kernels whose name
+    contains NAME, for spec NAME:N."""
+    name, n = spec.rsplit(":", 1)
+    ends = con.execute(
+        "SELECT end FROM kernels WHERE instr(name, ?) > 0 ORDER BY start",
+        (name,)).fetchall()
+    if len(ends) < int(n) or int(n) < 1:
+        raise ValueError(f"fewer than {n} launches of {name!r}")
+    return ends[int(n) - 1][0]
+
+
+def kernel_table(db_path: str, top: int = 30, after: str | None = None,
+                 until: str | None = None) -> str:
     con = sqlite3.connect(db_path)
     try:
-        t0 = 0
-        if after:
-            name, n = after.rsplit(":", 1)
-            ends = con.execute(
-                "SELECT end FROM kernels WHERE instr(name, ?) > 0 "
-                "ORDER BY start", (name,)).fetchall()
-            if len(ends) < int(n):
-                raise ValueError(f"fewer than {n} launches of {name!r}")
-            t0 = ends[int(n) - 1][0]
+        t0 = _end_of(con, after) if after else 0
+        t1 = _end_of(con, until) if until else (1 << 63) - 1
         rows = con.execute(
             "SELECT name, COUNT(*), SUM(end - start), AVG(end - start), "
-            "MIN(end - start), MAX(end - start) FROM kernels WHERE start > ? "
-            "GROUP BY name ORDER BY SUM(end - start) DESC", (t0,)).fetchall()
+            "MIN(end - start), MAX(end - start) FROM kernels "
+            "WHERE start > ? AND start < ? "
+            "GROUP BY name ORDER BY SUM(end - start) DESC", (t0, t1)).fetchall()
     finally:
         con.close()
     total = sum(r[2] for r in rows) or 1
@@ -84,12 +93,13 @@ def main() -> None:
     ap.add_argument("db")
     ap.add_argument("--top", type=int, default=30)
     ap.add_argument("--after", default=None, metavar="NAME:N")
+    ap.add_argument("--until", default=None, metavar="NAME:N")
     ap.add_argument("--pmc", action="store_true")
     a = ap.parse_args()
     if a.pmc:
         print(pmc_table(a.db, a.top))
     else:
-        print(kernel_table(a.db, a.top, a.after))
+        print(kernel_table(a.db, a.top, a.after, a.until))
 
 
 if __name__ == "__main__":

@@ -1,0 +1,46 @@
+"""The kernel-trace summary's launch windows, on a small synthetic trace
+database with rocprofv3's `kernels` view columns the summary reads."""
+
+import sqlite3
+
+from gpumap_amd import profiling
+
+
+def _db(tmp_path):
+    p = tmp_path / "run_results.db"
+    con = sqlite3.connect(p)
+    con.execute("CREATE TABLE kernels (name TEXT, start INTEGER, end INTEGER)")
+    t = 0
+    rows = []
+    # three sessions of (screen, sgd) launches; session s's sgd takes 10 s + 5
+    for sess in range(3):
+        for _ in range(2):
+            rows.append(("k_screen", t, t + 3))
+            t += 4
+            rows.append(("k_sgd_waves<2>", t, t + 10 * sess + 5))
+            t += 10 * sess + 6
+    con.executemany("INSERT INTO kernels VALUES (?, ?, ?)", rows)
+    con.commit()
+    con.close()
+    return str(p)
+
+
+def test_window_selects_one_session(tmp_path):
+    db = _db(tmp_path)
+    whole = profiling.kernel_table(db)
+    assert "| `k_sgd_waves<2>` | 6 |" in whole
+    # the second session: after the 2nd sgd launch, until the 4th
+    mid = profiling.kernel_table(db, after="k_sgd_waves:2", until="k_sgd_waves:4")
+    assert "| `k_sgd_waves<2>` | 2 | 0.00 | 0.01 | 0.01 | 0.01 |" in mid
+    assert "| `k_screen` | 2 |" in mid
+    last = profiling.kernel_table(db, after="k_sgd_waves:4")
+    assert "| `k_sgd_waves<2>` | 2 |" in last
+
+
+def test_window_past_the_launches_is_an_error(tmp_path):
+    db = _db(tmp_path)
+    try:
+        profiling.kernel_table(db, after="k_sgd_waves:7")
+    except ValueError:
+        return
+    raise AssertionError("expected ValueError")
