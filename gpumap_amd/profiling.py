@@ -10,7 +10,9 @@ time order) of the kernels whose name contains NAME has ended: the timed
 part of a run that first warms up.  ``--until NAME:N`` keeps those that
 start before the N-th launch of NAME has ended: with ``--after`` a window
 such as one session of a run of several.  ``--pmc`` summarises a ``--pmc`` run
-instead: the mean of each counter per dispatch, per kernel.
+instead: the mean and the maximum of each counter per dispatch, per kernel
+(the maximum picks out one large dispatch among many small ones of the same
+kernel, such as a calibration copy).
 
 rocprofv3's own ``--stats`` CSV holds the same figures; this reads the
 database, which is complete even when the profiler ends abnormally after the
@@ -67,23 +69,25 @@ def pmc_table(db_path: str, top: int = 30) -> str:
     try:
         rows = con.execute(
             "SELECT kernel_name, counter_name, COUNT(*), AVG(value), "
-            "SUM(value) FROM counters_collection "
+            "SUM(value), MAX(value) FROM counters_collection "
             "GROUP BY kernel_name, counter_name").fetchall()
     finally:
         con.close()
-    per: dict[str, dict[str, tuple[int, float, float]]] = {}
-    for k, c, n, avg, tot in rows:
-        per.setdefault(k, {})[c] = (n, avg, tot)
+    per: dict[str, dict[str, tuple[int, float, float, float]]] = {}
+    for k, c, n, avg, tot, mx in rows:
+        per.setdefault(k, {})[c] = (n, avg, tot, mx)
     counters = sorted({c for v in per.values() for c in v})
-    order = sorted(per, key=lambda k: -max(t for _, _, t in per[k].values()))
-    out = ["| kernel | dispatches | " + " | ".join(counters) + " |",
-           "|---|---|" + "---|" * len(counters)]
+    order = sorted(per, key=lambda k: -max(v[2] for v in per[k].values()))
+    heads = [f"{c} mean | {c} max" for c in counters]
+    out = ["| kernel | dispatches | " + " | ".join(heads) + " |",
+           "|---|---|" + "---|---|" * len(counters)]
     for k in order[:top]:
         short = k.split("(")[0]
         if len(short) > 64:
             short = short[:61] + "..."
         n = max(v[0] for v in per[k].values())
-        cells = [f"{per[k][c][1]:.4g}" if c in per[k] else "" for c in counters]
+        cells = [f"{per[k][c][1]:.6g} | {per[k][c][3]:.6g}" if c in per[k]
+                 else " | " for c in counters]
         out.append(f"| `{short}` | {n} | " + " | ".join(cells) + " |")
     return "\n".join(out)
 

@@ -44,3 +44,20 @@ def test_window_past_the_launches_is_an_error(tmp_path):
     except ValueError:
         return
     raise AssertionError("expected ValueError")
+
+
+def test_pmc_table_reports_mean_and_max(tmp_path):
+    # a calibration copy among small copies: its reading is the maximum
+    p = tmp_path / "pmc.db"
+    con = sqlite3.connect(p)
+    con.execute("CREATE TABLE counters_collection "
+                "(kernel_name TEXT, counter_name TEXT, value REAL)")
+    con.executemany("INSERT INTO counters_collection VALUES (?, ?, ?)",
+                    [("copy", "FETCH_SIZE", 4.0), ("copy", "FETCH_SIZE", 2.0),
+                     ("copy", "FETCH_SIZE", 262164.0), ("k", "FETCH_SIZE", 5.0)])
+    con.commit()
+    con.close()
+    t = profiling.pmc_table(str(p))
+    assert "| kernel | dispatches | FETCH_SIZE mean | FETCH_SIZE max |" in t
+    assert "| `copy` | 3 | 87390 | 262164 |" in t
+    assert "| `k` | 1 | 5 | 5 |" in t
