@@ -622,38 +622,38 @@ __global__ __launch_bounds__(64) void k_bf16_rerank(
       jr[q] = ok ? jq : 0u;
       live |= ok ? 1u << q : 0u;
     }
-    // (the zeroing is a bit mask rather than a select: a select lets the
-    // compiler move each load under a branch and wait for it there, which
-    // serialises the eight loads)
+    // (the loads are issued here and zeroed only when they are written to
+    // LDS, two chunks later: zeroing them here would wait for them at once)
     auto stage = [&](uint64_t f, float4 out[8]) {
       const bool fin = f + c4 < dp;
       const uint64_t col = fin ? f + c4 : dp - 4;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const float4 v =
+      for (int q = 0; q < 8; ++q)
+        out[q] =
             *reinterpret_cast<const float4 *>(X + (uint64_t)jr[q] * dp + col);
-        const uint32_t m = (fin && ((live >> q) & 1u)) ? ~0u : 0u;
-        out[q] = make_float4(__uint_as_float(__float_as_uint(v.x) & m),
-                             __uint_as_float(__float_as_uint(v.y) & m),
-                             __uint_as_float(__float_as_uint(v.z) & m),
-                             __uint_as_float(__float_as_uint(v.w) & m));
-      }
     };
-    float4 nv[8];
-    stage(0, nv);
     float acc = 0.0f;
-    for (uint64_t f0 = 0; f0 < dp; f0 += RC) {
+    // One chunk of RC features: its staged rows (nv, loaded two chunks
+    // earlier) into LDS, the loads of the chunk two ahead into nv, then this
+    // chunk's FMA chain.  Two register buffers alternate, so that two
+    // chunks' row loads are in flight during each chain; the chain runs over
+    // the features in order, one FMA each (the canonical distance).
+    auto chunk = [&](uint64_t f0, float4 nv[8]) {
+      // zeros for a row past the list and a float4 past dp (the loads
+      // re-read row 0 and the row's last float4; a bit mask rather than a
+      // select, which would let the compiler move the loads under a branch)
+      const bool fin = f0 + c4 < dp;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
+        const uint32_t m = (fin && ((live >> q) & 1u)) ? ~0u : 0u;
         float *d = sl + ((lane >> 3) + 8 * q) * RS + c4;
-        d[0] = nv[q].x;
-        d[1] = nv[q].y;
-        d[2] = nv[q].z;
-        d[3] = nv[q].w;
+        d[0] = __uint_as_float(__float_as_uint(nv[q].x) & m);
+        d[1] = __uint_as_float(__float_as_uint(nv[q].y) & m);
+        d[2] = __uint_as_float(__float_as_uint(nv[q].z) & m);
+        d[3] = __uint_as_float(__float_as_uint(nv[q].w) & m);
       }
       __syncthreads();
-      // next chunk's rows, in flight during this chunk's chain
-      stage(f0 + RC, nv);
+      stage(f0 + 2 * RC, nv);
       __builtin_amdgcn_sched_barrier(0);
       // u's chunk (wave-uniform); features past dp are zero on both sides and
       // fma(0, 0, acc) == acc
@@ -673,14 +673,15 @@ __global__ __launch_bounds__(64) void k_bf16_rerank(
         acc = __builtin_fmaf(t, t, acc);
       }
       __builtin_amdgcn_sched_barrier(0);
-      // keep the prefetched values as this iteration's (without this the
-      // loads are folded into the next iteration and no longer overlap)
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        asm volatile(""
-                     : "+v"(nv[q].x), "+v"(nv[q].y), "+v"(nv[q].z),
-                       "+v"(nv[q].w));
       __syncthreads();
+    };
+    float4 nva[8], nvb[8];
+    stage(0, nva);
+    stage(RC, nvb);
+    for (uint64_t f0 = 0; f0 < dp; f0 += 2 * RC) {
+      chunk(f0, nva);
+      if (f0 + RC < dp)
+        chunk(f0 + RC, nvb);
     }
     // forward keys within tau, appended to the row's key list (rcnt[nb + r]
     // counts them): the row's 32 nearest are among them, since the 32 columns
