@@ -146,3 +146,28 @@ def test_no_cpu_fallback_without_gpu():
     p.n_instances, p.n_features = 100, 8
     with pytest.raises(RuntimeError, match="device"):
         _core.GPUMAP(p)
+
+
+def test_debug_sgd_rejects_unknown_mode_before_any_device_call():
+    # gm_debug_sgd's mode: 0 the kernel that fits, 1 one launch per epoch,
+    # 2 the streaming kernel; anything else is refused by the argument check
+    # that runs before the engine touches the device
+    import ctypes
+
+    import numpy as np
+
+    from gpumap_amd import capi
+
+    emb = np.zeros((4, 2), np.float32)
+    head = np.array([0, 1], np.int32)
+    tail = np.array([1, 0], np.int32)
+    w = np.ones(2, np.float32)
+    f32, i32 = ctypes.c_float, ctypes.c_int32
+    for mode in (-1, 3):
+        rc = capi.lib().gm_debug_sgd(
+            capi._p(emb, f32), 4, 2, capi._p(head, i32), capi._p(tail, i32),
+            capi._p(w, f32), 2, 2, 1, ctypes.c_float(1.577),
+            ctypes.c_float(0.895), ctypes.c_float(1.0), ctypes.c_float(1.0),
+            ctypes.c_uint64(42), mode)
+        assert rc != 0
+        assert "invalid SGD arguments" in capi.lib().gm_last_error().decode()
