@@ -1617,6 +1617,11 @@ int gm_create(const gm_params *params, gm_handle **out) {
       throw Fail(GM_ERR_INVALID, "n_components must be in [1, 4]");
     if (q.n_instances == 0 || q.n_instances >= 0x7fffffffull)
       throw Fail(GM_ERR_INVALID, "n_instances must be in [1, 2^31-1)");
+    // the SGD kernels update the embedding through a buffer addressed with
+    // 32-bit byte offsets below 2^31
+    if (q.n_instances * q.n_components * sizeof(float) >= 0x80000000ull)
+      throw Fail(GM_ERR_INVALID,
+                 "n_instances * n_components must be below 2^29");
     if (q.n_features == 0)
       throw Fail(GM_ERR_INVALID, "n_features must be positive");
     if (q.metric != GM_EUCLIDEAN && q.metric != GM_COSINE)

@@ -106,7 +106,10 @@ typedef struct gm_handle gm_handle;
 void gm_default_params(gm_params *out);
 
 /* GPUMAP(const GPUMAPParams&) -- gpumap.cuh:73, impl.cu:5, impl.cuh:57-62.
- * Allocates every device array for n_instances rows up front. */
+ * Allocates every device array for n_instances rows up front.  Refuses
+ * (GM_ERR_INVALID) n_instances >= 2^31 - 1, n_instances * n_components >=
+ * 2^29 (the embedding is addressed with 32-bit byte offsets) and, in
+ * exact_bf16 mode, n_instances >= 2^30. */
 int gm_create(const gm_params *params, gm_handle **out);
 
 /* GPUMAP::initialize -- gpumap.cuh:83 (a no-op in the reference,

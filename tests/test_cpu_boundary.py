@@ -120,6 +120,11 @@ def test_invalid_params_rejected_before_any_device_call():
     p.n_neighbors, p.n_components = 15, 7
     with pytest.raises(ValueError):
         _core.GPUMAP(p)
+    # the embedding's byte size must stay below 2^31 (32-bit buffer offsets)
+    p.n_components = 4
+    p.n_instances = 1 << 27
+    with pytest.raises(ValueError, match="n_components"):
+        _core.GPUMAP(p)
 
 
 def test_facade_min_dist_check():
