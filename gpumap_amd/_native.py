@@ -6,6 +6,7 @@ extension raises ImportError naming the build command.
 
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 
@@ -37,5 +38,18 @@ _share_torch_runtime()
 _lib = ctypes.CDLL(LIB, mode=ctypes.RTLD_GLOBAL)
 
 from . import _core  # noqa: E402
+
+
+def _teardown() -> None:
+    """Free the sessions' device memory while the HIP runtime is still up.
+    Registered after torch is loaded, so it runs before torch's own exit
+    hooks; tensors exported by the getters that outlive it must not be read
+    afterwards.  Without it, sessions held by objects that interpreter
+    teardown collects late would be freed after the runtime (or a profiler
+    hooked into it) has finalized."""
+    _core.shutdown()
+
+
+atexit.register(_teardown)
 
 __all__ = ["_core", "LIB"]

@@ -7,6 +7,7 @@ through it.  ``Engine`` owns one session handle.
 
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 import re
@@ -104,6 +105,13 @@ def lib():
         L.gm_set_profiling.argtypes = [vp, i32]
         L.gm_get_kernel_stats.argtypes = [vp, i32, P(GmKernelStats)]
         L.gm_reset_kernel_stats.argtypes = [vp]
+        L.gm_release.argtypes = [vp]
+        L.gm_is_released.argtypes = [vp, P(i32)]
+        L.gm_shutdown.argtypes = []
+        L.gm_shutdown.restype = None
+        # sessions still alive at exit are freed while the HIP runtime is up
+        # (registered after torch's hooks, so it runs before them)
+        atexit.register(L.gm_shutdown)
         _lib = L
     return _lib
 
@@ -161,6 +169,17 @@ class Engine:
         if self.h:
             lib().gm_destroy(self.h)
             self.h = None
+
+    def release(self):
+        """gm_release: free the device memory now (no exports are made
+        through this class)."""
+        check(lib().gm_release(self.h))
+
+    @property
+    def released(self) -> bool:
+        r = i32(0)
+        check(lib().gm_is_released(self.h, ctypes.byref(r)))
+        return bool(r.value)
 
     def __del__(self):
         self.close()

@@ -199,6 +199,14 @@ struct Model {
 
   void close() { check(gm_close(s->h)); }
 
+  // frees the session's device memory once no exported tensor holds it
+  void release() { check(gm_release(s->h)); }
+  bool released() const {
+    int32_t r = 0;
+    check(gm_is_released(s->h, &r));
+    return r != 0;
+  }
+
   DeviceArray get(int32_t which) const {
     DeviceArray a{s, which, {}};
     check(gm_get_view(s->h, which, &a.v));
@@ -343,12 +351,21 @@ PYBIND11_MODULE(_core, m) {
       .def("run", &Model::run, py::arg("n_jobs_insertion"),
            py::arg("n_jobs_update"), py::arg("n_jobs_embedding"))
       .def("close", &Model::close)
+      .def("release", &Model::release,
+           "Close the session and free its device memory as soon as no "
+           "exported tensor holds it; later calls on it raise RuntimeError")
+      .def_property_readonly("released", &Model::released)
       .def("get_embedding", [](const Model &mm) { return mm.get(GM_EMBEDDING); })
       .def("get_neighbor_indices",
            [](const Model &mm) { return mm.get(GM_NEIGHBOR_INDICES); })
       .def("get_neighbor_dists",
            [](const Model &mm) { return mm.get(GM_NEIGHBOR_DISTS); });
 
+  m.def("shutdown", &gm_shutdown,
+        "Free every live session's device memory while the HIP runtime is "
+        "up (the package's exit hook); later session calls raise "
+        "RuntimeError",
+        py::call_guard<py::gil_scoped_release>());
   m.def("run_batch", &run_batch, py::arg("models"), py::arg("ops"),
         "One cycle of several sessions together: each runs its op triple "
         "as GPUMAP.run would, with the same results; the device work of "

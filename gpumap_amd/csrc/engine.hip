@@ -160,6 +160,11 @@ public:
 };
 DevLock g_dev_mu;
 
+struct Engine;
+// Every live session of the process, for gm_shutdown.
+std::mutex g_live_mu;
+std::vector<Engine *> g_live;
+
 struct Engine {
   gm_params p;
   uint64_t N, D, Dp;
@@ -168,6 +173,12 @@ struct Engine {
   int device = 0;
   hipStream_t s = nullptr;
   std::atomic<int> refs{1};
+  // gm_release: the device memory is freed as soon as no exported tensor
+  // holds the session (at the call, or when the last export is deleted);
+  // rel_mu orders that decision against the exports' deleters
+  std::mutex rel_mu;
+  bool release_pending = false;
+  std::atomic<bool> released{false};
 
   // device state
   float *X = nullptr;
@@ -290,8 +301,17 @@ struct Engine {
       release();
       throw;
     }
+    std::lock_guard<std::mutex> l(g_live_mu);
+    g_live.push_back(this);
   }
-  ~Engine() { release(); }
+  ~Engine() {
+    {
+      std::lock_guard<std::mutex> l(g_live_mu);
+      g_live.erase(std::remove(g_live.begin(), g_live.end(), this),
+                   g_live.end());
+    }
+    release();
+  }
 
   void allocate() {
     HIPC(hipStreamCreate(&s));
@@ -422,40 +442,30 @@ struct Engine {
     for (hipEvent_t ev : ev_free)
       (void)hipEventDestroy(ev);
     ev_free.clear();
-    if (kstats)
-      (void)hipFree(kstats);
-    kstats = nullptr;
-    if (gbar)
-      (void)hipFree(gbar);
-    gbar = nullptr;
-    for (void *q : {(void *)Xs, (void *)Xb, (void *)nrm, (void *)tau,
-                    (void *)rlist, (void *)rcnt, (void *)qctl, (void *)fkeys,
-                    (void *)lbm})
-      if (q)
-        (void)hipFree(q);
-    lbm = nullptr;
-    Xs = Xb = nullptr;
-    nrm = tau = nullptr;
-    rlist = nullptr;
-    rcnt = qctl = nullptr;
-    fkeys = nullptr;
-    if (pmax)
-      (void)hipFree(pmax);
-    if (part)
-      (void)hipFree(part);
-    pmax = nullptr;
-    part = nullptr;
-    void *ptrs[] = {X,     pool,    pidx,    pdist,   emb,        flag,
-                    list,  bcount,  scal,    fresh,   ins_ids,    cand,
-                    ins_cnt, ccount, cdist,  fn,      fo,         newmask,
-                    owner_mark, owners, rows, fnc,    foc,        rev_new,
-                    rev_old, sigma,  rho,    psum,    pcnt,       mean,
-                    w,     wflags,  head,    tail,    wout,       eps,
-                    nxt,   nneg,    snap,    rounding, deg,       maxdeg,
-                    dbatch, bmark};
-    for (void *q : ptrs)
-      if (q)
-        (void)(void)hipFree(q);
+    // every device pointer is freed once and cleared, so that release() may
+    // run again (from the destructor after an early gm_release)
+    void **ptrs[] = {
+        (void **)&kstats, (void **)&gbar,    (void **)&Xs,      (void **)&Xb,
+        (void **)&nrm,    (void **)&tau,     (void **)&rlist,   (void **)&rcnt,
+        (void **)&qctl,   (void **)&fkeys,   (void **)&lbm,     (void **)&pmax,
+        (void **)&part,   (void **)&X,       (void **)&pool,    (void **)&pidx,
+        (void **)&pdist,  (void **)&emb,     (void **)&flag,    (void **)&list,
+        (void **)&bcount, (void **)&scal,    (void **)&fresh,   (void **)&ins_ids,
+        (void **)&cand,   (void **)&ins_cnt, (void **)&ccount,  (void **)&cdist,
+        (void **)&fn,     (void **)&fo,      (void **)&newmask, (void **)&owner_mark,
+        (void **)&owners, (void **)&rows,    (void **)&fnc,     (void **)&foc,
+        (void **)&rev_new, (void **)&rev_old, (void **)&sigma,  (void **)&rho,
+        (void **)&psum,   (void **)&pcnt,    (void **)&mean,    (void **)&w,
+        (void **)&wflags, (void **)&head,    (void **)&tail,    (void **)&wout,
+        (void **)&eps,    (void **)&nxt,     (void **)&nneg,    (void **)&snap,
+        (void **)&rounding, (void **)&deg,   (void **)&maxdeg,  (void **)&dbatch,
+        (void **)&bmark};
+    for (void **q : ptrs) {
+      if (*q)
+        (void)hipFree(*q);
+      *q = nullptr;
+    }
+    part_cap = ins_cap = xb_cap = 0;
     if (h_stage)
       (void)hipHostFree(h_stage);
     if (h_list)
@@ -465,6 +475,7 @@ struct Engine {
     if (h_flagged)
       (void)hipHostFree(h_flagged);
     h_stage = nullptr;
+    h_stage_rows = 0;
     h_list = nullptr;
     h_scal = nullptr;
     h_flagged = nullptr;
@@ -1189,6 +1200,7 @@ struct Engine {
     std::memset(r, 0, sizeof(*r));
     if (!insertion_completed) {
       std::shared_lock<DevLock> dl(g_dev_mu);
+      check_live();
       auto t0 = clk::now();
       r->n_insertion_ops = insert(ni);
       HIPC(hipStreamSynchronize(s));
@@ -1198,6 +1210,7 @@ struct Engine {
     }
     if (!(nu == 0 || update_completed)) {
       std::shared_lock<DevLock> dl(g_dev_mu);
+      check_live();
       uint64_t ch = 0;
       auto t0 = clk::now();
       r->n_update_ops = update(nu, ch);
@@ -1209,12 +1222,20 @@ struct Engine {
     if (embedding_due(ne)) {
       std::unique_lock<DevLock> dl(g_dev_mu);
       std::lock_guard<std::mutex> lk(g_coop_mu);
+      check_live();
       auto t0 = clk::now();
       r->n_embedding_ops = embed(ne);
       HIPC(hipStreamSynchronize(s));
       r->embedding_time = std::chrono::duration<float>(clk::now() - t0).count();
     }
     finish_result(r);
+  }
+
+  // a session freed by gm_shutdown (or gm_release) between two phases of a
+  // call in flight on another thread takes no further device work
+  void check_live() const {
+    if (release_pending)
+      throw Fail(GM_ERR_STATE, "the session was released");
   }
 
   bool embedding_due(uint32_t ne) const {
@@ -1260,8 +1281,25 @@ struct Export {
   int64_t strides[2];
 };
 
+// Frees a released session's device memory once only the handle holds it.
+// The caller holds e->rel_mu.
+void release_if_unshared(Engine *e) {
+  if (e->release_pending && !e->released.load() && e->refs.load() == 1) {
+    (void)hipSetDevice(e->device);
+    e->release();
+    e->released.store(true);
+  }
+}
+
 void engine_unref(Engine *e) {
-  if (e->refs.fetch_sub(1) == 1)
+  int left;
+  {
+    std::lock_guard<std::mutex> l(e->rel_mu);
+    left = e->refs.fetch_sub(1) - 1;
+    if (left == 1)
+      release_if_unshared(e);
+  }
+  if (left == 0)
     delete e;
 }
 
@@ -1316,6 +1354,8 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
   };
   std::lock_guard<std::mutex> lk(g_batch_mu);
   std::unique_lock<DevLock> dl(g_dev_mu);
+  for (int i = 0; i < cnt; ++i)
+    es[i]->check_live();
   BatchCtx &bc = batch_ctx(es[0]->device);
   for (int i = 0; i < cnt; ++i)
     std::memset(&rs[i], 0, sizeof(gm_result));
@@ -1359,6 +1399,12 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
   // One batched SGD launch over `members` (one component count, at most
   // SGD_MAX_JOBS jobs and BATCH_MAX_WAVES waves), after their set-up; their
   // streams continue after it.  Launches run in order on the batch stream.
+  // A batched launch is a grid-wide kernel whose jobs' waves wait for each
+  // other at the epoch boundaries: it starts only once everything the cycle
+  // has enqueued so far, on every session's stream, has finished (a launch
+  // closed mid-loop would otherwise share the device with the insertions
+  // of the sessions not yet reached, whose kernels hold CUs that the
+  // launch's waiting waves need).
   auto launch_group = [&](const std::vector<int> &members) {
     if (members.empty())
       return;
@@ -1373,11 +1419,11 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
       waves += jobs[i].nwaves;
       ++b.njobs;
     }
-    for (int i : members) {
+    for (int i = 0; i < cnt; ++i) {
       hipEvent_t ev = es[i]->ev_get();
       HIPC(hipEventRecord(ev, es[i]->s));
       HIPC(hipStreamWaitEvent(bc.s, ev, 0));
-      es[i]->ev_free.push_back(ev);
+      dones.v.emplace_back(es[i], ev);
     }
     HIPC(hipMemsetAsync(bar, 0, (4 + (size_t)waves) * sizeof(unsigned), bc.s));
     std::vector<Engine::Timed> tm;
@@ -1551,6 +1597,8 @@ template <typename F> int guarded(F &&f) {
 void check_handle(gm_handle *h) {
   if (!h || !h->e)
     throw Fail(GM_ERR_INVALID, "null or destroyed gm_handle");
+  if (h->e->release_pending)
+    throw Fail(GM_ERR_STATE, "the session was released (gm_release)");
   HIPC(hipSetDevice(h->e->device));
 }
 
@@ -1683,9 +1731,62 @@ int gm_run(gm_handle *h, uint32_t ni, uint32_t nu, uint32_t ne,
 
 int gm_close(gm_handle *h) {
   return guarded([&] {
+    if (h && h->e && h->e->release_pending)
+      return; // released sessions are closed already
     check_handle(h);
     HIPC(hipStreamSynchronize(h->e->s));
     h->e->closed = true;
+  });
+}
+
+int gm_release(gm_handle *h) {
+  return guarded([&] {
+    if (!h || !h->e)
+      throw Fail(GM_ERR_INVALID, "null or destroyed gm_handle");
+    gm::Engine *e = h->e;
+    std::lock_guard<std::mutex> l(e->rel_mu);
+    if (e->release_pending)
+      return;
+    HIPC(hipSetDevice(e->device));
+    HIPC(hipStreamSynchronize(e->s));
+    e->closed = true;
+    e->release_pending = true;
+    gm::release_if_unshared(e);
+  });
+}
+
+void gm_shutdown(void) {
+  // no device work of any session is in flight once both locks are held
+  std::lock_guard<std::mutex> bl(gm::g_batch_mu);
+  std::unique_lock<gm::DevLock> dl(gm::g_dev_mu);
+  std::vector<gm::Engine *> live;
+  {
+    std::lock_guard<std::mutex> l(gm::g_live_mu);
+    live = gm::g_live;
+  }
+  for (gm::Engine *e : live) {
+    std::lock_guard<std::mutex> l(e->rel_mu);
+    (void)hipSetDevice(e->device);
+    e->closed = true;
+    e->release_pending = true;
+    e->release();
+    e->released.store(true);
+  }
+  for (auto &b : gm::g_batch) {
+    (void)hipSetDevice(b.device);
+    (void)hipStreamSynchronize(b.s);
+    (void)hipStreamDestroy(b.s);
+    (void)hipFree(b.bar);
+    (void)hipHostFree(b.h_flag);
+  }
+  gm::g_batch.clear();
+}
+
+int gm_is_released(gm_handle *h, int32_t *out) {
+  return guarded([&] {
+    if (!h || !h->e || !out)
+      throw Fail(GM_ERR_INVALID, "null argument");
+    *out = h->e->released.load() ? 1 : 0;
   });
 }
 
@@ -1693,7 +1794,10 @@ void gm_destroy(gm_handle *h) {
   if (!h)
     return;
   if (h->e) {
-    (void)hipSetDevice(h->e->device);
+    // a session freed by gm_shutdown makes no further runtime call (its
+    // handle may be destroyed after the runtime has finalized)
+    if (!h->e->released.load())
+      (void)hipSetDevice(h->e->device);
     gm::engine_unref(h->e);
   }
   h->e = nullptr;
@@ -1920,6 +2024,10 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
     if (n == 0 || c < 1 || c > 4 || nnz < 0 || nsr < 1 || per_epoch < 0 ||
         per_epoch > 2)
       throw Fail(GM_ERR_INVALID, "invalid SGD arguments");
+    // the SGD kernels address the embedding with 32-bit byte offsets (the
+    // limit gm_create applies to n_instances * n_components)
+    if (n * (uint64_t)c * sizeof(float) >= 0x80000000ull)
+      throw Fail(GM_ERR_INVALID, "n * c must be below 2^29");
     for (int64_t i = 0; i < nnz; ++i) {
       if (head[i] == -1 && tail[i] == -1)
         continue; // an empty slot

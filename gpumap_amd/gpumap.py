@@ -115,6 +115,19 @@ class GPUMAP:
             self._impl.close()
             self._closed = True
 
+    def release(self):
+        """Close the session and free its device memory as soon as no tensor
+        returned by a getter holds it (no counterpart in the reference, whose
+        storage lives until the object is destroyed); the session accepts no
+        further calls."""
+        self._impl.release()
+        self._closed = True
+
+    @property
+    def released(self) -> bool:
+        """True once a released session's device memory has been freed."""
+        return self._impl.released
+
     def _feed(self, n_insertion_ops: int) -> None:
         if n_insertion_ops <= 0:
             return

@@ -19,42 +19,71 @@ distribution, as two runs alone do.
 from __future__ import annotations
 
 import threading
+import time
 from concurrent.futures import Future
 
 from .gpumap import GPUMAP
 from ._native import _core
 
-_POOL_W = 32          # pool slots per row
-_SURVIVOR_CAP = 2048  # BF16 screening survivors per batch row (x8 above
-                      # 2^18 rows)
+_POOL_W = 32            # pool slots per row
+_CAND = 256             # NN-Descent candidates per inserted row
+_SURVIVOR_CAP = 2048    # BF16 screening survivors per batch row (x8 above
+                        # 2^18 rows)
+_MAX_SPLIT = 128        # column splits of the exact insertion's tiles
 _LB_STORE_MAX = 8 << 30
+_SGD_SLOTS = 4 + 512    # cooperative SGD barrier words and wave slots
+_GRANULE = 2 << 20      # device allocations are reserved in 2 MiB granules
+
+
+def _allocations(n: int, d: int, k: int, c: int, knn_mode: int,
+                 b: int) -> list[int]:
+    """Bytes of each device allocation of a session of n rows (Engine::
+    allocate) and of its insertion scratch for batches of up to b rows."""
+    dp = (d + 3) // 4 * 4
+    nk = n * k
+    sizes = [n * dp * 4,                      # rows
+             n * _POOL_W * 8,                 # pools
+             nk * 4, nk * 4,                  # public ids, distances
+             n * c * 4, n * c * 4,            # embedding, its snapshot
+             n, n, n * 4, n * 4, n * 4,       # flags, marks, list, pmax, deg
+             (max(2 * nk, n) // 1024 + 2) * 4,
+             n * _POOL_W * 4, n * _POOL_W * 4,        # sampled lists
+             n * _POOL_W * 8, n * _POOL_W * 8]        # reverse lists
+    sizes += [n * 4] * 6                      # masks, owners, rows, counts
+    sizes += [n * 8, n * 8]                   # sigma, rho
+    sizes += [nk * 4, nk]                     # weights, weight flags
+    sizes += [2 * nk * 4] * 6                 # COO and SGD schedule
+    sizes += [n * 4]                          # embedding batch
+    sizes += [256 * 8, 256 * 8, 8, 64, 8, 128, 4, 4, _SGD_SLOTS * 4]
+    # the exact tiles' per-(row, split) top lists (FP32 and BF16 modes) or
+    # the NN-Descent candidates
+    if knn_mode == 0:
+        sizes += [b * _POOL_W * 4, b * 4, b * _CAND * 4, b * _CAND * 4, b * 4]
+    else:
+        sizes += [b * _MAX_SPLIT * _POOL_W * 8]
+    if knn_mode == 2:
+        db = (d + 15) // 16 * 16
+        rows = (n + 127) // 128 * 128
+        cap = _SURVIVOR_CAP if n <= 1 << 18 else 8 * _SURVIVOR_CAP
+        sizes += [rows * db * 4, rows * 4, 16]            # split rows, norms
+        sizes += [(b + 127) // 128 * 128 * db * 4, b * 4,  # batch operands
+                  b * cap * 4, 2 * b * 4, b * cap * 8]
+        if n * b * 4 <= _LB_STORE_MAX:
+            sizes += [n * b * 4]                          # lower-bound store
+    return sizes
 
 
 def session_footprint(n_instances: int, n_features: int, n_neighbors: int = 15,
                       n_components: int = 2, knn_mode: int = 2,
                       max_batch_rows: int = 1000) -> int:
-    """Device bytes a session of this shape allocates (the engine's session
-    arrays plus the scratch of insertions of up to `max_batch_rows` rows)."""
-    n, d, k, c = n_instances, n_features, n_neighbors, n_components
-    dp = (d + 3) // 4 * 4
-    per_row = (4 * dp                    # rows
-               + 8 * _POOL_W * 4         # pool, reverse lists (2), flagged ids
-               + 8 * _POOL_W             # candidate lists (2 x u32)
-               + 8 * k                   # public rows
-               + 5 * k + 48 * k          # weights, flags, COO and schedule
-               + 8 * c                   # embedding and its snapshot
-               + 16 + 48)                # rho, sigma, per-row counters
-    total = n * per_row
-    b = max_batch_rows
-    if knn_mode == 2:
-        db = (d + 15) // 16 * 16
-        total += (n + 127) // 128 * 128 * (4 * db + 4)   # split rows, norms
-        total += (b + 127) // 128 * 128 * 4 * db          # batch operands
-        cap = _SURVIVOR_CAP if n <= 1 << 18 else 8 * _SURVIVOR_CAP
-        total += b * cap * 12 + b * 128 * _POOL_W * 8
-        if n * b * 4 <= _LB_STORE_MAX:
-            total += n * b * 4                            # lower-bound store
-    return int(total)
+    """Device bytes a session of this shape can hold: each of the engine's
+    allocations (its session arrays and the scratch of insertions of up to
+    `max_batch_rows` rows) rounded up to the allocator's 2 MiB granule."""
+    g = _GRANULE
+    return int(sum((s + g - 1) // g * g
+                   for s in _allocations(n_instances, n_features, n_neighbors,
+                                         n_components, knn_mode,
+                                         max_batch_rows)))
 
 
 def _device_free_bytes() -> int | None:
@@ -70,26 +99,52 @@ def _device_free_bytes() -> int | None:
 
 class Admission:
     """Footprint bookkeeping of a server: `acquire` waits until a footprint
-    fits the budget (and a session slot is free), `release` frees it."""
+    fits the budget (and a session slot is free), `release` frees it at once
+    and `release_when` once a probe reports that the session's device
+    memory has really been freed."""
 
-    def __init__(self, budget_bytes: int, max_sessions: int):
+    def __init__(self, budget_bytes: int, max_sessions: int,
+                 poll_interval: float = 0.02):
         self.budget = int(budget_bytes)
         self.max_sessions = int(max_sessions)
         self.used = 0
         self.sessions = 0
+        self.poll_interval = float(poll_interval)
+        self._freeing: list[tuple[int, object]] = []
         self._cv = threading.Condition()
+
+    def _sweep(self) -> None:
+        # caller holds the lock: footprints whose memory is gone are returned
+        keep = []
+        for nbytes, probe in self._freeing:
+            if probe():
+                self.used -= nbytes
+                self.sessions -= 1
+            else:
+                keep.append((nbytes, probe))
+        self._freeing = keep
 
     def acquire(self, nbytes: int, timeout: float | None = None) -> None:
         if nbytes > self.budget:
             raise MemoryError(f"session needs {nbytes} bytes, more than the "
                               f"server's budget of {self.budget}")
+        deadline = None if timeout is None else time.monotonic() + timeout
         with self._cv:
-            ok = self._cv.wait_for(
-                lambda: (self.used + nbytes <= self.budget
-                         and self.sessions < self.max_sessions), timeout)
-            if not ok:
-                raise TimeoutError("no room for the session before the "
-                                   "admission timeout")
+            while True:
+                self._sweep()
+                if (self.used + nbytes <= self.budget
+                        and self.sessions < self.max_sessions):
+                    break
+                left = None if deadline is None else deadline - time.monotonic()
+                if left is not None and left <= 0:
+                    raise TimeoutError("no room for the session before the "
+                                       "admission timeout")
+                if self._freeing:
+                    # memory held by a closed session's exported tensors is
+                    # freed without a notification: poll for it
+                    left = (self.poll_interval if left is None
+                            else min(left, self.poll_interval))
+                self._cv.wait(left)
             self.used += nbytes
             self.sessions += 1
 
@@ -98,6 +153,20 @@ class Admission:
             self.used -= nbytes
             self.sessions -= 1
             self._cv.notify_all()
+
+    def release_when(self, nbytes: int, probe) -> None:
+        """Return `nbytes` once `probe()` is true (checked now and by every
+        waiting `acquire`)."""
+        with self._cv:
+            self._freeing.append((nbytes, probe))
+            self._sweep()
+            self._cv.notify_all()
+
+    def pending_release(self) -> int:
+        """Bytes of closed sessions whose device memory is still held."""
+        with self._cv:
+            self._sweep()
+            return sum(nb for nb, _ in self._freeing)
 
 
 class ServedSession:
@@ -111,6 +180,10 @@ class ServedSession:
         self._model = model
         self.footprint = footprint
         self._closed = False
+        # (insertion, update, embedding) ops of each cycle the session ran,
+        # in order: its run() calls replayed by run_with_n_ops give the same
+        # results
+        self.ops_history: list[tuple[int, int, int]] = []
 
     @property
     def params(self):
@@ -185,6 +258,7 @@ class Server:
         self._cv = threading.Condition()
         self._pending: dict[ServedSession, tuple] = {}
         self._queued: list[tuple] = []
+        self._running: set = set()  # sessions of the cycle in flight
         self._stop = False
         self.cycles = 0          # batched cycles run
         self.requests_run = 0    # requests over those cycles
@@ -218,14 +292,27 @@ class Server:
 
     def _close_session(self, s: ServedSession) -> None:
         with self._cv:
-            # a queued request of a closing session is cancelled
+            # every request of a closing session still waiting is cancelled:
+            # the pending one and those queued behind it
             req = self._pending.pop(s, None)
+            dropped = [q for q in self._queued if q[0] is s]
             self._queued = [q for q in self._queued if q[0] is not s]
+            self._last = [q for q in self._last if q is not s]
             self._cv.notify_all()
+            # a cycle in flight still uses the session's device arrays (the
+            # loop thread itself never waits: it closes sessions only from
+            # result callbacks, after the cycle's device work)
+            if threading.current_thread() is not self._thread:
+                self._cv.wait_for(lambda: s not in self._running)
         if req is not None:
             req[2].cancel()
-        s._model.close()
-        self.admission.release(s.footprint)
+        for q in dropped:
+            q[2].cancel()
+        # the device memory is freed once no tensor from a getter holds it;
+        # the footprint returns to the budget only then
+        model = s._model
+        model.release()
+        self.admission.release_when(s.footprint, lambda: model.released)
 
     # -- requests --------------------------------------------------------
     def _submit(self, s, target_latency, n_ops) -> Future:
@@ -263,6 +350,7 @@ class Server:
                 batch = list(self._pending.items())
                 self._pending.clear()
                 self._last = [s for s, _ in batch]
+                self._running = set(self._last)
                 # the next queued request of each session joins the
                 # following cycle
                 rest = []
@@ -272,7 +360,16 @@ class Server:
                     else:
                         rest.append(q)
                 self._queued = rest
-            self._run_cycle(batch)
+            try:
+                self._run_cycle(batch)
+            finally:
+                self._cycle_done()
+
+    def _cycle_done(self) -> None:
+        # the cycle's device work has finished: its sessions may be released
+        with self._cv:
+            self._running = set()
+            self._cv.notify_all()
 
     def _run_cycle(self, batch) -> None:
         live, ops = [], []
@@ -300,13 +397,16 @@ class Server:
         try:
             results = _core.run_batch([s._model._impl for s, _ in live], ops)
         except BaseException as e:
+            self._cycle_done()
             for _, fut in live:
                 fut.set_exception(e)
             return
+        self._cycle_done()
         self.cycles += 1
         self.requests_run += len(live)
         self.max_batch = max(self.max_batch, len(live))
-        for (s, fut), r in zip(live, results):
+        for (s, fut), o, r in zip(live, ops, results):
+            s.ops_history.append(tuple(int(v) for v in o))
             try:
                 fut.set_result(s._model._finish(r))
             except BaseException as e:  # pragma: no cover - strategy errors

@@ -146,6 +146,24 @@ int gm_close(gm_handle *h);
  * of the handle is deleted. */
 void gm_destroy(gm_handle *h);
 
+/* Close the session and free its device memory as soon as no exported DLPack
+ * tensor holds it: at this call, or when the last such tensor is deleted.
+ * Every later call on the handle except gm_close, gm_is_released and
+ * gm_destroy fails with GM_ERR_STATE.  No counterpart in the reference,
+ * whose storage lives until the object is destroyed (impl.cuh:502-507); a
+ * server uses it to return a closed session's HBM before admitting others. */
+int gm_release(gm_handle *h);
+
+/* *out = 1 once the device memory of a released session has been freed. */
+int gm_is_released(gm_handle *h, int32_t *out);
+
+/* Process teardown: waits for the device work in flight, then frees the
+ * device memory of every live session (exported tensors included) and the
+ * batched cycles' streams, while the HIP runtime is still up.  Afterwards
+ * every session call fails with GM_ERR_STATE.  Meant for an exit hook (the
+ * Python package registers one); no counterpart in the reference. */
+void gm_shutdown(void);
+
 /* get_embedding / get_neighbor_indices / get_neighbor_dists --
  * gpumap.cuh:89-91, impl.cuh:509-533.  The reference returns the full
  * capacity; these describe the n_inserted rows (the slicing the Python facade

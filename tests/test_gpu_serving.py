@@ -268,3 +268,104 @@ def test_server_admits_a_waiting_session_when_one_closes():
         assert r1.n_inserted_instances == r2.n_inserted_instances == 400
         b.close()
         c.close()
+
+
+def test_batched_launch_closed_mid_loop_waits_for_later_insertions():
+    # 16 small sessions fill a batched SGD launch, which closes while the
+    # loop has not yet reached two later sessions whose large FP32-tile
+    # insertions were enqueued at the start of the cycle.  The launch waits
+    # for them (its waves would otherwise wait at the epoch boundaries for
+    # CUs the insertion kernels hold); the cycle completes and the large
+    # sessions' results equal their run alone.
+    small = [ev.gaussian10(n=600, d=8, index=60 + i) for i in range(16)]
+    big = [ev.gaussian10(n=24_000, d=256, index=80 + i) for i in range(2)]
+    es = [capi.Engine(*x.shape, knn_mode=2) for x in small]
+    for e, x in zip(es, small):
+        e.append(x)
+
+    def big_sessions():
+        out = []
+        for x in big:
+            e = capi.Engine(*x.shape, knn_mode=1)
+            e.append(x[:20_000])
+            e.run(20_000, 0, 0)
+            e.append(x[20_000:])
+            out.append(e)
+        return out
+
+    ref = big_sessions()
+    ref_rs = [e.run(4000, 0, 0) for e in ref]
+    bs = big_sessions()
+    ops = [(600, 0, 600)] * 16 + [(4000, 0, 0)] * 2
+    rs = capi.Engine.run_batch(es + bs, ops)
+    assert [q.n_embedding_ops for q in rs[:16]] == [600] * 16
+    for e in es:
+        assert np.isfinite(e.embedding()).all()
+    for r, q, e, er in zip(ref_rs, rs[16:], ref, bs):
+        assert [getattr(r, f) for f in RESULT_FIELDS] == \
+            [getattr(q, f) for f in RESULT_FIELDS]
+        assert np.array_equal(e.knn()[0], er.knn()[0])
+
+
+def _free_bytes():
+    import torch
+
+    torch.cuda.synchronize()
+    return torch.cuda.mem_get_info()[0]
+
+
+def test_footprint_covers_what_a_session_allocates_and_release_frees_it():
+    # hipMemGetInfo before and after creating a session and running one
+    # insertion of the largest batch the footprint allows for: the estimate
+    # is at least what the session took.  gm_release frees it once the
+    # tensor exported by a getter is gone.
+    import gc
+
+    from gpumap_amd import GPUMAP
+    from gpumap_amd.data_loader import ArrayDataLoader
+    from gpumap_amd.serving import session_footprint
+
+    x = ev.image_like(n=40_000, d=784, index=5)
+    n, d = x.shape
+    for mode in ("exact_bf16", "exact_fp32", "nndescent"):
+        p = _params(n, d)
+        p.knn_mode = mode
+        code = {"nndescent": 0, "exact_fp32": 1, "exact_bf16": 2}[mode]
+        f0 = _free_bytes()
+        m = GPUMAP(ArrayDataLoader(x), p)
+        m.run_with_n_ops(1000, 1000, 1000)
+        m.run_with_n_ops(1000, 1000, 1000)
+        used = f0 - _free_bytes()
+        fp = session_footprint(n, d, 15, 2, code, max_batch_rows=1000)
+        assert fp >= used, (mode, fp, used)
+        emb = m.get_embedding()
+        assert emb.shape == (2000, 2)
+        m.release()
+        assert not m.released               # the exported tensor holds it
+        with pytest.raises(RuntimeError):
+            m.run_with_n_ops(0, 0, 100)
+        held = f0 - _free_bytes()
+        del emb
+        gc.collect()
+        assert m.released
+        freed = f0 - _free_bytes()
+        assert held - freed >= 0.9 * used, (mode, held, freed, used)
+        del m
+        gc.collect()
+
+
+def test_server_close_returns_the_memory_before_admitting():
+    from gpumap_amd.data_loader import ArrayDataLoader
+    from gpumap_amd.serving import Server
+
+    x = ev.gaussian10(n=30_000, d=128, index=12)
+    with Server(max_sessions=8) as srv:
+        f0 = _free_bytes()
+        a = srv.open(ArrayDataLoader(x), _params(*x.shape))
+        a.run_with_n_ops(1000, 1000, 1000)
+        held = f0 - _free_bytes()
+        assert held > 30_000 * 128 * 4
+        a.close()
+        assert srv.admission.pending_release() == 0
+        assert srv.admission.used == 0
+        assert f0 - _free_bytes() < 0.1 * held
