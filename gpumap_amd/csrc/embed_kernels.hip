@@ -1132,6 +1132,16 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
   uint32_t dr[8];
   neg_draws(seed, e, dr);
   EdgeLoads<NC> L0, L1;
+  // What this lane's edge and its twin added to the edge's two rows in the
+  // previous epoch: epoch e + 1's positions are read before epoch e's
+  // updates are issued, so the lane adds its own pair's share of them to
+  // what it reads (updates of the same rows by other lanes stay one epoch
+  // old, a Hogwild read).  Without it an edge sampled every epoch would
+  // take each step from the position before its previous one and overshoot.
+  float own_h[NC], own_t[NC];
+#pragma unroll
+  for (int d = 0; d < NC; ++d)
+    own_h[d] = own_t[d] = 0.0f;
   bool act0 = !(nx > 0.0f);
   sgd_gather_all<NC>(emb, act0, hh, tt, n, ep_neg, nn, 0, dr, L0);
   if (act0)
@@ -1160,11 +1170,24 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
     neg_draws(es + 2, e, dr);
     float gh[NC], gt[NC];
 #pragma unroll
-    for (int d = 0; d < NC; ++d)
+    for (int d = 0; d < NC; ++d) {
       gh[d] = gt[d] = 0.0f;
+      L0.cur[d] += own_h[d];
+      L0.oth[d] += own_t[d];
+    }
     if (act0)
       sgd_apply<NC>(emb, hh, e, n, ep_neg, alpha, gamma, a, b, es, rounding,
                     L0, gh, gt, c_pos, c_neg);
+    // this epoch's share of the pair's rows, for the positions of the next
+    // (paired: the head row gets the edge's head and the twin's tail
+    // contributions, the tail row the twin's total)
+#pragma unroll
+    for (int d = 0; d < NC; ++d) {
+      const float twin_t = __shfl_xor(gt[d], 1, 64);
+      const float pair_h = gh[d] + twin_t;
+      own_h[d] = ag.paired ? pair_h : gh[d];
+      own_t[d] = ag.paired ? __shfl_xor(pair_h, 1, 64) : gt[d];
+    }
     alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
     // boundary: epoch + 1 starts once every slot shows epoch - lag + 1 (the
     // first check is outside the polling loop so that its wait is only for

@@ -96,6 +96,13 @@ def lib():
         L.or_sgd.argtypes = [P(ctypes.c_float), u64, i32, P(i32), P(i32),
                              P(ctypes.c_float), i64, i32, i32, f32, f32, f32,
                              f32, u64, i32]
+        L.or_sgd_trace.argtypes = [P(ctypes.c_float), u64, i32, P(i32),
+                                   P(i32), P(ctypes.c_float), i64, i32, i32,
+                                   f32, f32, f32, f32, u64, vp, i64,
+                                   P(ctypes.c_float)]
+        L.or_sgd_trace.restype = i64
+        L.or_rounding_factor.argtypes = [P(i32), i64, u64, f32]
+        L.or_rounding_factor.restype = f32
         _lib = L
     return _lib
 
@@ -153,6 +160,41 @@ def sgd(emb, head, tail, w, n_epochs, nsr, a, b, alpha0=1.0, gamma=1.0,
     return emb
 
 
+# or_sgd_event (oracle.h): one applied gradient step
+SGD_EVENT = np.dtype([("epoch", np.int32), ("edge", np.int32),
+                      ("kind", np.int32), ("other", np.int32),
+                      ("alpha", np.float32), ("d2", np.float32),
+                      ("coef", np.float32), ("y", np.float32, 4),
+                      ("o", np.float32, 4), ("g", np.float32, 4)])
+SGD_POSITIVE, SGD_NEGATIVE, SGD_NEGATIVE_SKIPPED = 0, 1, 2
+
+
+def sgd_trace(emb, head, tail, w, n_epochs, nsr, a, b, alpha0=1.0, gamma=1.0,
+              seed=42, cap=1 << 20):
+    """Serial SGD with every gradient step recorded: (embedding, events,
+    rounding grid)."""
+    emb = np.ascontiguousarray(emb, dtype=np.float32).copy()
+    head = np.ascontiguousarray(head, dtype=np.int32)
+    tail = np.ascontiguousarray(tail, dtype=np.int32)
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    ev = np.zeros(cap, SGD_EVENT)
+    r = ctypes.c_float(0.0)
+    m = lib().or_sgd_trace(_ptr(emb, ctypes.c_float), emb.shape[0],
+                           emb.shape[1], _ptr(head, ctypes.c_int32),
+                           _ptr(tail, ctypes.c_int32), _ptr(w, ctypes.c_float),
+                           len(w), n_epochs, nsr, a, b, alpha0, gamma, seed,
+                           ev.ctypes.data, cap, ctypes.byref(r))
+    if m > cap:
+        raise ValueError(f"{m} SGD steps exceed the trace capacity {cap}")
+    return emb, ev[:m], float(r.value)
+
+
+def rounding_factor(head, n, alpha):
+    head = np.ascontiguousarray(head, dtype=np.int32)
+    return float(lib().or_rounding_factor(_ptr(head, ctypes.c_int32),
+                                          len(head), n, alpha))
+
+
 class Oracle:
     """The oracle engine with the reference's run(i, u, e) contract."""
 
@@ -202,6 +244,11 @@ class Oracle:
         e = np.zeros((self.n, self.c), np.float32)
         lib().or_get_embedding(self._h, _ptr(e, ctypes.c_float))
         return e
+
+    def set_embedding(self, e):
+        e = np.ascontiguousarray(e, dtype=np.float32)
+        assert e.shape == (self.n, self.c)
+        lib().or_set_embedding(self._h, _ptr(e, ctypes.c_float))
 
     def pool(self):
         p = np.zeros((self.n, 32), np.uint64)

@@ -417,11 +417,15 @@ float rounding_factor(const std::vector<int32_t> &head, uint64_t n,
 // Epoch loop of run_optimize_layout / optimize_batch_kernel
 // (embedding_kernel.cuh:509-589,766-793) for any n_components, on the
 // already-trimmed COO with weights w.  n_threads > 1 runs the edges
-// Hogwild-style with relaxed atomic adds like the device kernel.
+// Hogwild-style with relaxed atomic adds like the device kernel.  With
+// `trace` (serial runs only) every gradient step is recorded as it is
+// applied, for the tests that check each one.
 void sgd_epochs(float *emb, uint64_t n, int c, const int32_t *head,
                 const int32_t *tail, const float *w, int64_t nnz,
                 int n_epochs, int nsr, float a, float b, float alpha0,
-                float gamma, uint64_t seed, int n_threads) {
+                float gamma, uint64_t seed, int n_threads,
+                std::vector<or_sgd_event> *trace = nullptr,
+                float *rounding_out = nullptr) {
   if (nnz <= 0 || n == 0 || n_epochs <= 0)
     return;
   float max_w = 0.0f;
@@ -439,8 +443,30 @@ void sgd_epochs(float *emb, uint64_t n, int c, const int32_t *head,
   }
   std::vector<int32_t> hv(head, head + nnz);
   const float rounding = rounding_factor(hv, n, alpha0);
+  if (rounding_out)
+    *rounding_out = rounding;
+  if (trace)
+    n_threads = 1;
   auto add = [&](float *p, float v) {
     std::atomic_ref<float>(*p).fetch_add(v, std::memory_order_relaxed);
+  };
+  auto record = [&](int epoch, int64_t e, int kind, int32_t other, float alpha,
+                    float d2, float coef, const float *y, const float *o,
+                    const float *g) {
+    or_sgd_event ev{};
+    ev.epoch = epoch;
+    ev.edge = (int32_t)e;
+    ev.kind = kind;
+    ev.other = other;
+    ev.alpha = alpha;
+    ev.d2 = d2;
+    ev.coef = coef;
+    for (int d = 0; d < c; ++d) {
+      ev.y[d] = y[d];
+      ev.o[d] = o ? o[d] : 0.0f;
+      ev.g[d] = g ? g[d] : 0.0f;
+    }
+    trace->push_back(ev);
   };
   float alpha = alpha0;
   for (int epoch = 0; epoch < n_epochs; ++epoch) {
@@ -455,7 +481,7 @@ void sgd_epochs(float *emb, uint64_t n, int c, const int32_t *head,
       const int32_t j = head[e], kk = tail[e];
       float *cur_p = emb + (size_t)j * c;
       float *oth_p = emb + (size_t)kk * c;
-      float cur[4], oth[4], grads[4];
+      float cur[4], oth[4], grads[4], step[4];
       for (int d = 0; d < c; ++d) {
         cur[d] = std::atomic_ref<float>(cur_p[d]).load(std::memory_order_relaxed);
         oth[d] = std::atomic_ref<float>(oth_p[d]).load(std::memory_order_relaxed);
@@ -468,11 +494,15 @@ void sgd_epochs(float *emb, uint64_t n, int c, const int32_t *head,
       float attr = 0.0f;
       if (d2 > 0.0f)
         attr = (-2.0f * a * b * powf(d2, b - 1.0f)) / (a * powf(d2, b) + 1.0f);
+      float y0[4];
       for (int d = 0; d < c; ++d) {
+        y0[d] = cur[d];
         const float g = fclip(attr * (cur[d] - oth[d])) * alpha;
         cur[d] += g;
         grads[d] = g;
       }
+      if (trace)
+        record(epoch, e, 0, kk, alpha, d2, attr, y0, oth, grads);
       for (int d = 0; d < c; ++d)
         add(&oth_p[d], (rounding + -grads[d]) - rounding);
       nxt[e] = ns + ep;
@@ -484,8 +514,12 @@ void sgd_epochs(float *emb, uint64_t n, int c, const int32_t *head,
           // index in [0, n): high word of the 32 x 32-bit product
           const uint32_t r =
               (uint32_t)(((uint64_t)rng.next() * (uint64_t)n) >> 32);
-          if ((int32_t)r == j)
+          if ((int32_t)r == j) {
+            if (trace)
+              record(epoch, e, 2, (int32_t)r, alpha, 0.0f, 0.0f, cur, nullptr,
+                     nullptr);
             continue;
+          }
           const float *neg_p = emb + (size_t)r * c;
           float neg[4];
           for (int d = 0; d < c; ++d)
@@ -500,11 +534,15 @@ void sgd_epochs(float *emb, uint64_t n, int c, const int32_t *head,
           if (d2n > 0.0f)
             rep = (2.0f * gamma * b) / ((0.001f + d2n) * (a * powf(d2n, b) + 1.0f));
           for (int d = 0; d < c; ++d) {
+            y0[d] = cur[d];
             const float g =
                 rep > 0.0f ? fclip(rep * (cur[d] - neg[d])) * alpha : 4.0f * alpha;
             cur[d] += g;
             grads[d] += g;
+            step[d] = g;
           }
+          if (trace)
+            record(epoch, e, 1, (int32_t)r, alpha, d2n, rep, y0, neg, step);
         }
         nneg[e] = nn + (float)n_neg * ep_neg;
       }
@@ -1134,6 +1172,29 @@ int64_t or_fuzzy(const uint32_t *idx, const float *dist, uint64_t n, int k,
     std::memcpy(rhos, fz.rho.data(), n * 8);
   return (int64_t)nnz;
 }
+int64_t or_sgd_trace(float *emb, uint64_t n, int c, const int32_t *head,
+                     const int32_t *tail, const float *w, int64_t nnz,
+                     int n_epochs, int nsr, float a, float b, float alpha0,
+                     float gamma, uint64_t seed, or_sgd_event *events,
+                     int64_t cap, float *rounding) {
+  std::vector<or_sgd_event> tr;
+  float r = 0.0f;
+  sgd_epochs(emb, n, c, head, tail, w, nnz, n_epochs, nsr, a, b, alpha0,
+             gamma, seed, 1, &tr, &r);
+  if (rounding)
+    *rounding = r;
+  const int64_t m = std::min<int64_t>(cap, (int64_t)tr.size());
+  for (int64_t i = 0; i < m; ++i)
+    events[i] = tr[i];
+  return (int64_t)tr.size();
+}
+
+float or_rounding_factor(const int32_t *head, int64_t nnz, uint64_t n,
+                         float alpha) {
+  std::vector<int32_t> hv(head, head + nnz);
+  return rounding_factor(hv, n, alpha);
+}
+
 void or_sgd(float *emb, uint64_t n, int c, const int32_t *head,
             const int32_t *tail, const float *w, int64_t nnz, int n_epochs,
             int nsr, float a, float b, float alpha0, float gamma,

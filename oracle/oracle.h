@@ -82,6 +82,29 @@ int64_t or_fuzzy(const uint32_t *idx, const float *dist, uint64_t n, int k,
                  int euclidean, const uint32_t *batch, uint64_t n_batch,
                  int n_epochs, int32_t *head, int32_t *tail, float *w,
                  double *sigmas, double *rhos);
+/* One gradient step of the SGD, as applied: kind 0 the positive sample
+ * (other = the tail), 1 a negative sample (other = the drawn row), 2 a
+ * negative draw equal to the head, skipped.  y is the head's local position
+ * before the step, o the other row's position read, g the step added to the
+ * head (alpha and clipping included); d2 and coef are the squared distance
+ * and the gradient coefficient of the step. */
+typedef struct or_sgd_event {
+  int32_t epoch, edge, kind, other;
+  float alpha, d2, coef;
+  float y[4], o[4], g[4];
+} or_sgd_event;
+
+/* or_sgd run serially with every step recorded (up to cap events); returns
+ * the number of steps; *rounding (may be NULL) receives the rounding grid */
+int64_t or_sgd_trace(float *emb, uint64_t n, int c, const int32_t *head,
+                     const int32_t *tail, const float *w, int64_t nnz,
+                     int n_epochs, int nsr, float a, float b, float alpha0,
+                     float gamma, uint64_t seed, or_sgd_event *events,
+                     int64_t cap, float *rounding);
+/* the SGD's rounding grid for a COO's head degrees (embedding_kernel.cuh:
+ * 405-460) */
+float or_rounding_factor(const int32_t *head, int64_t nnz, uint64_t n,
+                         float alpha);
 /* SGD epochs on a given COO (weights w, untrimmed allowed) */
 void or_sgd(float *emb, uint64_t n, int c, const int32_t *head,
             const int32_t *tail, const float *w, int64_t nnz, int n_epochs,

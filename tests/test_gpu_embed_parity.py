@@ -1,0 +1,156 @@
+"""Device parity of the embedding-side kernels against the CPU oracle, through
+the C ABI: the initial positions of inserted rows (k_init_embedding), the
+embedding noise (k_minmax_col0, k_noise), the reposition of rows outside a
+batch (k_reposition), the engine's result fields on a hand-computed script,
+the SGD with negative samples and its rounding grid at head degrees above 1.
+
+Tolerances: the device's Box-Muller and gradient use the hardware log, cos,
+exp2 and reciprocal (a few ulp); everything else is the same arithmetic."""
+
+import numpy as np
+import pytest
+
+import oracle
+from gpumap_amd import capi
+from gpumap_amd import evaluation as ev
+from tests.test_oracle_embed_pins import EXPECTED, FIELDS, scripted_run
+from tests.test_oracle_sgd_pins import W_REL
+
+pytestmark = pytest.mark.gpu
+
+
+def test_orchestration_follows_the_hand_computed_sequence():
+    e = capi.Engine(12, 2, n_neighbors=3, knn_mode=1, max_epoch=0)
+    got = scripted_run(e)
+    for step, (g, w) in enumerate(zip(got, EXPECTED)):
+        assert g == w, (step + 1, dict(zip(FIELDS, g)), dict(zip(FIELDS, w)))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_initial_positions_match_oracle(mode):
+    # insertion-only cycles: each batch's rows start from the device's draws
+    # or neighbour means; the graphs are equal, so the positions must be
+    x = ev.gaussian10(n=2400, d=16, index=35)
+    n, d = x.shape
+    e = capi.Engine(n, d, knn_mode=mode)
+    o = oracle.Oracle(n, d, n_threads=8, knn_mode=min(mode, 1))
+    for s in range(0, n, 400):
+        for m in (e, o):
+            m.append(x[s:s + 400])
+            m.run(400, 400, 0)
+        i1, _ = e.knn()
+        i2, _ = o.knn()
+        assert np.array_equal(i1, i2), s
+        np.testing.assert_allclose(e.embedding(), o.embedding(), rtol=1e-5,
+                                   atol=1e-5, err_msg=f"after row {s + 400}")
+
+
+@pytest.mark.parametrize("metric", [0, 1])
+def test_noise_and_reposition_match_oracle(metric):
+    # max_epoch = 0: every embedding op is the noise on its batch and the
+    # reposition of the other rows, deterministic on both sides; partial ops
+    # (cursor sweeps, reposition active) and full sweeps, new-row queues
+    x = ev.gaussian10(n=3000, d=24, index=36)
+    n, d = x.shape
+    e = capi.Engine(n, d, knn_mode=1, max_epoch=0, metric=metric)
+    o = oracle.Oracle(n, d, n_threads=8, knn_mode=1, max_epoch=0,
+                      metric=metric)
+    g = np.random.default_rng(37)
+    plan = [(500, 300), (500, 700), (500, 200), (0, 900), (1500, 100),
+            (0, 3000), (0, 450), (0, 450), (0, 3000), (0, 1000)]
+    pos = 0
+    for cyc, (ins, emb_ops) in enumerate(plan):
+        for m in (e, o):
+            if ins:
+                m.append(x[pos:pos + ins])
+        pos += ins
+        if cyc == 3:
+            # a spread-out start so that the noise and the means are large
+            e0 = g.uniform(-8, 8, size=(pos, 2)).astype(np.float32)
+            e.set_embedding(e0)
+            o.set_embedding(e0)
+        r1, r2 = e.run(ins, n, emb_ops), o.run(ins, n, emb_ops)
+        assert r1.n_embedding_ops == r2.n_embedding_ops
+        assert r1.embedding_completed == r2.embedding_completed
+        np.testing.assert_allclose(e.embedding(), o.embedding(), rtol=1e-5,
+                                   atol=2e-5, err_msg=f"cycle {cyc}")
+
+
+def _conflict_free_case(n=1 << 16, seed0=100):
+    """Eight edges on rows 0..15 with relative weights in W_REL-like ratios,
+    positions spread wide; a seed whose negative samples never land on an
+    edge's rows (checked with the oracle's trace), so that the device's
+    Hogwild order cannot matter."""
+    g = np.random.default_rng(38)
+    emb = g.uniform(-10, 10, size=(n, 2)).astype(np.float32)
+    emb[:16] = g.uniform(-2, 2, size=(16, 2))
+    head = np.arange(0, 16, 2, dtype=np.int32)
+    tail = head + 1
+    w = np.array((W_REL * 2)[:8], np.float32)
+    for seed in range(seed0, seed0 + 50):
+        _, evs, _ = oracle.sgd_trace(emb, head, tail, w, 30, 5, 1.577, 0.895,
+                                     seed=seed)
+        negs = evs[evs["kind"] != oracle.SGD_POSITIVE]
+        if not (negs["other"] < 16).any():
+            return emb, head, tail, w, seed, evs
+    raise AssertionError("no conflict-free seed")
+
+
+def test_sgd_with_negative_samples_matches_oracle():
+    # one launch per epoch (the reference's launch pattern): each epoch reads
+    # the rows as the previous one left them, as the oracle does; 30 epochs,
+    # 5 negative samples per positive sample
+    emb, head, tail, w, seed, evs = _conflict_free_case()
+    assert (evs["kind"] == oracle.SGD_NEGATIVE).sum() > 300
+    want = oracle.sgd(emb, head, tail, w, 30, 5, 1.577, 0.895, seed=seed)
+    got = capi.sgd(emb, head, tail, w, 30, 5, 1.577, 0.895, seed=seed,
+                   per_epoch=True)
+    moved = np.abs(want[:16] - emb[:16]).max()
+    assert moved > 0.5
+    np.testing.assert_allclose(got[:16], want[:16], rtol=1e-4, atol=1e-4)
+    np.testing.assert_array_equal(got[16:], emb[16:])
+
+
+@pytest.mark.parametrize("stream", [False, True])
+def test_sgd_cooperative_kernels_match_oracle(stream):
+    # the cooperative kernels (per-wave k_sgd_waves, streaming k_sgd_stream):
+    # k_sgd_waves reads an epoch's positions before the previous epoch's
+    # updates have landed and adds its own edge's last step to them, so on
+    # the conflict-free set both follow the oracle's trajectory
+    emb, head, tail, w, seed, _ = _conflict_free_case()
+    want = oracle.sgd(emb, head, tail, w, 30, 5, 1.577, 0.895, seed=seed)
+    got = capi.sgd(emb, head, tail, w, 30, 5, 1.577, 0.895, seed=seed,
+                   stream=stream)
+    np.testing.assert_array_equal(got[16:], emb[16:])
+    print(f"max |device - oracle| {np.abs(got[:16] - want[:16]).max():.3g}",
+          flush=True)
+    np.testing.assert_allclose(got[:16], want[:16], rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("maxdeg,step", [(1, 2.0 ** -21), (3, 2.0 ** -20),
+                                         (6, 2.0 ** -19)])
+def test_sgd_rounding_grid_follows_the_maximum_head_degree(maxdeg, step):
+    # edge (0, 1) at the maximum weight is sampled once (2 epochs, nsr 1);
+    # maxdeg - 1 edges of head 5 with weights too small to be sampled before
+    # the end set the maximum degree.  The rounding factor is the smallest
+    # power of two above maxdeg * 4 / (1 - 2 maxdeg eps): 8, 16, 32, and the
+    # updates are multiples of factor * 2^-24 (the floats just below it);
+    # starting from coordinates on a 2^-8 grid, the moved rows land on the
+    # finer grid and not on the next coarser one
+    emb = np.zeros((12, 2), np.float32)
+    emb[0] = [0.0, 0.0]
+    emb[1] = [1.3125, -0.6875]
+    head = np.array([0] + [5] * (maxdeg - 1), np.int32)
+    tail = np.array([1] + list(range(6, 5 + maxdeg)), np.int32)
+    w = np.array([1.0] + [1e-3] * (maxdeg - 1), np.float32)
+    rf = oracle.rounding_factor(head, 12, 1.0)
+    assert rf * 2.0 ** -24 == step
+    for out in (capi.sgd(emb, head, tail, w, 2, 1, 1.577, 0.895),
+                capi.sgd(emb, head, tail, w, 2, 1, 1.577, 0.895,
+                         per_epoch=True),
+                oracle.sgd(emb, head, tail, w, 2, 1, 1.577, 0.895)):
+        dlt = (out[:2] - emb[:2]).astype(np.float64)
+        assert (dlt != 0).all()
+        q = dlt / step
+        assert np.array_equal(q, np.round(q)), q
+        assert not np.array_equal(q / 2, np.round(q / 2)), q

@@ -19,7 +19,8 @@ pytestmark = pytest.mark.gpu
 
 RESULT_FIELDS = ["n_insertion_ops", "n_update_ops", "n_embedding_ops",
                  "insertion_completed", "update_completed",
-                 "n_inserted_instances", "n_update_changed_edges",
+                 "embedding_completed", "n_inserted_instances",
+                 "n_update_changed_edges", "embedding_pending_edges",
                  "insertion_queue_size", "update_queue_size",
                  "embedding_queue_size"]
 
