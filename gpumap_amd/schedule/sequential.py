@@ -1,12 +1,15 @@
-"""Insertion, then update, then embedding, each with the whole budget
-(reference python/gpumap/schedule/sequential.py:9-52)."""
+"""One operation at a time with an unbounded op count: insertion until it
+completes, then the update, then the embedding (behaviour of the reference's
+SequentialPriorityStrategy, python/gpumap/schedule/sequential.py:9-52).  The
+engine caps each count at what is left, so every cycle runs the whole
+remaining operation."""
 
 from __future__ import annotations
 
 from typing import Optional
 
-from .estimator import Estimator, MeanRateEstimator
-from .strategy import BaseStrategy
+from .estimator import Estimator
+from .strategy import UINT32_MAX, BaseStrategy
 
 
 class SequentialPriorityStrategy(BaseStrategy):
@@ -16,19 +19,12 @@ class SequentialPriorityStrategy(BaseStrategy):
                  embedding_estimator: Optional[Estimator] = None,
                  default_n_ops: int = 100, confidence: float = 0.95,
                  min_insertion_ops: int = 500):
-        super().__init__(
-            weights=(1.0, 1.0, 1.0), reserved=0.1,
-            insertion_estimator=insertion_estimator or MeanRateEstimator(),
-            update_estimator=update_estimator or MeanRateEstimator(),
-            embedding_estimator=embedding_estimator or MeanRateEstimator(),
-            n_instances=n_instances, n_neighbors=n_neighbors,
-            default_n_ops=default_n_ops, confidence=confidence,
-            min_insertion_ops=min_insertion_ops)
-        self._update_counter = 0
+        super().__init__((1.0, 1.0, 1.0), 0.1, insertion_estimator,
+                         update_estimator, embedding_estimator, n_instances,
+                         n_neighbors, default_n_ops, confidence,
+                         min_insertion_ops)
 
     def allocate(self, target_latency: float) -> tuple[int, int, int]:
-        if not self.completeness[0]:
-            return (self.max_uint32, 0, 0)
-        if not self.completeness[1]:
-            return (0, self.max_uint32, 0)
-        return (0, 0, self.max_uint32)
+        # the first unfinished operation, the embedding once all are done
+        first = next((i for i, op in enumerate(self._ops) if not op.done), 2)
+        return tuple(UINT32_MAX if i == first else 0 for i in range(3))

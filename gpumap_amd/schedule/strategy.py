@@ -1,62 +1,67 @@
-"""Time-budget split between insertion, update and embedding.
+"""How one run() cycle's time budget is shared by insertion, update and
+embedding.
 
-Contracts of the reference's BaseStrategy (python/gpumap/schedule/
-strategy.py:14-160): weights split the budget as T * w_i / (sum(w) * (1 +
-reserved)); an estimator turns each share into an op count; a cold or failing
-estimator falls back to ``default_n_ops`` +-10 % (numpy's global generator);
-a finished operation's weight drops to 0.
+Behaviour of the reference's BaseStrategy (python/gpumap/schedule/
+strategy.py:14-160): the budget T gives operation i the share
+T * w_i / (sum(w) * (1 + reserved)); its cost model turns the share into an
+op count; a cold or failing model, or one that answers 0 for a positive
+share, is replaced by `default_n_ops` varied by up to 10 % with numpy's
+global generator; a completed operation's weight becomes 0.  Progress per
+operation is inserted / total, 1 - update queue / inserted and
+1 - embedding queue / pending rows.
 """
 
 from __future__ import annotations
 
 from abc import ABC, abstractmethod
+from dataclasses import dataclass
 from typing import Optional
 
 import numpy as np
 
 from .estimator import Estimator, MeanRateEstimator
 
-OPERATIONS = {"insertion": 0, "update": 1, "embedding": 2}
+OPERATIONS = ("insertion", "update", "embedding")
+UINT32_MAX = 2**32 - 1
+
+
+def clip_ops(n_ops: int, ops_range: tuple[int, int]) -> int:
+    """n_ops limited to [low, high]."""
+    low, high = ops_range
+    return min(max(n_ops, low), high)
 
 
 def _jitter(base: int) -> int:
-    lo = max(1, base // 10)
-    return base + int(np.random.randint(-lo, lo))
+    """`base` moved by a draw in [-s, s) with s = max(1, base // 10)."""
+    spread = max(1, base // 10)
+    return base + int(np.random.randint(-spread, spread))
 
 
 def _estimate_or_fallback(estimator: Estimator, budget_sec: float,
                           confidence: float, default_ops: int) -> int:
+    """The model's op count for a budget; a positive budget always gets a
+    positive count (the jittered default when the model has none)."""
     if budget_sec <= 0.0:
         return 0
     try:
-        ops = max(0, int(estimator.estimate_ops(budget_sec, confidence)))
-    except ValueError:
-        return _jitter(default_ops)
-    # a positive budget always makes progress
+        ops = int(estimator.estimate_ops(budget_sec, confidence))
+    except ValueError:  # a model without enough samples
+        ops = 0
     return ops if ops > 0 else _jitter(default_ops)
 
 
-def clip_ops(n_ops: int, ops_range: tuple[int, int]) -> int:
-    return max(ops_range[0], min(n_ops, ops_range[1]))
+@dataclass
+class _Operation:
+    """One of the three operations as a strategy sees it."""
+    name: str
+    weight: float
+    model: Estimator
+    done: bool = False
+    progress: float = 0.0
 
 
 class BaseStrategy(ABC):
-    """Tracks progress in [0, 1] per operation and splits a budget by weight.
-
-    insertion_progress = inserted / total; update_progress = 1 - update queue /
-    inserted; embedding_progress = 1 - embedding queue / pending.
-    """
-
-    insertion_ops_range: tuple[int, int] = (0, 0)
-    update_ops_range: tuple[int, int] = (0, 0)
-    embedding_ops_range: tuple[int, int] = (0, 0)
-
-    _n_instances: int = 0
-    _n_inserted: int = 0
-
-    insertion_progress: float = 0.0
-    update_progress: float = 0.0
-    embedding_progress: float = 0.0
+    """Budget split and progress tracking; `allocate` is the policy."""
 
     def __init__(self, weights: tuple[float, float, float] = (1.0, 1.0, 1.0),
                  reserved: float = 0.2,
@@ -66,60 +71,92 @@ class BaseStrategy(ABC):
                  n_instances: int = 0, n_neighbors: int = 15,
                  default_n_ops: int = 100, confidence: float = 0.95,
                  min_insertion_ops: Optional[int] = None):
-        self._weights = list(weights)
-        self._reserved = reserved
-        self.insertion_estimator = insertion_estimator or MeanRateEstimator()
-        self.update_estimator = update_estimator or MeanRateEstimator()
-        self.embedding_estimator = embedding_estimator or MeanRateEstimator()
-        self._n_instances = n_instances
-        self._n_neighbors = n_neighbors
-        self.default_n_ops = default_n_ops
-        self.confidence = confidence
-        self.min_insertion_ops = min_insertion_ops or int(0.01 * n_instances)
-        self.max_uint32 = 2**32 - 1
-        self._result_history: list = []
-        self.completeness = [False, False, False]
+        models = (insertion_estimator, update_estimator, embedding_estimator)
+        self._ops = tuple(
+            _Operation(name, float(w), m if m is not None else MeanRateEstimator())
+            for name, w, m in zip(OPERATIONS, weights, models))
+        self.reserved = float(reserved)
+        self.n_instances = int(n_instances)
+        self.n_neighbors = int(n_neighbors)
+        self.default_n_ops = int(default_n_ops)
+        self.confidence = float(confidence)
+        # by default an insertion takes at least 1 % of the rows per cycle
+        self.min_insertion_ops = (min_insertion_ops if min_insertion_ops
+                                  else int(0.01 * n_instances))
+        self.n_inserted = 0
 
+    # -- the three operations' models, weights and state ---------------
+    @property
+    def insertion_estimator(self) -> Estimator:
+        return self._ops[0].model
+
+    @property
+    def update_estimator(self) -> Estimator:
+        return self._ops[1].model
+
+    @property
+    def embedding_estimator(self) -> Estimator:
+        return self._ops[2].model
+
+    @property
+    def weights(self) -> tuple[float, float, float]:
+        return tuple(op.weight for op in self._ops)
+
+    @property
+    def completeness(self) -> tuple[bool, bool, bool]:
+        return tuple(op.done for op in self._ops)
+
+    @property
+    def insertion_progress(self) -> float:
+        return self._ops[0].progress
+
+    @property
+    def update_progress(self) -> float:
+        return self._ops[1].progress
+
+    @property
+    def embedding_progress(self) -> float:
+        return self._ops[2].progress
+
+    def get_progress(self) -> tuple[float, float, float]:
+        return tuple(op.progress for op in self._ops)
+
+    # -- policy ---------------------------------------------------------
     @abstractmethod
-    def allocate(self, target_latency: float) -> tuple[int, int, int]: ...
+    def allocate(self, target_latency: float) -> tuple[int, int, int]:
+        """(insertion, update, embedding) ops for one cycle within
+        `target_latency` seconds."""
 
+    def _split(self, total: float) -> tuple[float, float, float]:
+        scale = sum(op.weight for op in self._ops) * (1.0 + self.reserved)
+        if scale <= 0:
+            return (0.0, 0.0, 0.0)
+        return tuple(total * op.weight / scale for op in self._ops)
+
+    # -- feedback from each cycle ------------------------------------------
     def update(self, result) -> None:
-        self._result_history.append(result)
-        if not self.completeness[0] and result.n_insertion_ops > 0:
-            self.insertion_estimator.add_sample(result.n_insertion_ops,
-                                                result.insertion_time)
-        if result.n_update_ops > 0:
-            self.update_estimator.add_sample(result.n_update_ops,
-                                             result.update_time)
-        if result.n_embedding_ops > 0:
-            self.embedding_estimator.add_sample(result.n_embedding_ops,
-                                                result.embedding_time)
-        self._track_progress(result)
+        """Learn the cycle's per-operation costs and track progress.  The
+        insertion's model stops learning once the insertion has completed
+        (its last batches are partial)."""
+        observed = ((result.n_insertion_ops, result.insertion_time),
+                    (result.n_update_ops, result.update_time),
+                    (result.n_embedding_ops, result.embedding_time))
+        for op, (n, secs) in zip(self._ops, observed):
+            if n > 0 and not (op.name == "insertion" and op.done):
+                op.model.add_sample(n, secs)
+        self.n_inserted = result.n_inserted_instances
+        ins, upd, emb = self._ops
+        ins.progress = (self.n_inserted / self.n_instances
+                        if self.n_instances > 0 else 0.0)
+        upd.progress = (1.0 - result.update_queue_size / self.n_inserted
+                        if self.n_inserted > 0 else 0.0)
+        pending = result.embedding_pending_edges
+        emb.progress = (1.0 - result.embedding_queue_size / pending
+                        if pending > 0 else 0.0)
 
     def on_operation_complete(self, operation: str) -> None:
         if operation not in OPERATIONS:
             raise ValueError(f"Unknown operation: {operation}")
-        i = OPERATIONS[operation]
-        self.completeness[i] = True
-        self._weights[i] = 0.0
-
-    def _track_progress(self, result) -> None:
-        self._n_inserted = result.n_inserted_instances
-        n_all, n_in = self._n_instances, self._n_inserted
-        self.insertion_progress = n_in / n_all if n_all > 0 else 0.0
-        self.update_progress = (1.0 - result.update_queue_size / n_in
-                                if n_in > 0 else 0.0)
-        pending = result.embedding_pending_edges
-        self.embedding_progress = (1.0 - result.embedding_queue_size / pending
-                                   if pending > 0 else 0.0)
-
-    def get_progress(self) -> tuple[float, float, float]:
-        return (self.insertion_progress, self.update_progress,
-                self.embedding_progress)
-
-    def _split(self, total: float) -> tuple[float, float, float]:
-        denom = sum(self._weights) * (1 + self._reserved)
-        if denom <= 0:
-            return (0.0, 0.0, 0.0)
-        a, b, c = (total * w / denom for w in self._weights)
-        return (a, b, c)
+        op = self._ops[OPERATIONS.index(operation)]
+        op.done = True
+        op.weight = 0.0

@@ -1,12 +1,14 @@
-"""Default strategy (reference python/gpumap/schedule/weighted.py:9-59):
-weights (1, 2, 1), 10 % headroom, insertion floored at ``min_insertion_ops``."""
+"""The default strategy: the budget shared 1 : 2 : 1 by insertion, update and
+embedding with 10 % held back, each share turned into ops by its cost model,
+and at least `min_insertion_ops` rows inserted per cycle (behaviour of the
+reference's WeightedStrategy, python/gpumap/schedule/weighted.py:9-59)."""
 
 from __future__ import annotations
 
 from typing import Optional
 
-from .estimator import Estimator, MeanRateEstimator
-from .strategy import BaseStrategy, _estimate_or_fallback, clip_ops
+from .estimator import Estimator
+from .strategy import UINT32_MAX, BaseStrategy, _estimate_or_fallback, clip_ops
 
 
 class WeightedStrategy(BaseStrategy):
@@ -18,22 +20,17 @@ class WeightedStrategy(BaseStrategy):
                  embedding_estimator: Optional[Estimator] = None,
                  default_n_ops: int = 100, confidence: float = 0.95,
                  min_insertion_ops: int = 500):
-        super().__init__(
-            weights=weights, reserved=reserved,
-            insertion_estimator=insertion_estimator or MeanRateEstimator(),
-            update_estimator=update_estimator or MeanRateEstimator(),
-            embedding_estimator=embedding_estimator or MeanRateEstimator(),
-            n_instances=n_instances, n_neighbors=n_neighbors,
-            default_n_ops=default_n_ops, confidence=confidence,
-            min_insertion_ops=min_insertion_ops)
+        super().__init__(weights, reserved, insertion_estimator,
+                         update_estimator, embedding_estimator, n_instances,
+                         n_neighbors, default_n_ops, confidence,
+                         min_insertion_ops)
 
     def allocate(self, target_latency: float) -> tuple[int, int, int]:
         shares = self._split(max(0.0, target_latency))
-        ests = (self.insertion_estimator, self.update_estimator,
-                self.embedding_estimator)
-        ins, upd, emb = (
-            _estimate_or_fallback(e, t, self.confidence, self.default_n_ops)
-            for e, t in zip(ests, shares))
-        return (clip_ops(ins, (self.min_insertion_ops, self.max_uint32)),
-                clip_ops(upd, (0, self.max_uint32)),
-                clip_ops(emb, (0, self.max_uint32)))
+        floors = (self.min_insertion_ops, 0, 0)
+        out = []
+        for op, share, floor in zip(self._ops, shares, floors):
+            n = _estimate_or_fallback(op.model, share, self.confidence,
+                                      self.default_n_ops)
+            out.append(clip_ops(n, (floor, UINT32_MAX)))
+        return tuple(out)

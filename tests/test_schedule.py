@@ -102,7 +102,7 @@ def test_weighted_strategy_split_and_allocate():
     assert upd == 250  # 0.5 / 0.002
     assert emb == 500  # 0.25 / 0.0005
     s.on_operation_complete("insertion")
-    assert s._weights[0] == 0.0 and s.completeness[0]
+    assert s.weights[0] == 0.0 and s.completeness[0]
     assert s._split(0.33) == pytest.approx((0.0, 0.2, 0.1))  # 0.33 * w / 3.3
     with pytest.raises(ValueError):
         s.on_operation_complete("nope")
@@ -113,10 +113,11 @@ def test_strategy_update_and_progress():
     s.update(res(ni=500, ti=0.5, nu=100, tu=0.1, ne=200, te=0.2, inserted=500,
                  uq=50, pending=500, eq=500))
     assert s.get_progress() == pytest.approx((0.5, 0.9, 0.0))
-    assert len(s.insertion_estimator._samples) == 1
+    assert s.insertion_estimator.n_samples == 1
     s.on_operation_complete("insertion")
     s.update(res(ni=10, ti=0.01, inserted=1000, pending=1000, eq=1000))
-    assert len(s.insertion_estimator._samples) == 1  # frozen once complete
+    assert s.insertion_estimator.n_samples == 1  # frozen once complete
+    assert s.update_estimator.n_samples == 1
 
 
 def test_sequential_strategy_cascade():
@@ -126,6 +127,8 @@ def test_sequential_strategy_cascade():
     s.on_operation_complete("insertion")
     assert s.allocate(1.0) == (0, m, 0)
     s.on_operation_complete("update")
+    assert s.allocate(1.0) == (0, 0, m)
+    s.on_operation_complete("embedding")
     assert s.allocate(1.0) == (0, 0, m)
 
 
