@@ -105,6 +105,8 @@ def lib():
         L.gm_set_profiling.argtypes = [vp, i32]
         L.gm_get_kernel_stats.argtypes = [vp, i32, P(GmKernelStats)]
         L.gm_reset_kernel_stats.argtypes = [vp]
+        L.gm_sgd_noalloc.argtypes = [vp, u64, i32, vp, vp, vp, i64, i32, i32,
+                                     f32, f32, f32, f32, u64, i32]
         L.gm_release.argtypes = [vp]
         L.gm_is_released.argtypes = [vp, P(i32)]
         L.gm_shutdown.argtypes = []
@@ -146,6 +148,39 @@ def sgd(emb, head, tail, w, n_epochs, nsr, a, b, alpha0=1.0, gamma=1.0, seed=42,
                              n_epochs, nsr, a, b, alpha0, gamma, seed,
                              2 if stream else 1 if per_epoch else 0))
     return emb
+
+
+def sgd_noalloc_device(emb, head, tail, w, n_epochs, nsr, a, b, alpha0=1.0,
+                       gamma=1.0, seed=42, per_epoch=False, stream=False):
+    """gm_sgd_noalloc on torch device tensors (emb float32 (n, c), updated in
+    place; head, tail int32 and w float32 of one length)."""
+    import torch
+
+    for t in (emb, head, tail, w):
+        if not (t.is_cuda and t.is_contiguous()):
+            raise ValueError("device tensors, contiguous, expected")
+    assert emb.dtype == torch.float32 and w.dtype == torch.float32
+    assert head.dtype == torch.int32 and tail.dtype == torch.int32
+    torch.cuda.current_stream().synchronize()
+    check(lib().gm_sgd_noalloc(emb.data_ptr(), emb.shape[0], emb.shape[1],
+                               head.data_ptr(), tail.data_ptr(), w.data_ptr(),
+                               w.numel(), n_epochs, nsr, a, b, alpha0, gamma,
+                               seed, 2 if stream else 1 if per_epoch else 0))
+
+
+def sgd_noalloc(emb, head, tail, w, n_epochs, nsr, a, b, alpha0=1.0, gamma=1.0,
+                seed=42, per_epoch=False, stream=False):
+    """sgd_noalloc_device on host arrays (copied to and from the device)."""
+    import torch
+
+    dev = torch.device("cuda")
+    e = torch.from_numpy(np.ascontiguousarray(emb, np.float32).copy()).to(dev)
+    h = torch.from_numpy(np.ascontiguousarray(head, np.int32)).to(dev)
+    t = torch.from_numpy(np.ascontiguousarray(tail, np.int32)).to(dev)
+    ww = torch.from_numpy(np.ascontiguousarray(w, np.float32)).to(dev)
+    sgd_noalloc_device(e, h, t, ww, n_epochs, nsr, a, b, alpha0, gamma, seed,
+                       per_epoch, stream)
+    return e.cpu().numpy()
 
 
 class Engine:

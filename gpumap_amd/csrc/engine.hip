@@ -160,6 +160,88 @@ public:
 };
 DevLock g_dev_mu;
 
+ExclusiveDevice::ExclusiveDevice() {
+  g_dev_mu.lock();
+  g_coop_mu.lock();
+}
+ExclusiveDevice::~ExclusiveDevice() {
+  g_coop_mu.unlock();
+  g_dev_mu.unlock();
+}
+
+void set_last_error(const char *msg) {
+  if (msg && *msg)
+    g_err = msg;
+  else
+    g_err.clear();
+}
+
+// The SGD's launches on a prepared schedule (eps, nxt, nneg, the rounding
+// grid and zeroed barrier words, sgd_setup's work): all epochs in one
+// cooperative launch of the kernel that fits the edge set (SGD_AUTO), the
+// streaming cooperative kernel (SGD_STREAM), or one launch per epoch
+// (SGD_PER_EPOCH, and whenever no cooperative launch is possible).  Returns
+// true when a cooperative kernel ran (its timeout flag is gbar[2]); *variant
+// names it (2 per-wave slots, 0 resident counter grid, 1 streaming).
+// Epoch lag of the cooperative SGD: a block starts epoch e + 1 once every
+// block has finished epoch e - SGD_LAG (0 would be a full grid barrier per
+// epoch, the reference's one launch per epoch).  The Hogwild updates are
+// unordered within an epoch already; the lag lets the boundary's counter read
+// overlap the next epoch's work.
+bool sgd_launches(int c, float *e, const int32_t *head, const int32_t *tail,
+                  const uint32_t *nnz_p, uint32_t nrows, const float *eps,
+                  float *nxt, float *nneg, int n_epochs, float alpha0,
+                  float gamma, float a, float b, float nsr_inv, uint64_t seed,
+                  const float *rounding, unsigned *gbar,
+                  unsigned long long *st, uint64_t cap, int num_cu,
+                  hipStream_t s, int sgd_mode, int *variant, bool verbose) {
+  if (sgd_mode != SGD_PER_EPOCH) {
+    const hipError_t fe = launch_sgd_fused(
+        c, e, head, tail, nnz_p, nrows, eps, nxt, nneg, n_epochs, alpha0,
+        gamma, a, b, nsr_inv, seed, rounding, gbar, st, SGD_LAG, cap, num_cu,
+        s, variant, sgd_mode == SGD_STREAM);
+    if (fe == hipSuccess)
+      return true;
+    // too many edges for the resident grid, or cooperative launch not
+    // available in this context: one launch per epoch instead
+    (void)hipGetLastError();
+    if (fe != hipErrorInvalidConfiguration && verbose) {
+      fprintf(stderr, "[gpumap_amd] cooperative SGD launch unavailable; one "
+                      "launch per epoch\n");
+      fflush(stderr);
+    }
+  }
+  const unsigned grid = std::min<unsigned>(blocks_for(cap, 256), 16384);
+  float alpha = alpha0;
+  for (int epoch = 0; epoch < n_epochs; ++epoch) {
+    const uint64_t es = seed + (uint64_t)epoch;
+    switch (c) {
+    case 1:
+      k_sgd_epoch<1><<<grid, 256, 0, s>>>(e, head, tail, nnz_p, nrows, eps,
+                                          nxt, nneg, alpha, epoch, gamma, a, b,
+                                          nsr_inv, es, rounding, st);
+      break;
+    case 2:
+      k_sgd_epoch<2><<<grid, 256, 0, s>>>(e, head, tail, nnz_p, nrows, eps,
+                                          nxt, nneg, alpha, epoch, gamma, a, b,
+                                          nsr_inv, es, rounding, st);
+      break;
+    case 3:
+      k_sgd_epoch<3><<<grid, 256, 0, s>>>(e, head, tail, nnz_p, nrows, eps,
+                                          nxt, nneg, alpha, epoch, gamma, a, b,
+                                          nsr_inv, es, rounding, st);
+      break;
+    default:
+      k_sgd_epoch<4><<<grid, 256, 0, s>>>(e, head, tail, nnz_p, nrows, eps,
+                                          nxt, nneg, alpha, epoch, gamma, a, b,
+                                          nsr_inv, es, rounding, st);
+    }
+    LAUNCH_CHECK();
+    alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
+  }
+  return false;
+}
+
 struct Engine;
 // Every live session of the process, for gm_shutdown.
 std::mutex g_live_mu;
@@ -979,13 +1061,6 @@ struct Engine {
     LAUNCH_CHECK();
   }
 
-  // Epoch lag of the cooperative SGD: a block starts epoch e + 1 once every
-  // block has finished epoch e - SGD_LAG (0 would be a full grid barrier per
-  // epoch, the reference's one launch per epoch).  The Hogwild updates are
-  // unordered within an epoch already; the lag lets the boundary's counter
-  // read overlap the next epoch's work.
-  static constexpr int SGD_LAG = 1;
-
   // SGD on the COO in head/tail/wout with nnz in scal[S_NNZ] and the
   // maximum weight in scal[S_MAXW] (reference run_optimize_layout,
   // embedding_kernel.cuh:726-798).  `cap` bounds nnz for the launch grids.
@@ -1009,7 +1084,6 @@ struct Engine {
 
   // sgd_mode: SGD_AUTO (the kernel that fits), SGD_PER_EPOCH (one launch per
   // epoch) or SGD_STREAM (the streaming cooperative kernel at any size)
-  static constexpr int SGD_AUTO = 0, SGD_PER_EPOCH = 1, SGD_STREAM = 2;
   void run_sgd(float *e, uint64_t nrows, uint64_t cap, int n_epochs, int nsr,
                float a, float b, float alpha0, float gamma, uint64_t seed,
                int sgd_mode = 0, bool deg_zeroed = false) {
@@ -1017,60 +1091,18 @@ struct Engine {
       return;
     const float nsr_inv = 1.0f / (float)nsr;
     sgd_setup(nrows, cap, n_epochs, nsr_inv, alpha0, deg_zeroed);
-    const unsigned grid = std::min<unsigned>(blocks_for(cap, 256), 16384);
     unsigned long long *st = prof ? kstats : nullptr;
     ev_start(GM_KSTAT_SGD);
-    bool fused = false;
-    if (sgd_mode != SGD_PER_EPOCH) {
-      const hipError_t fe = launch_sgd_fused(
-          c, e, head, tail, scal + S_NNZ, (uint32_t)nrows, eps, nxt, nneg,
-          n_epochs, alpha0, gamma, a, b, nsr_inv, seed, rounding, gbar, st,
-          SGD_LAG, cap, num_cu, s, &sgd_variant, sgd_mode == SGD_STREAM);
-      if (fe == hipSuccess) {
-        fused = true;
-        fused_used = true;
-        if (sgd_variant == 2)
-          ++sgd_waves_launches;
-        if (prof && sgd_variant == 1)
-          pending.back().which = GM_KSTAT_SGD_STREAM;
-      } else {
-        // too many edges for the resident grid, or cooperative launch not
-        // available in this context: one launch per epoch instead
-        (void)hipGetLastError();
-        if (fe != hipErrorInvalidConfiguration)
-          log("cooperative SGD launch unavailable; one launch per epoch");
-      }
-    }
-    float alpha = alpha0;
-    for (int epoch = 0; epoch < n_epochs && !fused; ++epoch) {
-      const uint64_t es = seed + (uint64_t)epoch;
-      switch (c) {
-      case 1:
-        k_sgd_epoch<1><<<grid, 256, 0, s>>>(e, head, tail, scal + S_NNZ,
-                                            (uint32_t)nrows, eps, nxt, nneg,
-                                            alpha, epoch, gamma, a, b, nsr_inv,
-                                            es, rounding, st);
-        break;
-      case 2:
-        k_sgd_epoch<2><<<grid, 256, 0, s>>>(e, head, tail, scal + S_NNZ,
-                                            (uint32_t)nrows, eps, nxt, nneg,
-                                            alpha, epoch, gamma, a, b, nsr_inv,
-                                            es, rounding, st);
-        break;
-      case 3:
-        k_sgd_epoch<3><<<grid, 256, 0, s>>>(e, head, tail, scal + S_NNZ,
-                                            (uint32_t)nrows, eps, nxt, nneg,
-                                            alpha, epoch, gamma, a, b, nsr_inv,
-                                            es, rounding, st);
-        break;
-      default:
-        k_sgd_epoch<4><<<grid, 256, 0, s>>>(e, head, tail, scal + S_NNZ,
-                                            (uint32_t)nrows, eps, nxt, nneg,
-                                            alpha, epoch, gamma, a, b, nsr_inv,
-                                            es, rounding, st);
-      }
-      LAUNCH_CHECK();
-      alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
+    const bool fused = sgd_launches(
+        c, e, head, tail, scal + S_NNZ, (uint32_t)nrows, eps, nxt, nneg,
+        n_epochs, alpha0, gamma, a, b, nsr_inv, seed, rounding, gbar, st, cap,
+        num_cu, s, sgd_mode, &sgd_variant, p.verbose != 0);
+    if (fused) {
+      fused_used = true;
+      if (sgd_variant == 2)
+        ++sgd_waves_launches;
+      if (prof && sgd_variant == 1)
+        pending.back().which = GM_KSTAT_SGD_STREAM;
     }
     ev_stop(fused ? 1 : (uint64_t)n_epochs);
     if (prof) {
@@ -1434,7 +1466,7 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
         tm.push_back(x);
       }
     const hipError_t err = launch_sgd_batch(c, b, waves, bar,
-                                            2 * Engine::SGD_LAG, bc.num_cu,
+                                            2 * SGD_LAG, bc.num_cu,
                                             bc.s);
     if (err != hipSuccess) {
       (void)hipGetLastError();
@@ -2064,9 +2096,9 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
     if (mw > 0.0f && nnz > 0)
       e->run_sgd(e->emb, n, (uint64_t)nnz, n_epochs, nsr, a, b, alpha0, gamma,
                  seed,
-                 per_epoch == 1   ? gm::Engine::SGD_PER_EPOCH
-                 : per_epoch == 2 ? gm::Engine::SGD_STREAM
-                                  : gm::Engine::SGD_AUTO);
+                 per_epoch == 1   ? gm::SGD_PER_EPOCH
+                 : per_epoch == 2 ? gm::SGD_STREAM
+                                  : gm::SGD_AUTO);
     HIPC(hipMemcpyAsync(emb, e->emb, n * c * 4, hipMemcpyDeviceToHost, e->s));
     HIPC(hipStreamSynchronize(e->s));
     e->check_barrier();

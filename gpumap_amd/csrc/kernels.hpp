@@ -190,6 +190,28 @@ hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
                             unsigned *bar, unsigned long long *stats, int lag,
                             uint64_t nnz, int num_cu, hipStream_t s,
                             int *variant, bool force_stream = false);
+// engine.hip: holds the process's device lock exclusively (no session's
+// device work is in flight meanwhile) and the cooperative-launch mutex
+struct ExclusiveDevice {
+  ExclusiveDevice();
+  ~ExclusiveDevice();
+  ExclusiveDevice(const ExclusiveDevice &) = delete;
+  ExclusiveDevice &operator=(const ExclusiveDevice &) = delete;
+};
+// engine.hip: the message gm_last_error() returns on this thread
+void set_last_error(const char *msg);
+
+// engine.hip: the SGD's launches on a prepared schedule (engine and the
+// allocation-free entry point share them)
+constexpr int SGD_AUTO = 0, SGD_PER_EPOCH = 1, SGD_STREAM = 2;
+constexpr int SGD_LAG = 1;
+bool sgd_launches(int c, float *e, const int32_t *head, const int32_t *tail,
+                  const uint32_t *nnz_p, uint32_t nrows, const float *eps,
+                  float *nxt, float *nneg, int n_epochs, float alpha0,
+                  float gamma, float a, float b, float nsr_inv, uint64_t seed,
+                  const float *rounding, unsigned *gbar,
+                  unsigned long long *st, uint64_t cap, int num_cu,
+                  hipStream_t s, int sgd_mode, int *variant, bool verbose);
 __global__ void k_reposition(const uint32_t *pidx, const float *pdist,
                              uint64_t n, int k, int euclid, const double *sigma,
                              const double *rho, const double *mean,

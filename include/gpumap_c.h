@@ -215,6 +215,28 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
                  int32_t n_epochs, int32_t nsr, float a, float b, float alpha0,
                  float gamma, uint64_t seed, int32_t per_epoch);
 
+/* gm_debug_sgd without any device allocation: the same SGD with the same
+ * kernels and launch shapes, on the caller's device arrays (emb n x c float32,
+ * updated in place; head, tail int32 and w float32 of nnz COO slots, all in
+ * device memory), with the schedule, head degrees, rounding grid and the
+ * cooperative kernels' barrier words in one static device array per device
+ * (about 16 MiB, held until the process ends).  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Refuses (GM_ERR_INVALID) n above
+ * GM_SGD_NOALLOC_MAX_ROWS, nnz above GM_SGD_NOALLOC_MAX_EDGES and COO
+ * indices out of range (checked on the device).  Blocking; calls in one
+ * process are serialised with the sessions' device work by the engine's
+ * device lock, so they do not race on the shared array, but each call
+ * overwrites what the previous one left in it. */
+#define GM_SGD_NOALLOC_MAX_ROWS (1u << 20)
+#define GM_SGD_NOALLOC_MAX_EDGES (1u << 20)
+int gm_sgd_noalloc(float *emb, uint64_t n, int32_t c, const int32_t *head,
+                   const int32_t *tail, const float *w, int64_t nnz,
+                   int32_t n_epochs, int32_t nsr, float a, float b,
+                   float alpha0, float gamma, uint64_t seed,
+                   int32_t per_epoch);
+
 /* Per-kernel measurement (no counterpart in the reference, which only times
  * whole operations, impl.cuh:159-203).  With profiling on, HIP events on the
  * session's stream bracket the timed kernels and device counters record their
