@@ -7,6 +7,8 @@
 // materialised: every kernel reads it through embed_entry() from the public
 // (n x k) arrays.
 #include "common.hpp"
+
+#include <type_traits>
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -1080,16 +1082,38 @@ __device__ inline unsigned slot_min(const unsigned sv[8], unsigned nw,
 }
 
 // All epochs in one cooperative launch for edge sets of at most 512 waves
-// (one edge per thread, one wave per block): the per-cycle embedding batches.
-// Epoch boundary without atomics or block barriers: wave w stores epoch + 1
-// into its arrival slot slots[w] when it finishes an epoch, and may start
-// epoch e + 1 once every slot of its edge set is >= e - lag + 1.  Each lane
-// reads up to 8 slots with the epoch's gathers and the wave reduces them at
-// the boundary, so a wave that is not more than `lag` epochs ahead waits for
-// nothing.  The wait is bounded (*timeout is set, as epoch_end's bar[2]).
-// sgd_waves_body is one wave's work; k_sgd_waves runs one edge set,
-// k_sgd_waves_multi several (one session each) side by side, each paced only
-// by its own waves.
+// (one edge per thread, one wave per block): the per-cycle embedding
+// batches.  With a few hundred waves on 1024 SIMDs an epoch costs about one
+// wave's instruction stream plus what its memory operations cannot hide.
+//
+// Per epoch a wave issues, in this order: the next epoch's gathers (two
+// epochs in flight, so that the loads overlap the arithmetic), then the
+// previous epoch's row updates (device-scope float atomics, whose cost is
+// their device-wide throughput rather than their place in the stream), then
+// the epoch's arithmetic.  The buffers of the two epochs in flight swap
+// roles from one epoch to the next (the loop is unrolled), so that no
+// register copy waits for loads in flight.
+//
+// The positions an epoch reads were gathered before the previous two
+// epochs' updates of this wave were issued: each lane adds its own pair's
+// share of those two epochs' updates (what its edge and its twin added to
+// the edge's two rows), so that its edge steps from where its own previous
+// steps left it.  Updates of the same rows by other lanes are up to two
+// epochs old when read, a Hogwild read.  On a conflict-free edge set all of
+// this is deterministic (the wave's own issue order decides it).
+//
+// Epoch boundary: wave w stores the count of epochs it has finished into
+// its arrival slot every SGD_POLL epochs, and every SGD_POLL epochs reads
+// all slots (coherent stores and reads on a few shared cache lines slow the
+// whole device when every wave makes them every epoch).  Epoch e + 1 starts
+// once the slots last read show e - lag + 1, lag being the counter kernels'
+// lag plus SGD_POLL; a boundary that the last read does not clear reads the
+// slots again until it does.  A stored count trails its wave by at most
+// SGD_POLL - 1 epochs, so the slowest wave never waits and every wave is at
+// most lag + 2 (SGD_POLL - 1) epochs ahead of it.  The wait is bounded
+// (*timeout is set, as epoch_end's bar[2]).
+constexpr int SGD_POLL = 4;
+
 template <int NC>
 __device__ __attribute__((always_inline)) inline void sgd_waves_body(
     float *__restrict__ emb, const int32_t *__restrict__ head,
@@ -1101,6 +1125,7 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
     int lag) {
   const int lane = threadIdx.x & 63;
   const uint64_t e = (uint64_t)w * 64 + lane;
+  lag += SGD_POLL;
   int32_t hh, tt;
   float ep, nx, nn;
   if (e < nnz && head[e] >= 0) {
@@ -1120,107 +1145,138 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
   const float ep_neg = ep * nsr_inv;
   unsigned long long c_pos = 0, c_neg = 0;
   float alpha = alpha0;
-  // Two epochs in flight: epoch e + 1's gathers are issued before epoch e's
-  // arithmetic, so their memory latency overlaps it.  The schedule does not
-  // depend on the positions, so it is advanced when the gathers are issued;
-  // the positions that epoch e + 1 reads miss the updates of epoch e (the
-  // wave's own and others'), a Hogwild read as the lag already allows for
-  // other waves.  Per epoch, every lane issues, in this order: the next
-  // epoch's 10 gathers, its 8 slot reads, its row updates (2 x NC, dropped
-  // where there is nothing to add) and its arrival, so that every wait
-  // counts exactly the operations issued after what it waits for.
   uint32_t dr[8];
   neg_draws(seed, e, dr);
   EdgeLoads<NC> L0, L1;
-  // What this lane's edge and its twin added to the edge's two rows in the
-  // previous epoch: epoch e + 1's positions are read before epoch e's
-  // updates are issued, so the lane adds its own pair's share of them to
-  // what it reads (updates of the same rows by other lanes stay one epoch
-  // old, a Hogwild read).  Without it an edge sampled every epoch would
-  // take each step from the position before its previous one and overshoot.
-  float own_h[NC], own_t[NC];
+  // the own pair's share of the last two epochs' updates (1: the previous
+  // epoch, 2: the one before), and the previous epoch's contributions, whose
+  // row updates are issued in this epoch
+  float own1_h[NC], own1_t[NC], own2_h[NC], own2_t[NC], pgh[NC], pgt[NC];
 #pragma unroll
   for (int d = 0; d < NC; ++d)
-    own_h[d] = own_t[d] = 0.0f;
-  bool act0 = !(nx > 0.0f);
+    own1_h[d] = own1_t[d] = own2_h[d] = own2_t[d] = pgh[d] = pgt[d] = 0.0f;
+  bool act0 = !(nx > 0.0f), act1 = false;
   sgd_gather_all<NC>(emb, act0, hh, tt, n, ep_neg, nn, 0, dr, L0);
   if (act0)
     advance_schedule(ep, ep_neg, L0.n_neg, nx, nn);
   neg_draws(seed + 1, e, dr);
-  unsigned sv[8];
-  poll_slots(slots, nw, lane, sv);
-  {
-    float z[NC];
-#pragma unroll
-    for (int d = 0; d < NC; ++d)
-      z[d] = 0.0f;
-    buf_row_add<NC>(er, -1, false, z);
-    buf_row_add<NC>(er, -1, false, z);
-    __hip_atomic_store(slots + w, 0u, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  }
-  for (int epoch = 0; epoch < n_epochs; ++epoch) {
+  // what a step leaves in flight after its gathers, so that the loop is
+  // entered with the same pending operations as it repeats with: the row
+  // updates (zero here, dropped) and the arrival
+  agg_add_buf<NC>(er, ag, hh, tt, pgh, pgt);
+  __hip_atomic_store(slots + w, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned smin = 0; // smallest slot at the last read taken in
+  unsigned sv[8];    // a read of the slots in flight
+  unsigned spins = 0;
+  bool gave_up = false;
+  // One epoch: Lc holds its positions, Ln receives the next epoch's.  A
+  // `read` step issues a read of the slots, which the next step (`take`)
+  // takes in and where the wave stores its arrival: the read was issued
+  // after that step's row updates, and in order, so waiting for it at once
+  // would wait for them.  The loop runs SGD_POLL epochs per pass with the
+  // buffers swapping roles, so that no register copy between them waits for
+  // loads in flight.
+  auto step = [&](int epoch, EdgeLoads<NC> &Lc, bool &ac, EdgeLoads<NC> &Ln,
+                  bool &an, auto read, auto take) {
     const uint64_t es = seed + (uint64_t)epoch;
     // the next epoch's gathers (the last epoch issues them for nothing:
     // every lane reads row 0)
-    const bool act1 = epoch + 1 < n_epochs && !(nx > (float)(epoch + 1));
-    sgd_gather_all<NC>(emb, act1, hh, tt, n, ep_neg, nn, epoch + 1, dr, L1);
-    if (act1)
-      advance_schedule(ep, ep_neg, L1.n_neg, nx, nn);
+    an = epoch + 1 < n_epochs && !(nx > (float)(epoch + 1));
+    sgd_gather_all<NC>(emb, an, hh, tt, n, ep_neg, nn, epoch + 1, dr, Ln);
+    if (an)
+      advance_schedule(ep, ep_neg, Ln.n_neg, nx, nn);
+    // the previous epoch's row updates (zero contributions at epoch 0)
+    agg_add_buf<NC>(er, ag, hh, tt, pgh, pgt);
+    if constexpr (decltype(read)::value)
+      poll_slots(slots, nw, lane, sv);
     neg_draws(es + 2, e, dr);
     float gh[NC], gt[NC];
 #pragma unroll
     for (int d = 0; d < NC; ++d) {
       gh[d] = gt[d] = 0.0f;
-      L0.cur[d] += own_h[d];
-      L0.oth[d] += own_t[d];
+      Lc.cur[d] += own1_h[d] + own2_h[d];
+      Lc.oth[d] += own1_t[d] + own2_t[d];
     }
-    if (act0)
+    // The epoch's gathers have landed, on every path (also when no lane
+    // samples, where the compiler would otherwise consume them only inside
+    // the branch below): a load left pending here would make the next write
+    // to its registers wait for every memory operation in flight.
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+#pragma unroll
+      for (int d = 0; d < NC; ++d)
+        __asm__ volatile("" ::"v"(Lc.neg[q][d]));
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      __asm__ volatile("" ::"v"(Lc.cur[d]), "v"(Lc.oth[d]));
+    if (ac)
       sgd_apply<NC>(emb, hh, e, n, ep_neg, alpha, gamma, a, b, es, rounding,
-                    L0, gh, gt, c_pos, c_neg);
-    // this epoch's share of the pair's rows, for the positions of the next
-    // (paired: the head row gets the edge's head and the twin's tail
-    // contributions, the tail row the twin's total)
+                    Lc, gh, gt, c_pos, c_neg);
+    // this epoch's share of the pair's rows (paired: the head row gets the
+    // edge's head and the twin's tail contributions, the tail row the
+    // twin's total)
 #pragma unroll
     for (int d = 0; d < NC; ++d) {
       const float twin_t = __shfl_xor(gt[d], 1, 64);
       const float pair_h = gh[d] + twin_t;
-      own_h[d] = ag.paired ? pair_h : gh[d];
-      own_t[d] = ag.paired ? __shfl_xor(pair_h, 1, 64) : gt[d];
+      own2_h[d] = own1_h[d];
+      own2_t[d] = own1_t[d];
+      own1_h[d] = ag.paired ? pair_h : gh[d];
+      own1_t[d] = ag.paired ? __shfl_xor(pair_h, 1, 64) : gt[d];
+      pgh[d] = gh[d];
+      pgt[d] = gt[d];
     }
     alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
-    // boundary: epoch + 1 starts once every slot shows epoch - lag + 1 (the
-    // first check is outside the polling loop so that its wait is only for
-    // the slot reads)
+    // boundary: epoch + 1 starts once every slot shows epoch - lag + 1
+    if constexpr (decltype(take)::value)
+      smin = slot_min(sv, nw, lane);
     if (epoch >= lag && epoch + 1 < n_epochs) {
       const unsigned target = (unsigned)(epoch - lag + 1);
-      if (slot_min(sv, nw, lane) < target) {
-        unsigned spins = 0;
-        do {
-          // bounded: a timed-out wait anywhere in the launch ends the
-          // others within 1024 polls
-          if ((++spins & 1023u) == 0 &&
-              (spins > (1u << 20) ||
-               __builtin_amdgcn_readfirstlane(__hip_atomic_load(
-                   timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))) {
-            if (lane == 0)
-              __hip_atomic_store(timeout, 1u, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-          poll_slots(slots, nw, lane, sv);
-        } while (slot_min(sv, nw, lane) < target);
+      while (!gave_up && smin < target) {
+        // bounded: a timed-out wait anywhere in the launch ends the others
+        // within 1024 polls
+        if ((++spins & 1023u) == 0 &&
+            (spins > (1u << 20) ||
+             __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                 timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))) {
+          if (lane == 0)
+            __hip_atomic_store(timeout, 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+          gave_up = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        unsigned sw[8];
+        poll_slots(slots, nw, lane, sw);
+        smin = slot_min(sw, nw, lane);
       }
     }
-    poll_slots(slots, nw, lane, sv);
-    agg_add_buf<NC>(er, ag, hh, tt, gh, gt);
-    // arrival (every lane stores the same value: no branch)
-    __hip_atomic_store(slots + w, (unsigned)epoch + 1u, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    L0 = L1;
-    act0 = act1;
+    // arrival, every SGD_POLL epochs (every lane stores the same value)
+    if constexpr (decltype(take)::value)
+      __hip_atomic_store(slots + w, (unsigned)epoch + 1u, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  };
+  using P1 = std::integral_constant<bool, true>;
+  using P0 = std::integral_constant<bool, false>;
+  static_assert(SGD_POLL == 4, "the loop below runs four epochs per pass");
+  int epoch = 0;
+  for (; epoch + 3 < n_epochs; epoch += 4) {
+    step(epoch, L0, act0, L1, act1, P1{}, P0{});
+    step(epoch + 1, L1, act1, L0, act0, P0{}, P1{});
+    step(epoch + 2, L0, act0, L1, act1, P0{}, P0{});
+    step(epoch + 3, L1, act1, L0, act0, P0{}, P0{});
   }
+  // the last one to three epochs
+  if (epoch < n_epochs)
+    step(epoch, L0, act0, L1, act1, P1{}, P0{});
+  if (epoch + 1 < n_epochs)
+    step(epoch + 1, L1, act1, L0, act0, P0{}, P1{});
+  if (epoch + 2 < n_epochs)
+    step(epoch + 2, L0, act0, L1, act1, P0{}, P0{});
+  // the last epoch's row updates and arrival
+  agg_add_buf<NC>(er, ag, hh, tt, pgh, pgt);
+  __hip_atomic_store(slots + w, (unsigned)n_epochs, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
   flush_sgd_stats(stats, c_pos, c_neg);
 }
 
