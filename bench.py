@@ -405,27 +405,59 @@ def serving_mix(ev, n_sessions=64, rate=4.0, budget=0.5, seed=1003,
     }
 
 
+def dist_error_vs_fp64(x, idx, dist, q, metric):
+    """Published neighbour distances of the rows q against the same pairs'
+    squared distances in FP64 (cosine: of the FP64-normalised rows):
+    relative error max and mean."""
+    xd = np.asarray(x, dtype=np.float64)
+    if metric == 1:
+        xd = xd / np.maximum(np.linalg.norm(xd, axis=1, keepdims=True), 1e-300)
+    a = np.asarray(idx)[q].astype(np.int64)
+    ok = (a >= 0) & (a < xd.shape[0])
+    a = np.where(ok, a, 0)
+    diff = xd[a] - xd[q][:, None, :]
+    d64 = (diff * diff).sum(-1)
+    d32 = np.asarray(dist, dtype=np.float64)[q]
+    rel = np.abs(d32 - d64) / np.maximum(d64, 1e-30)
+    rel = rel[ok & (d64 > 0)]
+    return {"pairs": int(rel.size), "max_rel": float(rel.max()),
+            "mean_rel": float(rel.mean())}
+
+
 def quality(ev, x, e, n):
     k, _, metric, _ = CFG
     mname = "cosine" if metric == 1 else "euclidean"
-    idx, _ = e.knn()
+    idx, dist = e.knn()
     q = np.random.Generator(np.random.PCG64(7)).choice(n, 2000, replace=False)
     ei, ed = ev.exact_knn(x, k, queries=q, metric=mname, device="cuda:0")
     return {f"recall_at_{k}": ev.recall_at_k(idx, x, ei, ed, queries=q,
                                              metric=mname),
+            "distance_error_vs_fp64": dist_error_vs_fp64(x, idx, dist, q,
+                                                         metric),
             "trustworthiness_at_15": ev.trustworthiness(
                 x, e.embedding(), sample=5000, metric=mname)}
 
 
-def cpu_baseline(x, budget_rows, knn_mode=1):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_session(xs, threads, knn_mode):
+    """Wall time of the oracle's run() calls over one session of the op plan
+    on the rows `xs`."""
     import oracle
 
-    xs = np.ascontiguousarray(x[:budget_rows])
     n, d = xs.shape
-    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     k, c, metric, _ = CFG
     o = oracle.Oracle(n, d, n_neighbors=k, n_components=c, metric=metric,
-                      n_threads=threads, knn_mode=min(knn_mode, 1))
+                      n_threads=threads, knn_mode=knn_mode)
     t_run, pos = 0.0, 0
     while True:
         ins = min(CHUNK, n - pos)
@@ -442,11 +474,122 @@ def cpu_baseline(x, budget_rows, knn_mode=1):
                 o.run(0, 0, n)
                 t_run += time.perf_counter() - t0
             break
-    return {"value": n / t_run, "unit": "points/s", "cores": threads,
+    return t_run
+
+
+def cpu_baseline(x, budget_rows, knn_mode=1):
+    """The oracle (OpenMP C++ restatement, exact FP32 insertion as the
+    headline path) on two prefixes of the workload, budget_rows / 2 and
+    budget_rows.  An exact insertion's cost per row grows with the rows
+    already inserted, so the two session times also fix t(n) = a n + b n^2,
+    whose value at the full row count is reported beside the measured
+    rate."""
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover - Linux has it
+        nproc = os.cpu_count() or 1
+    kmode = min(knn_mode, 1)
+    sizes = (budget_rows // 2, budget_rows)
+    times = [_oracle_session(np.ascontiguousarray(x[:m]), threads, kmode)
+             for m in sizes]
+    n_full = x.shape[0]
+    (n1, n2), (t1, t2) = sizes, times
+    # t = a n + b n^2 through both points
+    b = (t2 / n2 - t1 / n1) / (n2 - n1)
+    a = t1 / n1 - b * n1
+    t_full = a * n_full + b * n_full * n_full
+    return {"value": n2 / t2, "unit": "points/s", "cores": threads,
             "kind": "port",
-            "sample": f"first {n} of the workload's rows, same op plan and "
+            "cpu_model": _cpu_model(), "nproc": nproc,
+            "sample": f"first {n2} of the workload's rows, same op plan and "
                       f"completion rule, exact FP32 insertion, OpenMP C++ "
-                      f"restatement on {threads} threads ({t_run:.1f} s)"}
+                      f"restatement on {threads} threads ({t2:.1f} s; the "
+                      f"first {n1} rows took {t1:.1f} s)",
+            "fitted_full_config": {
+                "rows": n_full, "seconds": t_full,
+                "points_per_s": n_full / t_full if t_full > 0 else None,
+                "model": "t(n) = a n + b n^2 through the two samples"}}
+
+
+def hbm_copy_rate(mib=2048, reps=5):
+    """Streaming-copy bandwidth of this GPU in this run: torch's device copy
+    of `mib` MiB (read + write counted), the best of `reps`."""
+    import torch
+
+    src = torch.empty(mib << 18, dtype=torch.float32, device="cuda:0")
+    src.fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    best = 0.0
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        dst.copy_(src)
+        b.record()
+        b.synchronize()
+        best = max(best, 2.0 * src.numel() * 4 / (a.elapsed_time(b) * 1e-3))
+    del src, dst
+    torch.cuda.empty_cache()
+    return best
+
+
+def strategy_leg(ev, x, xdev, n, d, knn_mode, budget=0.5, max_cycles=2000):
+    """The workload under the facade's default WeightedStrategy: run(budget)
+    until insertion, update and embedding have completed.  Reports the
+    points/s, every run()'s latency against the budget and the on-chip
+    per-op cost model, the seconds per op of each operation measured per
+    cycle, tabulated over the inserted row count."""
+    import gpumap_amd as gm
+    from gpumap_amd.data_loader import ArrayDataLoader
+
+    k, c, metric, _ = CFG
+    p = gm.GPUMAPParams()
+    p.n_instances, p.n_features, p.n_neighbors = n, d, k
+    p.n_components = c
+    p.metric = "cosine" if metric == 1 else "euclidean"
+    p.knn_mode = knn_mode
+    m = gm.GPUMAP(ArrayDataLoader(xdev), p)
+    lat, rows = [], []
+    for _ in range(max_cycles):
+        t0 = time.perf_counter()
+        r = m.run(budget)
+        lat.append(time.perf_counter() - t0)
+        rows.append((r.n_inserted_instances, r.n_insertion_ops,
+                     r.insertion_time, r.n_update_ops, r.update_time,
+                     r.n_embedding_ops, r.embedding_time))
+        if r.insertion_completed and r.update_completed and \
+                r.embedding_completed:
+            break
+    emb = m.get_embedding().cpu().numpy()
+    done = bool(r.insertion_completed and r.update_completed and
+                r.embedding_completed)
+    m.close()
+    # per-op seconds per cycle, grouped into tenths of the row count
+    table = []
+    edges = np.linspace(0, n, 11)
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        sel = [q for q in rows if lo < q[0] <= hi]
+        ent = {"rows": [int(lo), int(hi)], "cycles": len(sel)}
+        for name, i_ops, i_t in (("insertion", 1, 2), ("update", 3, 4),
+                                 ("embedding", 5, 6)):
+            ops = sum(q[i_ops] for q in sel)
+            secs = sum(q[i_t] for q in sel)
+            ent[f"{name}_s_per_op"] = secs / ops if ops else None
+        table.append(ent)
+    lat_ms = 1e3 * np.asarray(lat)
+    out = {"value": n / float(np.sum(lat)), "unit": "points/s",
+           "budget_s": budget, "cycles": len(lat), "completed": done,
+           "latency_p50_ms": float(np.percentile(lat_ms, 50)),
+           "latency_p99_ms": float(np.percentile(lat_ms, 99)),
+           "latency_max_ms": float(lat_ms.max()),
+           "over_budget": int((lat_ms > 1e3 * budget).sum()),
+           "cost_model": table}
+    out["trustworthiness_at_15"] = ev.trustworthiness(
+        x, emb, sample=5000, metric=p.metric)
+    return out
 
 
 def load_traffic(kernel):
@@ -480,6 +623,10 @@ def main():
                          "the workload ('' for none)")
     ap.add_argument("--no-serving-mix", action="store_true",
                     help="skip the 64-session serving mix")
+    ap.add_argument("--no-strategy", action="store_true",
+                    help="skip the WeightedStrategy run(target_latency) leg")
+    ap.add_argument("--budget", type=float, default=0.5,
+                    help="run(target_latency) budget of the strategy leg, s")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device copy of MIB MiB (a kernel of "
                          "known traffic that scales rocprofv3 PMC counters)")
@@ -571,6 +718,21 @@ def main():
                 f"{m['request_latency_p50_ms']:.1f} ms p99 "
                 f"{m['request_latency_p99_ms']:.1f} ms")
 
+    strat = None
+    if not args.no_strategy:
+        strat = strategy_leg(ev, x, xdev, n, d, args.knn_mode,
+                             budget=args.budget)
+        log(f"[strategy run({args.budget})] {strat['value']:.0f} points/s "
+            f"over {strat['cycles']} cycles, p50 "
+            f"{strat['latency_p50_ms']:.1f} ms p99 "
+            f"{strat['latency_p99_ms']:.1f} ms, "
+            f"{strat['over_budget']} over budget")
+    copy_rate = hbm_copy_rate()
+    log(f"streaming copy {copy_rate / 1e9:.0f} GB/s")
+    if roof:
+        roof["peak_copy_measured"] = copy_rate / 1e9
+        roof["frac_of_copy"] = roof["achieved"] / (copy_rate / 1e9)
+
     cpu = None
     if not args.no_cpu_baseline:
         # a sample of about 10-30 s of CPU work on 16 cores
@@ -610,6 +772,8 @@ def main():
         "compare": compare,
         "served": served_out,
         "serving_mix": mix,
+        "strategy": strat,
+        "hbm_copy_GBps": copy_rate / 1e9,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -85,3 +85,27 @@ def test_run_target_latency_completes():
     assert np.isfinite(emb).all()
     assert ev.trustworthiness(x, emb, sample=3000) > 0.85
     m.close()
+
+
+def test_reference_module_path_runs_the_engine():
+    # `import gpumap` as the reference's users write it: the same session as
+    # through gpumap_amd, and the same kNN graph as the oracle
+    import gpumap
+    from gpumap._core import GPUMAPParams
+    from gpumap.data_loader import ArrayDataLoader
+
+    x = ev.gaussian10(n=1200, d=16, index=44)
+    p = GPUMAPParams()
+    p.n_instances, p.n_features, p.n_neighbors = 1200, 16, 15
+    m = gpumap.GPUMAP(ArrayDataLoader(x), p)
+    assert isinstance(m.strategy, gpumap.WeightedStrategy)
+    o = oracle.Oracle(1200, 16, a=m.params.a, b=m.params.b, n_threads=8,
+                      knn_mode=1)
+    for s in range(0, 1200, 400):
+        res = m.run_with_n_ops(400, 400, 0)
+        o.append(x[s:s + 400])
+        o.run(400, 400, 0)
+    i2, d2 = o.knn()
+    assert np.array_equal(m.get_neighbor_indices().cpu().numpy(), i2)
+    assert np.array_equal(m.get_neighbor_dists().cpu().numpy(), d2)
+    m.close()
