@@ -567,17 +567,14 @@ def strategy_leg(ev, x, xdev, n, d, knn_mode, budget=0.5, max_cycles=2000):
     done = bool(r.insertion_completed and r.update_completed and
                 r.embedding_completed)
     m.close()
-    # per-op seconds per cycle, grouped into tenths of the row count
+    # per-op seconds of each cycle, against the rows inserted at its end
     table = []
-    edges = np.linspace(0, n, 11)
-    for lo, hi in zip(edges[:-1], edges[1:]):
-        sel = [q for q in rows if lo < q[0] <= hi]
-        ent = {"rows": [int(lo), int(hi)], "cycles": len(sel)}
+    for q in rows:
+        ent = {"rows": int(q[0])}
         for name, i_ops, i_t in (("insertion", 1, 2), ("update", 3, 4),
                                  ("embedding", 5, 6)):
-            ops = sum(q[i_ops] for q in sel)
-            secs = sum(q[i_t] for q in sel)
-            ent[f"{name}_s_per_op"] = secs / ops if ops else None
+            ent[f"{name}_ops"] = int(q[i_ops])
+            ent[f"{name}_s_per_op"] = q[i_t] / q[i_ops] if q[i_ops] else None
         table.append(ent)
     lat_ms = 1e3 * np.asarray(lat)
     out = {"value": n / float(np.sum(lat)), "unit": "points/s",
@@ -720,13 +717,19 @@ def main():
 
     strat = None
     if not args.no_strategy:
+        # the requested budget, then a tenth of it (more cycles: a finer
+        # per-op cost model over the row count)
         strat = strategy_leg(ev, x, xdev, n, d, args.knn_mode,
                              budget=args.budget)
-        log(f"[strategy run({args.budget})] {strat['value']:.0f} points/s "
-            f"over {strat['cycles']} cycles, p50 "
-            f"{strat['latency_p50_ms']:.1f} ms p99 "
-            f"{strat['latency_p99_ms']:.1f} ms, "
-            f"{strat['over_budget']} over budget")
+        strat["tenth_budget"] = strategy_leg(ev, x, xdev, n, d,
+                                             args.knn_mode,
+                                             budget=args.budget / 10)
+        for s_ in (strat, strat["tenth_budget"]):
+            log(f"[strategy run({s_['budget_s']})] {s_['value']:.0f} "
+                f"points/s over {s_['cycles']} cycles, p50 "
+                f"{s_['latency_p50_ms']:.1f} ms p99 "
+                f"{s_['latency_p99_ms']:.1f} ms, "
+                f"{s_['over_budget']} over budget")
     copy_rate = hbm_copy_rate()
     log(f"streaming copy {copy_rate / 1e9:.0f} GB/s")
     if roof:
@@ -773,6 +776,7 @@ def main():
         "served": served_out,
         "serving_mix": mix,
         "strategy": strat,
+        "screen_shapes": capi.screen_shapes(),
         "hbm_copy_GBps": copy_rate / 1e9,
         "cpu_baseline": cpu,
     }

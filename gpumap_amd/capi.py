@@ -111,6 +111,8 @@ def lib():
         L.gm_is_released.argtypes = [vp, P(i32)]
         L.gm_shutdown.argtypes = []
         L.gm_shutdown.restype = None
+        L.gm_screen_shapes.argtypes = [ctypes.c_char_p, u64]
+        L.gm_screen_shapes.restype = i64
         # sessions still alive at exit are freed while the HIP runtime is up
         # (registered after torch's hooks, so it runs before them)
         atexit.register(L.gm_shutdown)
@@ -125,6 +127,21 @@ def _p(a, ct):
 def check(rc):
     if rc != 0:
         raise RuntimeError(f"gm status {rc}: {lib().gm_last_error().decode()}")
+
+
+def screen_shapes():
+    """The BF16 screen's launch shapes tuned so far in this process (see
+    gm_screen_shapes), as a list of dicts."""
+    import json
+
+    need = lib().gm_screen_shapes(None, 0)
+    if need < 0:
+        check(int(need))
+    buf = ctypes.create_string_buffer(int(need) + 1)
+    got = lib().gm_screen_shapes(buf, len(buf))
+    if got < 0:
+        check(int(got))
+    return json.loads(buf.value.decode())
 
 
 def philox(ctr, key):

@@ -27,6 +27,8 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <map>
+#include <tuple>
 #include <unordered_set>
 #include <functional>
 #include <mutex>
@@ -244,6 +246,102 @@ bool sgd_launches(int c, float *e, const int32_t *head, const int32_t *tail,
 
 struct Engine;
 // Every live session of the process, for gm_shutdown.
+// ------------------------------------------------- screen shape tuner
+// The BF16 screen's launch shape -- the column splits per 128-row tile
+// (bsplit, the grid's second factor) and how the survivors are listed (a
+// scan of the stored lower bounds, or a second pass over the tiles) -- is
+// chosen per (device, padded feature count, row count within a factor of 4,
+// batch tiles) by timing the candidates around the size heuristic on the
+// first insertions that meet the key: each candidate once, on the session
+// stream between two events, then the fastest for every later insertion of
+// any session in the process.  Results do not depend on the shape (the
+// pools are exact); only the time does.  GPUMAP_AMD_TUNE=0 keeps the
+// heuristic.
+struct ScreenShape {
+  int bsplit;
+  bool lbscan;
+};
+struct TuneKey {
+  int dev;
+  uint64_t db;
+  int nlog4;
+  uint64_t brt;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(dev, db, nlog4, brt) < std::tie(o.dev, o.db, o.nlog4, o.brt);
+  }
+};
+struct TuneEntry {
+  std::vector<ScreenShape> cand;
+  std::vector<float> ms; // < 0: not timed yet
+  int issued = 0;        // candidates handed out for timing
+  int best = -1;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneEntry> g_tune;
+
+bool tune_enabled() {
+  static const bool on = [] {
+    const char *v = std::getenv("GPUMAP_AMD_TUNE");
+    return !(v && v[0] == '0');
+  }();
+  return on;
+}
+
+// The shape for an insertion under `key`; *trial is the candidate index to
+// time, or -1 when the shape is the tuned (or the only) one.
+ScreenShape tune_choose(const TuneKey &key, int heuristic, int max_split,
+                        bool lbm_ok, int *trial) {
+  *trial = -1;
+  if (!tune_enabled())
+    return ScreenShape{heuristic, lbm_ok};
+  std::lock_guard<std::mutex> l(g_tune_mu);
+  TuneEntry &t = g_tune[key];
+  if (t.cand.empty()) {
+    for (int f : {1, 2, 4}) {
+      const int b = std::max(1, std::min(heuristic * f / 2, max_split));
+      for (bool scan : {true, false}) {
+        if (scan && !lbm_ok)
+          continue;
+        bool dup = false;
+        for (auto &c : t.cand)
+          dup |= c.bsplit == b && c.lbscan == scan;
+        if (!dup)
+          t.cand.push_back(ScreenShape{b, scan});
+      }
+    }
+    t.ms.assign(t.cand.size(), -1.0f);
+  }
+  if (t.best >= 0)
+    return t.cand[t.best];
+  if (t.cand.size() == 1) {
+    t.best = 0;
+    return t.cand[0];
+  }
+  if (t.issued < (int)t.cand.size()) {
+    *trial = t.issued++;
+    return t.cand[*trial];
+  }
+  // every candidate handed out, some not timed yet: the heuristic meanwhile
+  return ScreenShape{heuristic, lbm_ok};
+}
+
+void tune_report(const TuneKey &key, int trial, float ms) {
+  std::lock_guard<std::mutex> l(g_tune_mu);
+  auto it = g_tune.find(key);
+  if (it == g_tune.end() || trial < 0 || trial >= (int)it->second.ms.size())
+    return;
+  TuneEntry &t = it->second;
+  t.ms[trial] = ms;
+  int best = -1;
+  for (int i = 0; i < (int)t.ms.size(); ++i) {
+    if (t.ms[i] < 0.0f)
+      return; // not all timed yet
+    if (best < 0 || t.ms[i] < t.ms[best])
+      best = i;
+  }
+  t.best = best;
+}
+
 std::mutex g_live_mu;
 std::vector<Engine *> g_live;
 
@@ -353,6 +451,13 @@ struct Engine {
     bool on = false;
     uint64_t b0 = 0, B = 0, pre = 0;
   } pins;
+  // a screen shape being timed for the tuner (read in insert_finish)
+  struct TunePending {
+    bool on = false;
+    TuneKey key{};
+    int trial = -1;
+    hipEvent_t a = nullptr, b = nullptr;
+  } tune_pend;
 
   // host state (reference Impl members, impl.cuh:33-55)
   uint64_t n = 0;
@@ -768,6 +873,15 @@ struct Engine {
   // batch rows, then the earlier rows whose pools changed (ascending), go to
   // the update queue, as the reference's insert pushes them.
   void insert_finish() {
+    if (tune_pend.on) {
+      tune_pend.on = false;
+      float ms = 0.0f;
+      HIPC(hipEventSynchronize(tune_pend.b));
+      HIPC(hipEventElapsedTime(&ms, tune_pend.a, tune_pend.b));
+      tune_report(tune_pend.key, tune_pend.trial, ms);
+      ev_free.push_back(tune_pend.a);
+      ev_free.push_back(tune_pend.b);
+    }
     if (!pins.on)
       return;
     pins.on = false;
@@ -819,13 +933,28 @@ struct Engine {
     while (nsplit < 128 && row_tiles * (uint64_t)nsplit < 1024 &&
            (uint64_t)nsplit * 2 <= col_tiles)
       nsplit *= 2;
-    // BF16 screening: 128 x 128 block tiles, about two blocks per CU
+    // BF16 screening: 128 x 128 block tiles; the heuristic takes about two
+    // blocks per CU, the tuner times its neighbours (tune_choose)
     const uint64_t brt = (B + 127) / 128, bct = (n + 127) / 128;
     int bsplit = 1;
-    if (p.knn_mode == GM_KNN_EXACT_BF16)
+    ScreenShape shape{1, false};
+    TuneKey tkey{};
+    int trial = -1;
+    if (p.knn_mode == GM_KNN_EXACT_BF16) {
+      ensure_bf16_batch(B);
       while (bsplit < 128 && brt * (uint64_t)bsplit * 2 <= 2 * (uint64_t)num_cu &&
              (uint64_t)bsplit * 2 <= bct)
         bsplit *= 2;
+      int max_split = 1;
+      while (max_split < 128 && (uint64_t)max_split * 2 <= bct)
+        max_split *= 2;
+      int nlog4 = 0;
+      while (nlog4 < 31 && (4ull << (2 * nlog4)) <= n)
+        ++nlog4;
+      tkey = TuneKey{device, Db, nlog4, brt};
+      shape = tune_choose(tkey, bsplit, max_split, lbm != nullptr, &trial);
+      bsplit = shape.bsplit;
+    }
     const uint64_t need = B * (uint64_t)std::max(nsplit, bsplit) * W;
     if (need > part_cap) {
       if (part)
@@ -840,7 +969,6 @@ struct Engine {
     const dim3 grid((unsigned)row_tiles, (unsigned)nsplit);
     const unsigned *guard = nullptr;
     if (p.knn_mode == GM_KNN_EXACT_BF16) {
-      ensure_bf16_batch(B);
       // Error of d~ = s - 2 P against the exact squared distance d*, with
       // u = 2^-24, s = fl(n_u + n_j), P the screened dot product:
       //   norms n_u, n_j: each lane's 8 ceil(Db / 512) terms in sequence,
@@ -868,16 +996,22 @@ struct Engine {
       const int nsteps = (int)(Db / 16);
       const unsigned blocks = (unsigned)(brt * bsplit);
       float *part_f = reinterpret_cast<float *>(part);
+      // the lower-bound store only when the survivors come from its scan
+      float *const lbs = shape.lbscan ? lbm : nullptr;
       k_bf16_prep<<<(unsigned)B, 64, 0, s>>>(X, Dp, Db, Xs, Xb, nrm, flag,
                                              rcnt, qctl, b0, B);
       LAUNCH_CHECK();
+      if (trial >= 0) {
+        tune_pend = TunePending{true, tkey, trial, ev_get(), ev_get()};
+        HIPC(hipEventRecord(tune_pend.a, s));
+      }
       k_bf16_screen<1><<<blocks, 256, 0, s>>>(
           Xs, Xb, nsteps, nrm, pmax, tau, b0, B, n, bsplit, (int)brt, part_f,
-          rlist, rcnt, rcap, qctl, coef, cfac, lbm);
+          rlist, rcnt, rcap, qctl, coef, cfac, lbs);
       LAUNCH_CHECK();
       k_tau_select<<<(unsigned)B, 256, 0, s>>>(part_f, bsplit, tau, B);
       LAUNCH_CHECK();
-      if (lbm) {
+      if (lbs) {
         // waves = row groups x column ranges, about 16 waves per CU
         const uint64_t groups = (B + 63) / 64;
         const uint64_t ranges =
@@ -885,7 +1019,7 @@ struct Engine {
         const uint64_t cpw = std::max<uint64_t>(8, (n + ranges - 1) / ranges);
         const uint64_t waves = groups * ((n + cpw - 1) / cpw);
         k_bf16_lbscan<<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(
-            lbm, B, n, b0, tau, pmax, rlist, rcnt, rcap, qctl, cpw);
+            lbs, B, n, b0, tau, pmax, rlist, rcnt, rcap, qctl, cpw);
       } else {
         k_bf16_screen<2><<<blocks, 256, 0, s>>>(
             Xs, Xb, nsteps, nrm, pmax, tau, b0, B, n, bsplit, (int)brt,
@@ -900,6 +1034,8 @@ struct Engine {
       k_bf16_select<<<(unsigned)B, 64, rcap * sizeof(uint64_t), s>>>(
           fkeys, rcnt + B, rcap, qctl, pool, b0, prof ? kstats : nullptr);
       LAUNCH_CHECK();
+      if (trial >= 0)
+        HIPC(hipEventRecord(tune_pend.b, s));
       // the FP32 tiles below run only if the survivor queue overflowed
       guard = qctl;
     }
@@ -1885,6 +2021,45 @@ int gm_append_data_device(gm_handle *h, const float *device_rows,
       throw Fail(GM_ERR_INVALID, "null data pointer");
     h->e->append_device(device_rows, rows);
   });
+}
+
+int64_t gm_screen_shapes(char *buf, uint64_t cap) {
+  try {
+    std::string out = "[";
+    {
+      std::lock_guard<std::mutex> l(gm::g_tune_mu);
+      bool first = true;
+      for (const auto &kv : gm::g_tune) {
+        const gm::TuneKey &k = kv.first;
+        const gm::TuneEntry &t = kv.second;
+        char head[160];
+        std::snprintf(head, sizeof head,
+                      "%s{\"device\": %d, \"db\": %llu, \"rows_log4\": %d, "
+                      "\"batch_tiles\": %llu, \"best\": %d, \"candidates\": [",
+                      first ? "" : ", ", k.dev, (unsigned long long)k.db,
+                      k.nlog4, (unsigned long long)k.brt, t.best);
+        out += head;
+        for (size_t c = 0; c < t.cand.size(); ++c) {
+          char item[96];
+          std::snprintf(item, sizeof item,
+                        "%s{\"bsplit\": %d, \"lbscan\": %s, \"ms\": %.6g}",
+                        c ? ", " : "", t.cand[c].bsplit,
+                        t.cand[c].lbscan ? "true" : "false", (double)t.ms[c]);
+          out += item;
+        }
+        out += "]}";
+        first = false;
+      }
+    }
+    out += "]";
+    if (buf && cap > out.size())
+      std::memcpy(buf, out.c_str(), out.size() + 1);
+    gm::set_last_error(nullptr);
+    return (int64_t)out.size();
+  } catch (const std::exception &x) {
+    gm::set_last_error(x.what());
+    return GM_ERR_HIP;
+  }
 }
 
 int gm_run_batch(gm_handle *const *hs, int32_t count, const uint32_t *ops,

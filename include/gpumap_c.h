@@ -277,6 +277,17 @@ typedef struct gm_kernel_stats {
  * read back by gm_get_kernel_stats (which synchronises the session). */
 int gm_set_profiling(gm_handle *h, int32_t enable);
 int gm_get_kernel_stats(gm_handle *h, int32_t which, gm_kernel_stats *out);
+
+/* The BF16 screen's launch shapes chosen so far by the process's shape
+ * tuner.  No counterpart in the reference, whose launch shapes are constants
+ * (insertion_kernel.cuh:961-969, update_kernel.cuh:386-396).  Per key
+ * (device, padded feature count "db", row count as floor(log4 n) "rows_log4",
+ * 128-row batch tiles "batch_tiles"): the candidates timed so far
+ * ({"bsplit", "lbscan", "ms"}, ms < 0 when not timed yet) and the chosen
+ * one's index ("best", -1 while tuning), as a NUL-terminated JSON array
+ * written to buf when buf holds it.  Returns the text's length without the
+ * NUL, whether or not it fitted, or a negative gm_status. */
+int64_t gm_screen_shapes(char *buf, uint64_t cap);
 int gm_reset_kernel_stats(gm_handle *h);
 
 /* Philox4x32-10 of the device code, for known-answer tests. */

@@ -332,6 +332,36 @@ def test_bf16_screening_survivor_paths(lb_store, monkeypatch):
     assert np.array_equal(np.sort(e1.pool(), 1), np.sort(e2.pool(), 1))
 
 
+def test_screen_shape_tuner_times_candidates_and_keeps_the_graph():
+    # the BF16 screen's launch shape (column splits, survivor listing) is
+    # tuned per key on the first insertions; every shape gives the FP32
+    # tiles' graph, and a key whose candidates are all timed uses the
+    # fastest of them
+    x = ev.image_like(n=6000, d=160, index=45)
+    n, d = x.shape
+    e1, e2 = capi.Engine(n, d, knn_mode=1), capi.Engine(n, d, knn_mode=2)
+    for s in range(0, n, 250):
+        for e in (e1, e2):
+            e.append(x[s:s + 250])
+            e.run(250, 250, 0)
+    (i1, d1), (i2, d2) = e1.knn(), e2.knn()
+    assert np.array_equal(i1, i2) and np.array_equal(d1, d2)
+    shapes = capi.screen_shapes()
+    mine = [t for t in shapes if t["db"] == 160]
+    assert mine, shapes
+    tuned = 0
+    for t in mine:
+        cand = t["candidates"]
+        assert len(cand) >= 1 and -1 <= t["best"] < len(cand)
+        assert all(c["bsplit"] >= 1 for c in cand)
+        if t["best"] >= 0:
+            ms = [c["ms"] for c in cand]
+            assert min(ms) >= 0 and ms[t["best"]] == min(ms), t
+            tuned += 1
+    # 24 insertions over the keys of n < 4096 and n >= 4096 (one batch tile)
+    assert tuned >= 1, mine
+
+
 def test_sgd_fused_equals_per_epoch_launches():
     # conflict-free edges, attraction only: the single cooperative launch and
     # the one-launch-per-epoch path compute the same epochs
