@@ -35,10 +35,22 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 BF16_PEAK = 2.5e15         # MI355X dense BF16 MFMA, flop/s
+FP8_PEAK = 5.0e15          # MI355X dense FP8 MFMA, flop/s
 CHUNK = 500  # the workload's insertion chunk (set from WORKLOAD_CFG)
 METRIC = ("points embedded/s @recall@15≥0.95, BF16-MFMA vs FP32; "
           "p50/p99 run() latency")
-MODES = {"nndescent": 0, "exact_fp32": 1, "exact_bf16": 2}
+MODES = {"nndescent": 0, "exact_fp32": 1, "exact_bf16": 2, "exact_fp8": 3}
+PRECISION = {
+    0: "NN-Descent candidates and local join in FP32",
+    1: "FP32",
+    2: ("distance tiles BF16 MFMA (hi/lo split, FP32 accumulation) screened "
+        "by a rigorous error bound and re-ranked in exact FP32; sigma/rho "
+        "FP64; SGD FP32"),
+    3: ("distance tiles FP8 E4M3 MFMA (per-row power-of-two scales, hi/lo "
+        "split, FP32 accumulation) screened by a rigorous error bound and "
+        "re-ranked in exact FP32; sigma/rho FP64; SGD FP32"),
+}
+DTYPE = {0: "f32", 1: "f32", 2: "bf16", 3: "fp8_e4m3"}
 
 WORKLOAD_DESC = {
     "image_like": "70,000x784 float32 image-like 10-cluster mixture, sparse "
@@ -671,9 +683,11 @@ def main():
                 "traffic": load_traffic(name),
                 "algo_bytes_per_launch": kd["algo_bytes_per_launch"],
                 "avg_launch_ms": kd["avg_ms"]}
-    if "insertion" in kern and mode == MODES["exact_bf16"]:
+    if "insertion" in kern and mode in (MODES["exact_bf16"],
+                                        MODES["exact_fp8"]):
         kern["insertion"]["mfma_frac_algorithmic"] = (
-            kern["insertion"]["achieved_TFLOPs"] * 1e12 / BF16_PEAK)
+            kern["insertion"]["achieved_TFLOPs"] * 1e12 /
+            (FP8_PEAK if mode == MODES["exact_fp8"] else BF16_PEAK))
 
     if args.copy_probe:
         src = torch.empty(args.copy_probe << 18, dtype=torch.float32,
@@ -754,15 +768,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16" if mode == MODES["exact_bf16"] else "f32",
+        "dtype": DTYPE[mode],
         "data": "synthetic",
         "config": {"workload": WORKLOAD_DESC[args.workload], "n_points": n,
                    "n_features": d, "knn_mode": args.knn_mode,
-                   "precision": ("distance tiles BF16 MFMA (hi/lo split, FP32 "
-                                 "accumulation) screened by a rigorous error "
-                                 "bound and re-ranked in exact FP32; sigma/rho "
-                                 "FP64; SGD FP32")
-                   if mode == MODES["exact_bf16"] else "FP32",
+                   "precision": PRECISION[mode],
                    "op_plan": f"run({CHUNK},{CHUNK},{CHUNK}) per "
                               f"{CHUNK}-row chunk, then run(0,N,N) until "
                               "update_completed, then one full embedding "

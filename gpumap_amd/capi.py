@@ -111,6 +111,7 @@ def lib():
         L.gm_is_released.argtypes = [vp, P(i32)]
         L.gm_shutdown.argtypes = []
         L.gm_shutdown.restype = None
+        L.gm_debug_e4m3.argtypes = [P(f32), P(ctypes.c_uint8), u64, i32]
         L.gm_screen_shapes.argtypes = [ctypes.c_char_p, u64]
         L.gm_screen_shapes.restype = i64
         # sessions still alive at exit are freed while the HIP runtime is up
@@ -142,6 +143,15 @@ def screen_shapes():
     if got < 0:
         check(int(got))
     return json.loads(buf.value.decode())
+
+
+def e4m3(x, on_device=False):
+    """The FP8 screen's float -> E4M3 codes (gm_debug_e4m3)."""
+    x = np.ascontiguousarray(x, np.float32).ravel()
+    out = np.zeros(x.size, np.uint8)
+    check(lib().gm_debug_e4m3(_p(x, f32), _p(out, ctypes.c_uint8), x.size,
+                              1 if on_device else 0))
+    return out
 
 
 def philox(ctr, key):

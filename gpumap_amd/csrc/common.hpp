@@ -12,6 +12,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+#include <cstring>
+
 namespace gm {
 
 constexpr int W = 32;        // pool width
@@ -305,6 +308,42 @@ __device__ inline bool pool_offer_lane(uint64_t *__restrict__ row,
     if (cas64(row + ml, mx, key))
       return true;
   }
+}
+
+// OCP FP8 E4M3 (the gfx950 matrix cores' E4M3 operands): sign, 4 exponent
+// bits (bias 7), 3 mantissa bits; normals 2^-6 .. 448, subnormals k 2^-9,
+// code 0x7f is NaN.  e4m3_rn rounds to nearest, ties to an even mantissa,
+// and saturates at +-448 (|x| above 464, which would round beyond the range,
+// also gives 448); the screens only convert values with |x| <= 448.  The
+// rounding error is at most 2^-4 |x| (normal range) or 2^-10 (subnormal).
+__host__ __device__ inline float e4m3_dec(uint8_t v) {
+  const int e = (v >> 3) & 15, m = v & 7;
+  const float r = e == 0 ? (float)m * 0.001953125f // m 2^-9
+                         : ldexpf(1.0f + (float)m * 0.125f, e - 7);
+  return (v & 0x80) ? -r : r;
+}
+
+__host__ __device__ inline uint8_t e4m3_rn(float x) {
+  uint32_t b;
+  memcpy(&b, &x, 4);
+  const uint8_t sign = (uint8_t)((b >> 24) & 0x80u);
+  b &= 0x7fffffffu;
+  float a;
+  memcpy(&a, &b, 4);
+  if (!(a < 464.0f))
+    return (uint8_t)(sign | 0x7e); // 448 (NaN also lands here)
+  if (a < 0.015625f) {             // below 2^-6: multiples of 2^-9
+    const float q = rintf(a * 512.0f); // exact scaling, then nearest-even
+    return (uint8_t)(sign | (uint8_t)q); // q = 8 is 2^-6, code 0x08
+  }
+  // round the 23-bit mantissa to 3 bits, nearest-even (a carry moves to
+  // the exponent); 2^-6 <= a < 464 keeps the exponent within 1 .. 15
+  const uint32_t r = (b + 0x7ffffu + ((b >> 20) & 1u)) & ~0xfffffu;
+  const int e = (int)(r >> 23) - 127 + 7;
+  const uint32_t m = (r >> 20) & 7u;
+  if (e > 15 || (e == 15 && m == 7))
+    return (uint8_t)(sign | 0x7e);
+  return (uint8_t)(sign | (uint8_t)(e << 3) | (uint8_t)m);
 }
 
 } // namespace gm

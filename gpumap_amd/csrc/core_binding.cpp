@@ -76,6 +76,8 @@ std::string knn_mode_to_string(int32_t m) {
     return "nndescent";
   case GM_KNN_EXACT_BF16:
     return "exact_bf16";
+  case GM_KNN_EXACT_FP8:
+    return "exact_fp8";
   default:
     return "exact_fp32";
   }
@@ -92,12 +94,14 @@ int32_t knn_mode_from_object(const py::object &v) {
       return GM_KNN_EXACT_FP32;
     if (s == "exact_bf16")
       return GM_KNN_EXACT_BF16;
+    if (s == "exact_fp8")
+      return GM_KNN_EXACT_FP8;
     throw py::value_error("Unknown knn_mode '" + s +
-                          "' -- expected 'nndescent', 'exact_fp32' or "
-                          "'exact_bf16'");
+                          "' -- expected 'nndescent', 'exact_fp32', "
+                          "'exact_bf16' or 'exact_fp8'");
   }
   const int m = v.cast<int>();
-  if (m < GM_KNN_NNDESCENT || m > GM_KNN_EXACT_BF16)
+  if (m < GM_KNN_NNDESCENT || m > GM_KNN_EXACT_FP8)
     throw py::value_error("Unknown knn_mode " + std::to_string(m));
   return m;
 }

@@ -250,7 +250,8 @@ struct Engine;
 // The BF16 screen's launch shape -- the column splits per 128-row tile
 // (bsplit, the grid's second factor) and how the survivors are listed (a
 // scan of the stored lower bounds, or a second pass over the tiles) -- is
-// chosen per (device, padded feature count, row count within a factor of 4,
+// chosen per (device, screen, padded feature count, row count within a
+// factor of 4,
 // batch tiles) by timing the candidates around the size heuristic on the
 // first insertions that meet the key: each candidate once, on the session
 // stream between two events, then the fastest for every later insertion of
@@ -263,11 +264,13 @@ struct ScreenShape {
 };
 struct TuneKey {
   int dev;
+  uint64_t mode; // knn_mode: the BF16 and FP8 screens are tuned apart
   uint64_t db;
   int nlog4;
   uint64_t brt;
   bool operator<(const TuneKey &o) const {
-    return std::tie(dev, db, nlog4, brt) < std::tie(o.dev, o.db, o.nlog4, o.brt);
+    return std::tie(dev, mode, db, nlog4, brt) <
+           std::tie(o.dev, o.mode, o.db, o.nlog4, o.brt);
   }
 };
 struct TuneEntry {
@@ -371,11 +374,12 @@ struct Engine {
   float *pmax = nullptr;      // per-row pool maximum distance (inf if not full)
   uint64_t *part = nullptr;   // exact insertion: per (row, split) top W
   uint64_t part_cap = 0;
-  // BF16 screening (knn_mode exact_bf16): hi/lo split rows, FP32 norms,
-  // per-batch-row tau
+  // BF16 / FP8 screening (knn_mode exact_bf16, exact_fp8): hi/lo split
+  // rows (Db features, 4 or 2 bytes each), FP32 norms, the FP8 rows' inverse
+  // scales, per-batch-row tau
   uint64_t Db = 0;
   __bf16 *Xs = nullptr, *Xb = nullptr; // column and batch operand stores
-  float *nrm = nullptr, *tau = nullptr;
+  float *nrm = nullptr, *tau = nullptr, *isc = nullptr;
   uint32_t *rlist = nullptr; // survivors per batch row: column | flags
   uint64_t *fkeys = nullptr; // forward keys of the survivors
   unsigned *rcnt = nullptr;  // survivors per batch row
@@ -451,6 +455,9 @@ struct Engine {
     bool on = false;
     uint64_t b0 = 0, B = 0, pre = 0;
   } pins;
+  bool screened() const {
+    return p.knn_mode == GM_KNN_EXACT_BF16 || p.knn_mode == GM_KNN_EXACT_FP8;
+  }
   // a screen shape being timed for the tuner (read in insert_finish)
   struct TunePending {
     bool on = false;
@@ -518,15 +525,24 @@ struct Engine {
     list = dalloc<uint32_t>(N);
     pmax = dalloc<float>(N);
     HIPC(hipMemsetD32Async((hipDeviceptr_t)pmax, 0x7f800000, N, s));
-    if (p.knn_mode == GM_KNN_EXACT_BF16) {
-      Db = (D + 15) / 16 * 16;
+    if (screened()) {
+      const bool f8 = p.knn_mode == GM_KNN_EXACT_FP8;
+      Db = f8 ? (D + 63) / 64 * 64 : (D + 15) / 16 * 16;
       const uint64_t rows_ = (N + 127) / 128 * 128;
-      Xs = dalloc<__bf16>(rows_ * Db * 2);
-      HIPC(hipMemsetAsync(Xs, 0, rows_ * Db * 2 * sizeof(__bf16), s));
+      // BF16: hi + lo, 2 bytes each; FP8: 1 byte each
+      const uint64_t xs_elems = f8 ? rows_ * Db : rows_ * Db * 2;
+      Xs = dalloc<__bf16>(xs_elems);
+      HIPC(hipMemsetAsync(Xs, 0, xs_elems * sizeof(__bf16), s));
       nrm = dalloc<float>(rows_);
       HIPC(hipMemsetAsync(nrm, 0, rows_ * sizeof(float), s));
-      rcap = 2048;
-      if (N > (1u << 18)) {
+      if (f8) {
+        isc = dalloc<float>(rows_);
+        HIPC(hipMemsetAsync(isc, 0, rows_ * sizeof(float), s));
+      }
+      // survivor list per batch row: the FP8 screen's looser bound keeps
+      // about five times the BF16 screen's survivors
+      rcap = f8 ? 8192 : 2048;
+      if (N > (1u << 18) || f8) {
         int smem = 0;
         HIPC(hipDeviceGetAttribute(
             &smem, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
@@ -646,7 +662,7 @@ struct Engine {
         (void **)&wflags, (void **)&head,    (void **)&tail,    (void **)&wout,
         (void **)&eps,    (void **)&nxt,     (void **)&nneg,    (void **)&snap,
         (void **)&rounding, (void **)&deg,   (void **)&maxdeg,  (void **)&dbatch,
-        (void **)&bmark};
+        (void **)&bmark,  (void **)&isc};
     for (void **q : ptrs) {
       if (*q)
         (void)hipFree(*q);
@@ -940,7 +956,7 @@ struct Engine {
     ScreenShape shape{1, false};
     TuneKey tkey{};
     int trial = -1;
-    if (p.knn_mode == GM_KNN_EXACT_BF16) {
+    if (screened()) {
       ensure_bf16_batch(B);
       while (bsplit < 128 && brt * (uint64_t)bsplit * 2 <= 2 * (uint64_t)num_cu &&
              (uint64_t)bsplit * 2 <= bct)
@@ -951,7 +967,7 @@ struct Engine {
       int nlog4 = 0;
       while (nlog4 < 31 && (4ull << (2 * nlog4)) <= n)
         ++nlog4;
-      tkey = TuneKey{device, Db, nlog4, brt};
+      tkey = TuneKey{device, (uint64_t)p.knn_mode, Db, nlog4, brt};
       shape = tune_choose(tkey, bsplit, max_split, lbm != nullptr, &trial);
       bsplit = shape.bsplit;
     }
@@ -968,7 +984,8 @@ struct Engine {
     ev_start(GM_KSTAT_INSERT);
     const dim3 grid((unsigned)row_tiles, (unsigned)nsplit);
     const unsigned *guard = nullptr;
-    if (p.knn_mode == GM_KNN_EXACT_BF16) {
+    if (screened()) {
+      const bool f8 = p.knn_mode == GM_KNN_EXACT_FP8;
       // Error of d~ = s - 2 P against the exact squared distance d*, with
       // u = 2^-24, s = fl(n_u + n_j), P the screened dot product:
       //   norms n_u, n_j: each lane's 8 ceil(Db / 512) terms in sequence,
@@ -985,29 +1002,74 @@ struct Engine {
       // rounded differences) is within (D + 2) u relative of d*.  coef = 1.25
       // A and cfac = 1.25 (D + 2) u; the factor 1.25 covers second-order
       // terms and the roundings of the bound arithmetic itself.
+      //
+      // FP8 (E4M3 hi/lo of a = s x, s a power of two with max|a| in
+      // (224, 448]): per feature |a - hi| <= 2^-4 |a| + 2^-10 and the
+      // residual e = a - hi - lo has |e| <= g |a| + h (g = 2^-8,
+      // h = 1.0625 2^-10), |lo| <= al |a| + be (al = 1.0625 2^-4,
+      // be = 2.0625 2^-10).  a.b - P' = lo_a lo_b + (hi_a + lo_a) e_b +
+      // e_a (hi_b + lo_b) + e_a e_b, so summed over the features
+      //   |a.b - P'| <= C1 sum|a||b| + C2 (sum|a| + sum|b|) + C3 D,
+      // C1 = al^2 + 2 (1 + g) g + g^2, C2 = al be + (1 + g) h + 2 g h,
+      // C3 = be^2 + 3 h^2; with sum|a| <= sqrt(D) |a|, 1 / s <= |x| / 224
+      // and sum|a||b| <= |a||b| the dot's split error is at most
+      // (C1 + 2 C2 sqrt(D) / 224 + C3 D / 224^2) |u||j| <= that times s / 2.
+      // Summation: each group of G = FP8_GROUP_STEPS steps' 192 G exact
+      // products (their magnitudes sum to at most 1.28 |u||j|) at 2u per
+      // addition, then the group adds: 1.28 (384 G + ceil(nsteps / G) + 1) u
+      // of |u||j|.  The scalings by 1 / s are exact.  The split and the
+      // summation terms replace the BF16 ones in A.
       const double u24 = std::ldexp(1.0, -24);
-      const double nsteps_d = (double)(Db / 16);
+      const double step = f8 ? 64.0 : 16.0;
+      const double nsteps_d = (double)Db / step;
       const double gnorm = (8.0 * (double)((Db + 511) / 512) + 6.0) * u24;
-      const double G = (double)BF16_GROUP_STEPS;
-      const double A = gnorm + 3.02 * u24 + 3.01 * std::ldexp(1.0, -16) +
-                       1.01 * (96.0 * G + std::ceil(nsteps_d / G) + 1.0) * u24;
+      double A;
+      if (!f8) {
+        const double G = (double)BF16_GROUP_STEPS;
+        A = gnorm + 3.02 * u24 + 3.01 * std::ldexp(1.0, -16) +
+            1.01 * (96.0 * G + std::ceil(nsteps_d / G) + 1.0) * u24;
+      } else {
+        const double G = (double)FP8_GROUP_STEPS;
+        const double g = std::ldexp(1.0, -8), h = 1.0625 * std::ldexp(1.0, -10);
+        const double al = 1.0625 * std::ldexp(1.0, -4);
+        const double be = 2.0625 * std::ldexp(1.0, -10);
+        const double C1 = al * al + 2.0 * (1.0 + g) * g + g * g;
+        const double C2 = al * be + (1.0 + g) * h + 2.0 * g * h;
+        const double C3 = be * be + 3.0 * h * h;
+        const double Dd = (double)Db;
+        const double split =
+            C1 + 2.0 * C2 * std::sqrt(Dd) / 224.0 + C3 * Dd / (224.0 * 224.0);
+        A = gnorm + 3.02 * u24 + split +
+            1.01 * 1.28 * (384.0 * G + std::ceil(nsteps_d / G) + 1.0) * u24;
+      }
       const float coef = (float)(1.25 * A);
       const float cfac = (float)(1.25 * (double)(D + 2) * u24);
-      const int nsteps = (int)(Db / 16);
+      const int nsteps = (int)(Db / (uint64_t)step);
       const unsigned blocks = (unsigned)(brt * bsplit);
       float *part_f = reinterpret_cast<float *>(part);
       // the lower-bound store only when the survivors come from its scan
       float *const lbs = shape.lbscan ? lbm : nullptr;
-      k_bf16_prep<<<(unsigned)B, 64, 0, s>>>(X, Dp, Db, Xs, Xb, nrm, flag,
-                                             rcnt, qctl, b0, B);
+      uint8_t *const xs8 = reinterpret_cast<uint8_t *>(Xs);
+      uint8_t *const xb8 = reinterpret_cast<uint8_t *>(Xb);
+      if (f8)
+        k_fp8_prep<<<(unsigned)B, 64, 0, s>>>(X, Dp, Db, xs8, xb8, nrm, isc,
+                                              flag, rcnt, qctl, b0, B);
+      else
+        k_bf16_prep<<<(unsigned)B, 64, 0, s>>>(X, Dp, Db, Xs, Xb, nrm, flag,
+                                               rcnt, qctl, b0, B);
       LAUNCH_CHECK();
       if (trial >= 0) {
         tune_pend = TunePending{true, tkey, trial, ev_get(), ev_get()};
         HIPC(hipEventRecord(tune_pend.a, s));
       }
-      k_bf16_screen<1><<<blocks, 256, 0, s>>>(
-          Xs, Xb, nsteps, nrm, pmax, tau, b0, B, n, bsplit, (int)brt, part_f,
-          rlist, rcnt, rcap, qctl, coef, cfac, lbs);
+      if (f8)
+        k_fp8_screen<1><<<blocks, 256, 0, s>>>(
+            xs8, xb8, nsteps, nrm, isc, pmax, tau, b0, B, n, bsplit, (int)brt,
+            part_f, rlist, rcnt, rcap, qctl, coef, cfac, lbs);
+      else
+        k_bf16_screen<1><<<blocks, 256, 0, s>>>(
+            Xs, Xb, nsteps, nrm, pmax, tau, b0, B, n, bsplit, (int)brt, part_f,
+            rlist, rcnt, rcap, qctl, coef, cfac, lbs);
       LAUNCH_CHECK();
       k_tau_select<<<(unsigned)B, 256, 0, s>>>(part_f, bsplit, tau, B);
       LAUNCH_CHECK();
@@ -1020,6 +1082,10 @@ struct Engine {
         const uint64_t waves = groups * ((n + cpw - 1) / cpw);
         k_bf16_lbscan<<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(
             lbs, B, n, b0, tau, pmax, rlist, rcnt, rcap, qctl, cpw);
+      } else if (f8) {
+        k_fp8_screen<2><<<blocks, 256, 0, s>>>(
+            xs8, xb8, nsteps, nrm, isc, pmax, tau, b0, B, n, bsplit, (int)brt,
+            part_f, rlist, rcnt, rcap, qctl, coef, cfac, nullptr);
       } else {
         k_bf16_screen<2><<<blocks, 256, 0, s>>>(
             Xs, Xb, nsteps, nrm, pmax, tau, b0, B, n, bsplit, (int)brt,
@@ -1727,6 +1793,12 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
     es[i]->finish_result(&rs[i]);
 }
 
+__global__ void k_e4m3_probe(const float *in, uint8_t *out, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    out[i] = e4m3_rn(in[i]);
+}
+
 __global__ void k_philox_probe(const uint32_t *in, uint32_t *out) {
   uint32_t c[4] = {in[0], in[1], in[2], in[3]};
   uint32_t kk[2] = {in[4], in[5]};
@@ -1842,11 +1914,14 @@ int gm_create(const gm_params *params, gm_handle **out) {
       throw Fail(GM_ERR_INVALID, "n_features must be positive");
     if (q.metric != GM_EUCLIDEAN && q.metric != GM_COSINE)
       throw Fail(GM_ERR_INVALID, "metric must be 0 (euclidean) or 1 (cosine)");
-    if (q.knn_mode < GM_KNN_NNDESCENT || q.knn_mode > GM_KNN_EXACT_BF16)
+    if (q.knn_mode < GM_KNN_NNDESCENT || q.knn_mode > GM_KNN_EXACT_FP8)
       throw Fail(GM_ERR_INVALID, "knn_mode must be 0 (nndescent), 1 "
-                                 "(exact_fp32) or 2 (exact_bf16)");
-    if (q.knn_mode == GM_KNN_EXACT_BF16 && q.n_instances >= (1ull << 30))
-      throw Fail(GM_ERR_INVALID, "exact_bf16 supports n_instances < 2^30");
+                                 "(exact_fp32), 2 (exact_bf16) or 3 "
+                                 "(exact_fp8)");
+    if ((q.knn_mode == GM_KNN_EXACT_BF16 || q.knn_mode == GM_KNN_EXACT_FP8) &&
+        q.n_instances >= (1ull << 30))
+      throw Fail(GM_ERR_INVALID,
+                 "exact_bf16 and exact_fp8 support n_instances < 2^30");
     if (q.negative_sample_rate < 1)
       throw Fail(GM_ERR_INVALID, "negative_sample_rate must be >= 1");
     if (q.max_epoch > 1000000)
@@ -2032,12 +2107,14 @@ int64_t gm_screen_shapes(char *buf, uint64_t cap) {
       for (const auto &kv : gm::g_tune) {
         const gm::TuneKey &k = kv.first;
         const gm::TuneEntry &t = kv.second;
-        char head[160];
+        char head[200];
         std::snprintf(head, sizeof head,
-                      "%s{\"device\": %d, \"db\": %llu, \"rows_log4\": %d, "
-                      "\"batch_tiles\": %llu, \"best\": %d, \"candidates\": [",
-                      first ? "" : ", ", k.dev, (unsigned long long)k.db,
-                      k.nlog4, (unsigned long long)k.brt, t.best);
+                      "%s{\"device\": %d, \"knn_mode\": %llu, \"db\": %llu, "
+                      "\"rows_log4\": %d, \"batch_tiles\": %llu, \"best\": %d, "
+                      "\"candidates\": [",
+                      first ? "" : ", ", k.dev, (unsigned long long)k.mode,
+                      (unsigned long long)k.db, k.nlog4,
+                      (unsigned long long)k.brt, t.best);
         out += head;
         for (size_t c = 0; c < t.cand.size(); ++c) {
           char item[96];
@@ -2279,6 +2356,32 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
     e->check_barrier();
   });
   gm_destroy(h);
+  return rc;
+}
+
+int gm_debug_e4m3(const float *in, uint8_t *out, uint64_t n,
+                  int32_t on_device) {
+  if (n > 0 && (!in || !out))
+    return GM_ERR_INVALID;
+  if (!on_device) {
+    for (uint64_t i = 0; i < n; ++i)
+      out[i] = gm::e4m3_rn(in[i]);
+    gm::set_last_error(nullptr);
+    return GM_OK;
+  }
+  float *d = nullptr;
+  int rc = guarded([&] {
+    if (n == 0)
+      return;
+    HIPC(hipMalloc(&d, n * 5));
+    HIPC(hipMemcpy(d, in, n * 4, hipMemcpyHostToDevice));
+    uint8_t *o = reinterpret_cast<uint8_t *>(d + n);
+    gm::k_e4m3_probe<<<(unsigned)((n + 255) / 256), 256>>>(d, o, n);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpy(out, o, n, hipMemcpyDeviceToHost));
+  });
+  if (d)
+    (void)hipFree(d);
   return rc;
 }
 

@@ -65,7 +65,22 @@ __global__ void k_exact_merge(const uint64_t *part, int nsplit, uint64_t *pool,
                               uint64_t b0, uint64_t nb, uint8_t *flag,
                               const unsigned *guard);
 
-// mfma_kernels.hip
+// mfma_kernels.hip, fp8_kernels.hip
+// Shared by the BF16 and FP8 screens and the survivor kernels: block tiles
+// of TB batch rows x TC columns; TOPL upper bounds kept per lane and batch
+// row in pass 1 (a lane sees a quarter of a block's columns for a row, so a
+// list of 8 is cut short only when more than 8 of the row's 32 nearest fall
+// among them, and tau is then larger, still valid); PARTW pass-1 values per
+// (batch row, split); survivor list entries: the column row and its flags.
+namespace screen {
+constexpr int TB = 128;
+constexpr int TC = 128;
+constexpr int TOPL = 8;
+constexpr int PARTW = 4 * TOPL;
+constexpr uint32_t LFWD = 0x80000000u; // forward pair
+constexpr uint32_t LREV = 0x40000000u; // reverse pair
+constexpr uint32_t LID = 0x3fffffffu;  // column row
+} // namespace screen
 // feature steps (16 features each) whose products k_bf16_screen sums on the
 // matrix core before one round-to-nearest FP32 add into the running dot
 // product; the screening bound's summation term depends on it
@@ -82,6 +97,22 @@ __global__ void k_bf16_screen(const __bf16 *Xs, const __bf16 *Xb, int nsteps,
                               uint32_t *rlist, unsigned *rcnt, unsigned rcap,
                               unsigned *qctl, float coef, float cfac,
                               float *lbm);
+// fp8_kernels.hip
+// feature steps (64 features each) of k_fp8_screen's groups (as
+// BF16_GROUP_STEPS)
+constexpr int FP8_GROUP_STEPS = 2;
+__global__ void k_fp8_prep(const float *X, uint64_t dp, uint64_t db,
+                           uint8_t *Xs, uint8_t *Xb, float *nrm, float *isc,
+                           uint8_t *flag, unsigned *rcnt, unsigned *qctl,
+                           uint64_t b0, uint64_t nb);
+template <int PASS>
+__global__ void k_fp8_screen(const uint8_t *Xs, const uint8_t *Xb, int nsteps,
+                             const float *nrm, const float *isc,
+                             const float *pmax, const float *tau, uint64_t b0,
+                             uint64_t nb, uint64_t n, int nsplit, int rtiles,
+                             float *part, uint32_t *rlist, unsigned *rcnt,
+                             unsigned rcap, unsigned *qctl, float coef,
+                             float cfac, float *lbm);
 __global__ void k_bf16_lbscan(const float *lbm, uint64_t nb, uint64_t n,
                               uint64_t b0, const float *tau, const float *pmax,
                               uint32_t *rlist, unsigned *rcnt, unsigned rcap,

@@ -49,16 +49,13 @@ namespace {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
 
-constexpr int TB = 128;  // batch rows per block tile
-constexpr int TC = 128;  // columns (earlier rows) per block tile
-// upper bounds kept per lane and batch row: a lane sees a quarter of a
-// block's columns for a row, so a list of 8 is cut short only when more than
-// 8 of the row's 32 nearest fall among them (tau is then larger, still valid)
-constexpr int TOPL = 8;
-constexpr int PARTW = 4 * TOPL; // pass-1 values per (batch row, split)
-constexpr uint32_t LFWD = 0x80000000u; // survivor list entry: forward pair
-constexpr uint32_t LREV = 0x40000000u; // survivor list entry: reverse pair
-constexpr uint32_t LID = 0x3fffffffu;  // survivor list entry: column row
+using screen::LFWD;
+using screen::LID;
+using screen::LREV;
+using screen::PARTW;
+using screen::TB;
+using screen::TC;
+using screen::TOPL;
 // a buffer offset past every buffer's end: the access is dropped
 constexpr uint32_t BUF_OFF = 0x80000000u;
 constexpr int RC = 32;                 // features per re-rank chunk

@@ -61,11 +61,17 @@ def _allocations(n: int, d: int, k: int, c: int, knn_mode: int,
         sizes += [b * _POOL_W * 4, b * 4, b * _CAND * 4, b * _CAND * 4, b * 4]
     else:
         sizes += [b * _MAX_SPLIT * _POOL_W * 8]
-    if knn_mode == 2:
-        db = (d + 15) // 16 * 16
+    if knn_mode in (2, 3):
+        # BF16: 4 bytes per padded feature (hi + lo); FP8: 2, and the
+        # rows' inverse scales
+        db = (d + 15) // 16 * 16 if knn_mode == 2 else (d + 63) // 64 * 64
         rows = (n + 127) // 128 * 128
-        cap = _SURVIVOR_CAP if n <= 1 << 18 else 8 * _SURVIVOR_CAP
-        sizes += [rows * db * 4, rows * 4, 16]            # split rows, norms
+        cap = (_SURVIVOR_CAP if n <= 1 << 18 and knn_mode == 2
+               else 8 * _SURVIVOR_CAP)
+        per = 4 if knn_mode == 2 else 2
+        sizes += [rows * db * per, rows * 4, 16]          # split rows, norms
+        if knn_mode == 3:
+            sizes += [rows * 4]                           # inverse scales
         sizes += [(b + 127) // 128 * 128 * db * 4, b * 4,  # batch operands
                   b * cap * 4, 2 * b * 4, b * cap * 8]
         if n * b * 4 <= _LB_STORE_MAX:
@@ -277,7 +283,8 @@ class Server:
         n = params.n_instances or data_loader.n_instances
         d = params.n_features or data_loader.n_features
         mode = getattr(params, "knn_mode", "exact_bf16")
-        mode = {"nndescent": 0, "exact_fp32": 1, "exact_bf16": 2}.get(mode, mode)
+        mode = {"nndescent": 0, "exact_fp32": 1, "exact_bf16": 2,
+                "exact_fp8": 3}.get(mode, mode)
         fp = session_footprint(n, d, params.n_neighbors, params.n_components,
                                int(mode), self.max_batch_rows)
         self.admission.acquire(fp, admission_timeout)

@@ -41,11 +41,15 @@ typedef enum gm_metric { GM_EUCLIDEAN = 0, GM_COSINE = 1 } gm_metric;
  *   GM_KNN_EXACT_FP32  exact: FP32 distance tiles against every inserted row
  *   GM_KNN_EXACT_BF16  exact: BF16 MFMA tiles screen with a proven error
  *                      bound, survivors re-ranked in FP32 (same graph as
- *                      GM_KNN_EXACT_FP32) */
+ *                      GM_KNN_EXACT_FP32)
+ *   GM_KNN_EXACT_FP8   exact: FP8 E4M3 MFMA tiles (per-row power-of-two
+ *                      scales, hi/lo split) screen with a proven, looser
+ *                      bound; survivors re-ranked in FP32 (same graph) */
 typedef enum gm_knn_mode {
   GM_KNN_NNDESCENT = 0,
   GM_KNN_EXACT_FP32 = 1,
-  GM_KNN_EXACT_BF16 = 2
+  GM_KNN_EXACT_BF16 = 2,
+  GM_KNN_EXACT_FP8 = 3
 } gm_knn_mode;
 
 /* Mirrors gpumap::GPUMAPParams (cpp/include/gpumap/gpumap.cuh:15-36) with
@@ -281,7 +285,8 @@ int gm_get_kernel_stats(gm_handle *h, int32_t which, gm_kernel_stats *out);
 /* The BF16 screen's launch shapes chosen so far by the process's shape
  * tuner.  No counterpart in the reference, whose launch shapes are constants
  * (insertion_kernel.cuh:961-969, update_kernel.cuh:386-396).  Per key
- * (device, padded feature count "db", row count as floor(log4 n) "rows_log4",
+ * (device, screen "knn_mode", padded feature count "db", row count as
+ * floor(log4 n) "rows_log4",
  * 128-row batch tiles "batch_tiles"): the candidates timed so far
  * ({"bsplit", "lbscan", "ms"}, ms < 0 when not timed yet) and the chosen
  * one's index ("best", -1 while tuning), as a NUL-terminated JSON array
@@ -293,6 +298,12 @@ int gm_reset_kernel_stats(gm_handle *h);
 /* Philox4x32-10 of the device code, for known-answer tests. */
 int gm_debug_philox(const uint32_t ctr[4], const uint32_t key[2],
                     uint32_t out[4]);
+
+/* The FP8 screen's float -> E4M3 rounding (round to nearest, ties to even,
+ * saturating at 448) of n values, on the host (on_device 0, no GPU needed)
+ * or in a kernel (on_device 1), for known-answer tests. */
+int gm_debug_e4m3(const float *in, uint8_t *out, uint64_t n,
+                  int32_t on_device);
 
 #ifdef __cplusplus
 }

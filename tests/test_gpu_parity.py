@@ -66,7 +66,7 @@ def test_philox_device_matches_published_vectors():
 
 
 MODES = [pytest.param(0, id="nndescent"), pytest.param(1, id="exact_fp32"),
-         pytest.param(2, id="exact_bf16")]
+         pytest.param(2, id="exact_bf16"), pytest.param(3, id="exact_fp8")]
 
 
 @pytest.mark.parametrize("mode", MODES)
