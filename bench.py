@@ -601,16 +601,19 @@ def strategy_leg(ev, x, xdev, n, d, knn_mode, budget=0.5, max_cycles=2000):
     return out
 
 
-def load_traffic(kernel):
-    """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters,
-    from the committed profile summary (profiles/traffic.json), or None."""
+def load_traffic(kernel, workload, knn_mode):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters
+    on this workload and knn mode, from the committed profile summary
+    (profiles/traffic.json), or None when it was measured on another."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
-        return t.get(kernel, {}).get("hbm_bytes_per_launch")
     except (OSError, ValueError):
         return None
+    if t.get("_workload") != workload or t.get("_knn_mode") != knn_mode:
+        return None
+    return t.get(kernel, {}).get("hbm_bytes_per_launch")
 
 
 def main():
@@ -680,7 +683,7 @@ def main():
         roof = {"kernel": name, "bound": "hbm",
                 "achieved": kd["achieved_GBps"], "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": kd["achieved_GBps"] / (HBM_PEAK / 1e9),
-                "traffic": load_traffic(name),
+                "traffic": load_traffic(name, args.workload, args.knn_mode),
                 "algo_bytes_per_launch": kd["algo_bytes_per_launch"],
                 "avg_launch_ms": kd["avg_ms"]}
     if "insertion" in kern and mode in (MODES["exact_bf16"],

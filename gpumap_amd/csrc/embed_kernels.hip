@@ -503,15 +503,27 @@ __device__ inline void neg_step(float cur[NC], float g[NC], const float neg[NC],
   }
 }
 
+// Where the SGD reads rows: the float embedding, or its fixed-point copy
+// (FixRows, below); load() gives a row's NC coordinates as floats.
+template <int NC> struct EmbRows {
+  const float *__restrict__ emb;
+  __device__ void load(uint64_t row, float out[NC]) const {
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      out[d] = emb[row * NC + d];
+  }
+};
+
 // the arithmetic of sgd_update; the caller advances the edge's schedule
-template <int NC>
-__device__ inline void sgd_apply(const float *__restrict__ emb, int32_t j,
-                                 uint64_t e, uint32_t n, float ep_neg,
-                                 float alpha, float gamma, float a, float b,
-                                 uint64_t seed, float rounding,
-                                 EdgeLoads<NC> &L, float gh[NC], float gt[NC],
-                                 unsigned long long &c_pos,
-                                 unsigned long long &c_neg) {
+template <int NC, typename Rows>
+__device__ inline void sgd_apply_rows(const Rows &rw, int32_t j, uint64_t e,
+                                      uint32_t n, float ep_neg, float alpha,
+                                      float gamma, float a, float b,
+                                      uint64_t seed, float rounding,
+                                      EdgeLoads<NC> &L, float gh[NC],
+                                      float gt[NC],
+                                      unsigned long long &c_pos,
+                                      unsigned long long &c_neg) {
   ++c_pos;
   float g[NC];
   float d2 = 0.0f;
@@ -552,9 +564,7 @@ __device__ inline void sgd_apply(const float *__restrict__ emb, int32_t j,
           continue;
         ++c_neg;
         float neg[NC];
-#pragma unroll
-        for (int d = 0; d < NC; ++d)
-          neg[d] = emb[(uint64_t)r * NC + d];
+        rw.load(r, neg);
         neg_step<NC>(L.cur, g, neg, alpha, gamma, a, b);
       }
     }
@@ -562,6 +572,18 @@ __device__ inline void sgd_apply(const float *__restrict__ emb, int32_t j,
 #pragma unroll
   for (int d = 0; d < NC; ++d)
     gh[d] = (rounding + g[d]) - rounding;
+}
+
+template <int NC>
+__device__ inline void sgd_apply(const float *__restrict__ emb, int32_t j,
+                                 uint64_t e, uint32_t n, float ep_neg,
+                                 float alpha, float gamma, float a, float b,
+                                 uint64_t seed, float rounding,
+                                 EdgeLoads<NC> &L, float gh[NC], float gt[NC],
+                                 unsigned long long &c_pos,
+                                 unsigned long long &c_neg) {
+  sgd_apply_rows<NC>(EmbRows<NC>{emb}, j, e, n, ep_neg, alpha, gamma, a, b,
+                     seed, rounding, L, gh, gt, c_pos, c_neg);
 }
 
 // the edge's schedule after a sample with L.n_neg negative samples
@@ -591,29 +613,33 @@ __device__ inline void sgd_update(const float *__restrict__ emb, int32_t j,
 // past its count, read row 0 (one cache line per wave).  With a fixed number
 // of loads the compiler's waits count exactly the operations issued after
 // them.
-template <int NC>
-__device__ inline void sgd_gather_all(const float *__restrict__ emb, bool act,
-                                      int32_t j, int32_t kk, uint32_t n,
-                                      float ep_neg, float nn, int epoch,
-                                      const uint32_t draw[8],
-                                      EdgeLoads<NC> &L) {
+template <int NC, typename Rows>
+__device__ inline void sgd_gather_rows(const Rows &rw, bool act, int32_t j,
+                                       int32_t kk, uint32_t n, float ep_neg,
+                                       float nn, int epoch,
+                                       const uint32_t draw[8],
+                                       EdgeLoads<NC> &L) {
   L.n_neg = act ? (int)(((float)epoch - nn) / ep_neg) : 0;
   const int cnt = min(8, L.n_neg);
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     L.r[q] = __umulhi(draw[q], n);
     const uint32_t rq = q < cnt ? L.r[q] : 0u;
-    const float *np_ = emb + (uint64_t)rq * NC;
-#pragma unroll
-    for (int d = 0; d < NC; ++d)
-      L.neg[q][d] = np_[d];
+    rw.load(rq, L.neg[q]);
   }
   const uint64_t jj = act ? (uint64_t)j : 0, kq = act ? (uint64_t)kk : 0;
-#pragma unroll
-  for (int d = 0; d < NC; ++d) {
-    L.cur[d] = emb[jj * NC + d];
-    L.oth[d] = emb[kq * NC + d];
-  }
+  rw.load(jj, L.cur);
+  rw.load(kq, L.oth);
+}
+
+template <int NC>
+__device__ inline void sgd_gather_all(const float *__restrict__ emb, bool act,
+                                      int32_t j, int32_t kk, uint32_t n,
+                                      float ep_neg, float nn, int epoch,
+                                      const uint32_t draw[8],
+                                      EdgeLoads<NC> &L) {
+  sgd_gather_rows<NC>(EmbRows<NC>{emb}, act, j, kk, n, ep_neg, nn, epoch,
+                      draw, L);
 }
 
 // Runs of equal row ids over the lanes of a wave (edges in COO order, so a
@@ -860,6 +886,167 @@ __device__ inline void agg_add_buf(__amdgpu_buffer_rsrc_t r, const Agg &a,
   const bool w2 = seg_sum<NC, 0>(a.sh, v2);
   buf_row_add<NC>(r, k1, w1, v1);
   buf_row_add<NC>(r, hh, w2 && !a.paired, v2);
+}
+
+// ------------------------------------------------ fixed-point row updates
+// The cooperative SGD kernels keep each row as a record (FixFrame): its
+// coordinates at the launch's start p0 (float) and the sum D of the
+// launch's updates as integers in units of q, the rounding grid's finest
+// spacing (rounding factor x 2^-24: every contribution is an exact integer
+// multiple of it).  A row reads as p0 + D q, the same value the float
+// embedding holds under float atomics whenever those adds are exact (the
+// grid's purpose); coordinates 2i and 2i + 1 share one 64-bit delta word
+// M = D_2i + 2^32 D_2i+1 (as integers, modulo 2^64), so one 64-bit integer
+// atomic adds to both.  A device-scope atomic costs per active lane: a 2-D
+// or 4-D row update takes half the lanes of one float atomic per
+// coordinate, and integer sums are exact in any order.  k_fix_out adds each
+// row's D q back to the float embedding (rows with D = 0 keep their bits).
+// Record layout: fix_stride<NC> float words, then as many delta words; the
+// low word of a pair's M is D_2i, its high word D_2i+1 - 1 when D_2i < 0
+// (the borrow), which fix_decode adds back.
+template <int NC> constexpr int fix_stride() { return NC == 3 ? 4 : NC; }
+
+template <int NC>
+__device__ inline void fix_decode(const int32_t *__restrict__ dw,
+                                  int32_t D[NC]) {
+  int32_t x[fix_stride<NC>()];
+#pragma unroll
+  for (int d = 0; d < fix_stride<NC>(); ++d)
+    x[d] = dw[d];
+#pragma unroll
+  for (int d = 0; d < NC; ++d)
+    D[d] = (d & 1) ? x[d] + (int32_t)((uint32_t)x[d - 1] >> 31) : x[d];
+}
+
+// a 64-bit word holding lo + 2^32 hi (modulo 2^64)
+__device__ inline unsigned long long fix_pack(int32_t lo, int32_t hi) {
+  return (unsigned long long)(int64_t)lo +
+         ((unsigned long long)(uint32_t)hi << 32);
+}
+
+template <int NC> struct FixRows {
+  const int32_t *__restrict__ rec;
+  float q;
+  __device__ void load(uint64_t row, float out[NC]) const {
+    constexpr int F = fix_stride<NC>();
+    const int32_t *r = rec + row * (2 * F);
+    int32_t D[NC];
+    fix_decode<NC>(r + F, D);
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      out[d] = __int_as_float(r[d]) + (float)D[d] * q;
+  }
+};
+
+// One row's contributions v (multiples of q) into its record: one 64-bit
+// atomic per coordinate pair, a 32-bit one for a lone third coordinate (or
+// the only one).
+template <int NC>
+__device__ inline void fix_row_add(int32_t *__restrict__ rec, float qinv,
+                                   int32_t row, bool wr, const float v[NC]) {
+  if (!(wr && row >= 0))
+    return;
+  constexpr int F = fix_stride<NC>();
+  int32_t *p = rec + (uint64_t)row * (2 * F) + F;
+  int32_t dx[NC];
+#pragma unroll
+  for (int d = 0; d < NC; ++d)
+    dx[d] = (int32_t)rintf(v[d] * qinv);
+#pragma unroll
+  for (int d = 0; d + 1 < NC; d += 2) {
+    const unsigned long long pk = fix_pack(dx[d], dx[d + 1]);
+    if (pk != 0ull)
+      __hip_atomic_fetch_add(reinterpret_cast<unsigned long long *>(p + d),
+                             pk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if constexpr (NC & 1) {
+    if (dx[NC - 1] != 0)
+      __hip_atomic_fetch_add(reinterpret_cast<unsigned *>(p + NC - 1),
+                             (unsigned)dx[NC - 1], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// agg_add_buf's row updates on the records
+template <int NC>
+__device__ inline void agg_add_fix(int32_t *__restrict__ rec, float qinv,
+                                   const Agg &a, int32_t hh, int32_t tt,
+                                   float gh[NC], float gt[NC]) {
+  float v1[NC], v2[NC];
+#pragma unroll
+  for (int d = 0; d < NC; ++d) {
+    const float tot = gh[d] + __shfl_xor(gt[d], 1, 64);
+    v1[d] = a.paired ? tot : gt[d];
+    v2[d] = a.paired ? 0.0f : gh[d];
+  }
+  const Seg s1{a.paired ? a.sp.m : a.st.m, a.paired ? a.sp.start : a.st.start};
+  const int32_t k1 = a.paired ? a.kp : tt;
+  const bool w1 = seg_sum<NC, 0>(s1, v1);
+  const bool w2 = seg_sum<NC, 0>(a.sh, v2);
+  fix_row_add<NC>(rec, qinv, k1, w1, v1);
+  fix_row_add<NC>(rec, qinv, hh, w2 && !a.paired, v2);
+}
+
+// The rows and row updates of one SGD kernel: the float embedding (buffer
+// float atomics) or the records.
+template <int NC> struct FloatStore {
+  EmbRows<NC> rows;
+  __amdgpu_buffer_rsrc_t er;
+  __device__ void add(const Agg &a, int32_t hh, int32_t tt, float gh[NC],
+                      float gt[NC]) const {
+    agg_add_buf<NC>(er, a, hh, tt, gh, gt);
+  }
+};
+template <int NC> struct FixStore {
+  FixRows<NC> rows;
+  int32_t *rec;
+  float qinv;
+  __device__ void add(const Agg &a, int32_t hh, int32_t tt, float gh[NC],
+                      float gt[NC]) const {
+    agg_add_fix<NC>(rec, qinv, a, hh, tt, gh, gt);
+  }
+};
+template <int NC>
+__device__ inline FixStore<NC> fix_store(int32_t *rec, float rounding) {
+  const float q = ldexpf(rounding, -24); // a power of two, as is 1 / q
+  return FixStore<NC>{FixRows<NC>{rec, q}, rec, 1.0f / q};
+}
+
+// every row's record: its coordinates, no updates yet
+template <int NC>
+__global__ __launch_bounds__(256) void k_fix_in(const float *__restrict__ emb,
+                                                uint64_t n, FixFrame fx) {
+  constexpr int F = fix_stride<NC>();
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    int32_t w[2 * F];
+#pragma unroll
+    for (int d = 0; d < 2 * F; ++d)
+      w[d] = d < NC ? __float_as_int(emb[r * NC + d]) : 0;
+    int32_t *p = fx.rec + r * (2 * F);
+#pragma unroll
+    for (int d = 0; d < 2 * F; ++d)
+      p[d] = w[d];
+  }
+}
+
+// each row's updates back into the float embedding: p0 + D q, one rounding
+template <int NC>
+__global__ __launch_bounds__(256) void k_fix_out(float *__restrict__ emb,
+                                                 uint64_t n, FixFrame fx,
+                                                 const float *__restrict__ rounding_p) {
+  constexpr int F = fix_stride<NC>();
+  const float q = ldexpf(*rounding_p, -24);
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const int32_t *p = fx.rec + r * (2 * F);
+    int32_t D[NC];
+    fix_decode<NC>(p + F, D);
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      if (D[d] != 0)
+        emb[r * NC + d] = __int_as_float(p[d]) + (float)D[d] * q;
+  }
 }
 
 __device__ inline void flush_sgd_stats(unsigned long long *stats,
@@ -1114,9 +1301,9 @@ __device__ inline unsigned slot_min(const unsigned sv[8], unsigned nw,
 // (*timeout is set, as epoch_end's bar[2]).
 constexpr int SGD_POLL = 4;
 
-template <int NC>
+template <int NC, typename Store>
 __device__ __attribute__((always_inline)) inline void sgd_waves_body(
-    float *__restrict__ emb, const int32_t *__restrict__ head,
+    const Store &st, const int32_t *__restrict__ head,
     const int32_t *__restrict__ tail, uint64_t nnz, uint32_t n,
     const float *__restrict__ eps, int n_epochs, float alpha0, float gamma,
     float a, float b, float nsr_inv, uint64_t seed, float rounding,
@@ -1141,7 +1328,6 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
     nn = 0.0f;
   }
   const Agg ag = agg_of(hh, tt);
-  const __amdgpu_buffer_rsrc_t er = emb_rsrc<NC>(emb, n);
   const float ep_neg = ep * nsr_inv;
   unsigned long long c_pos = 0, c_neg = 0;
   float alpha = alpha0;
@@ -1156,14 +1342,14 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
   for (int d = 0; d < NC; ++d)
     own1_h[d] = own1_t[d] = own2_h[d] = own2_t[d] = pgh[d] = pgt[d] = 0.0f;
   bool act0 = !(nx > 0.0f), act1 = false;
-  sgd_gather_all<NC>(emb, act0, hh, tt, n, ep_neg, nn, 0, dr, L0);
+  sgd_gather_rows<NC>(st.rows, act0, hh, tt, n, ep_neg, nn, 0, dr, L0);
   if (act0)
     advance_schedule(ep, ep_neg, L0.n_neg, nx, nn);
   neg_draws(seed + 1, e, dr);
   // what a step leaves in flight after its gathers, so that the loop is
   // entered with the same pending operations as it repeats with: the row
   // updates (zero here, dropped) and the arrival
-  agg_add_buf<NC>(er, ag, hh, tt, pgh, pgt);
+  st.add(ag, hh, tt, pgh, pgt);
   __hip_atomic_store(slots + w, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   unsigned smin = 0; // smallest slot at the last read taken in
   unsigned sv[8];    // a read of the slots in flight
@@ -1182,11 +1368,12 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
     // the next epoch's gathers (the last epoch issues them for nothing:
     // every lane reads row 0)
     an = epoch + 1 < n_epochs && !(nx > (float)(epoch + 1));
-    sgd_gather_all<NC>(emb, an, hh, tt, n, ep_neg, nn, epoch + 1, dr, Ln);
+    sgd_gather_rows<NC>(st.rows, an, hh, tt, n, ep_neg, nn, epoch + 1, dr,
+                        Ln);
     if (an)
       advance_schedule(ep, ep_neg, Ln.n_neg, nx, nn);
     // the previous epoch's row updates (zero contributions at epoch 0)
-    agg_add_buf<NC>(er, ag, hh, tt, pgh, pgt);
+    st.add(ag, hh, tt, pgh, pgt);
     if constexpr (decltype(read)::value)
       poll_slots(slots, nw, lane, sv);
     neg_draws(es + 2, e, dr);
@@ -1210,8 +1397,8 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
     for (int d = 0; d < NC; ++d)
       __asm__ volatile("" ::"v"(Lc.cur[d]), "v"(Lc.oth[d]));
     if (ac)
-      sgd_apply<NC>(emb, hh, e, n, ep_neg, alpha, gamma, a, b, es, rounding,
-                    Lc, gh, gt, c_pos, c_neg);
+      sgd_apply_rows<NC>(st.rows, hh, e, n, ep_neg, alpha, gamma, a, b, es,
+                         rounding, Lc, gh, gt, c_pos, c_neg);
     // this epoch's share of the pair's rows (paired: the head row gets the
     // edge's head and the twin's tail contributions, the tail row the
     // twin's total)
@@ -1274,7 +1461,7 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
   if (epoch + 2 < n_epochs)
     step(epoch + 2, L0, act0, L1, act1, P0{}, P0{});
   // the last epoch's row updates and arrival
-  agg_add_buf<NC>(er, ag, hh, tt, pgh, pgt);
+  st.add(ag, hh, tt, pgh, pgt);
   __hip_atomic_store(slots + w, (unsigned)n_epochs, __ATOMIC_RELAXED,
                      __HIP_MEMORY_SCOPE_AGENT);
   flush_sgd_stats(stats, c_pos, c_neg);
@@ -1386,6 +1573,8 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body_batch(
 }
 
 
+// On the records when the frame has them (k_fix_in made them), else on the
+// float embedding.
 template <int NC>
 __global__ __launch_bounds__(64) void k_sgd_waves(
     float *__restrict__ emb, const int32_t *__restrict__ head,
@@ -1393,10 +1582,18 @@ __global__ __launch_bounds__(64) void k_sgd_waves(
     uint32_t n, const float *__restrict__ eps, int n_epochs, float alpha0,
     float gamma, float a, float b, float nsr_inv, uint64_t seed,
     const float *__restrict__ rounding_p, unsigned *__restrict__ bar,
-    unsigned long long *__restrict__ stats, int lag) {
-  sgd_waves_body<NC>(emb, head, tail, *nnz_p, n, eps, n_epochs, alpha0, gamma,
-                     a, b, nsr_inv, seed, *rounding_p, bar + 4, gridDim.x,
-                     blockIdx.x, bar + 2, stats, lag);
+    unsigned long long *__restrict__ stats, int lag, FixFrame fx) {
+  if (fx.rec) {
+    const FixStore<NC> st = fix_store<NC>(fx.rec, *rounding_p);
+    sgd_waves_body<NC>(st, head, tail, *nnz_p, n, eps, n_epochs, alpha0,
+                       gamma, a, b, nsr_inv, seed, *rounding_p, bar + 4,
+                       gridDim.x, blockIdx.x, bar + 2, stats, lag);
+  } else {
+    const FloatStore<NC> st{EmbRows<NC>{emb}, emb_rsrc<NC>(emb, n)};
+    sgd_waves_body<NC>(st, head, tail, *nnz_p, n, eps, n_epochs, alpha0,
+                       gamma, a, b, nsr_inv, seed, *rounding_p, bar + 4,
+                       gridDim.x, blockIdx.x, bar + 2, stats, lag);
+  }
 }
 
 // Block b runs wave b - job.wave0 of the job whose wave range holds b; the
@@ -1454,9 +1651,9 @@ __device__ inline SlotSched slot_load(const int32_t *__restrict__ head,
 // fixed set of operations, so that waiting for a slot's schedule waits for
 // nothing issued later.  Each slot then costs one memory round trip (its
 // gathers) instead of four dependent ones.
-template <int NC>
-__global__ __launch_bounds__(1024) void k_sgd_stream(
-    float *__restrict__ emb, const int32_t *__restrict__ head,
+template <int NC, typename Store>
+__device__ __attribute__((always_inline)) inline void sgd_stream_body(
+    const Store &st, const int32_t *__restrict__ head,
     const int32_t *__restrict__ tail, const uint32_t *__restrict__ nnz_p,
     uint32_t n, const float *__restrict__ eps, float *__restrict__ nxt,
     float *__restrict__ nneg, int n_epochs, float alpha0, float gamma, float a,
@@ -1472,7 +1669,6 @@ __global__ __launch_bounds__(1024) void k_sgd_stream(
   // below are uniform
   const uint64_t w0 = (uint64_t)blockIdx.x * blockDim.x +
                       (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x - lane);
-  const __amdgpu_buffer_rsrc_t er = emb_rsrc<NC>(emb, n);
   unsigned long long c_pos = 0, c_neg = 0;
   unsigned seen = 0;
   float alpha = alpha0;
@@ -1497,15 +1693,16 @@ __global__ __launch_bounds__(1024) void k_sgd_stream(
         uint32_t dr[8];
         neg_draws(es, e, dr);
         EdgeLoads<NC> L;
-        sgd_gather_all<NC>(emb, act, hv, tv, n, ep_neg, cur.nn, epoch, dr, L);
+        sgd_gather_rows<NC>(st.rows, act, hv, tv, n, ep_neg, cur.nn, epoch, dr,
+                            L);
         float gh[NC], gt[NC];
 #pragma unroll
         for (int d = 0; d < NC; ++d)
           gh[d] = gt[d] = 0.0f;
         float nx = cur.nx, nn = cur.nn;
         if (act) {
-          sgd_apply<NC>(emb, hv, e, n, ep_neg, alpha, gamma, a, b, es,
-                        rounding, L, gh, gt, c_pos, c_neg);
+          sgd_apply_rows<NC>(st.rows, hv, e, n, ep_neg, alpha, gamma, a, b,
+                             es, rounding, L, gh, gt, c_pos, c_neg);
           advance_schedule(cur.ep, ep_neg, L.n_neg, nx, nn);
         }
         // the sampled slots' schedules back, the others' stores dropped
@@ -1518,7 +1715,7 @@ __global__ __launch_bounds__(1024) void k_sgd_stream(
             __float_as_uint(nn),
             __builtin_amdgcn_make_buffer_rsrc(nneg + base, 0, 256, 0x00020000),
             (int)so, 0, 0);
-        agg_add_buf<NC>(er, agg_of(hv, tv), hv, tv, gh, gt);
+        st.add(agg_of(hv, tv), hv, tv, gh, gt);
         cur = nxs;
       }
     }
@@ -1526,6 +1723,28 @@ __global__ __launch_bounds__(1024) void k_sgd_stream(
     epoch_end(bar, gridDim.x, epoch, lag, peeked, seen);
   }
   flush_sgd_stats(stats, c_pos, c_neg);
+}
+
+template <int NC>
+__global__ __launch_bounds__(1024) void k_sgd_stream(
+    float *__restrict__ emb, const int32_t *__restrict__ head,
+    const int32_t *__restrict__ tail, const uint32_t *__restrict__ nnz_p,
+    uint32_t n, const float *__restrict__ eps, float *__restrict__ nxt,
+    float *__restrict__ nneg, int n_epochs, float alpha0, float gamma, float a,
+    float b, float nsr_inv, uint64_t seed, const float *__restrict__ rounding_p,
+    unsigned *__restrict__ bar, unsigned long long *__restrict__ stats,
+    int lag, FixFrame fx) {
+  if (fx.rec) {
+    const FixStore<NC> st = fix_store<NC>(fx.rec, *rounding_p);
+    sgd_stream_body<NC>(st, head, tail, nnz_p, n, eps, nxt, nneg, n_epochs,
+                        alpha0, gamma, a, b, nsr_inv, seed, rounding_p, bar,
+                        stats, lag);
+  } else {
+    const FloatStore<NC> st{EmbRows<NC>{emb}, emb_rsrc<NC>(emb, n)};
+    sgd_stream_body<NC>(st, head, tail, nnz_p, n, eps, nxt, nneg, n_epochs,
+                        alpha0, gamma, a, b, nsr_inv, seed, rounding_p, bar,
+                        stats, lag);
+  }
 }
 
 // Host launchers of the cooperative SGD kernels, in the kernels' own
@@ -1575,7 +1794,7 @@ static hipError_t try_waves(float *emb, const int32_t *head,
                             float nsr_inv, uint64_t seed,
                             const float *rounding_p, unsigned *bar,
                             unsigned long long *stats, int lag, uint64_t nnz,
-                            int num_cu, hipStream_t s) {
+                            int num_cu, hipStream_t s, FixFrame fx) {
   static int per_cu = 0;
   if (per_cu <= 0)
     per_cu = blocks_per_cu(&k_sgd_waves<NC>, 64);
@@ -1585,7 +1804,7 @@ static hipError_t try_waves(float *emb, const int32_t *head,
     return hipErrorInvalidConfiguration;
   void *args[] = {&emb,     &head,   &tail,  &nnz_p, &n,       &eps,
                   &n_epochs, &alpha0, &gamma, &a,     &b,       &nsr_inv,
-                  &seed,    &rounding_p, &bar, &stats, &lag};
+                  &seed,    &rounding_p, &bar, &stats, &lag,    &fx};
   return hipLaunchCooperativeKernel(
       reinterpret_cast<const void *>(&k_sgd_waves<NC>), dim3((unsigned)need),
       dim3(64), args, 0, s);
@@ -1599,7 +1818,8 @@ static hipError_t try_stream(float *emb, const int32_t *head,
                              float gamma, float a, float b, float nsr_inv,
                              uint64_t seed, const float *rounding_p,
                              unsigned *bar, unsigned long long *stats, int lag,
-                             uint64_t nnz, int num_cu, hipStream_t s) {
+                             uint64_t nnz, int num_cu, hipStream_t s,
+                             FixFrame fx) {
   static int per_cu = 0;
   if (per_cu <= 0)
     per_cu = blocks_per_cu(&k_sgd_stream<NC>, 1024);
@@ -1613,7 +1833,7 @@ static hipError_t try_stream(float *emb, const int32_t *head,
   void *args[] = {&emb,   &head,    &tail,    &nnz_p, &n,  &eps,
                   &nxt,   &nneg,    &n_epochs, &alpha0, &gamma, &a,
                   &b,     &nsr_inv, &seed,    &rounding_p, &bar, &stats,
-                  &lag};
+                  &lag,   &fx};
   return hipLaunchCooperativeKernel(
       reinterpret_cast<const void *>(&k_sgd_stream<NC>),
       dim3((unsigned)blocks), dim3(1024), args, 0, s);
@@ -1629,19 +1849,19 @@ static hipError_t try_fused_any(float *emb, const int32_t *head,
                                 unsigned *bar, unsigned long long *stats,
                                 int lag, uint64_t nnz, int num_cu,
                                 hipStream_t s, int *variant,
-                                bool force_stream) {
+                                bool force_stream, FixFrame fx) {
   *variant = 1;
   if (force_stream)
     return try_stream<NC>(emb, head, tail, nnz_p, n, eps, nxt, nneg, n_epochs,
                           alpha0, gamma, a, b, nsr_inv, seed, rounding_p, bar,
-                          stats, lag, nnz, num_cu, s);
+                          stats, lag, nnz, num_cu, s, fx);
   *variant = 2;
   {
     // small edge sets: per-wave arrival slots, twice the counter kernels' lag
     const hipError_t e =
         try_waves<NC>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0, gamma,
                       a, b, nsr_inv, seed, rounding_p, bar, stats, 2 * lag,
-                      nnz, num_cu, s);
+                      nnz, num_cu, s, fx);
     if (e != hipErrorInvalidConfiguration)
       return e;
   }
@@ -1660,7 +1880,7 @@ static hipError_t try_fused_any(float *emb, const int32_t *head,
   *variant = 1;
   return try_stream<NC>(emb, head, tail, nnz_p, n, eps, nxt, nneg, n_epochs,
                         alpha0, gamma, a, b, nsr_inv, seed, rounding_p, bar,
-                        stats, lag, nnz, num_cu, s);
+                        stats, lag, nnz, num_cu, s, fx);
 }
 
 template <int NC>
@@ -1702,29 +1922,63 @@ hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
                             uint64_t seed, const float *rounding_p,
                             unsigned *bar, unsigned long long *stats, int lag,
                             uint64_t nnz, int num_cu, hipStream_t s,
-                            int *variant, bool force_stream) {
+                            int *variant, bool force_stream, FixFrame fx) {
   switch (nc) {
   case 1:
     return try_fused_any<1>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
                             n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
                             rounding_p, bar, stats, lag, nnz, num_cu, s, variant,
-                            force_stream);
+                            force_stream, fx);
   case 2:
     return try_fused_any<2>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
                             n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
                             rounding_p, bar, stats, lag, nnz, num_cu, s, variant,
-                            force_stream);
+                            force_stream, fx);
   case 3:
     return try_fused_any<3>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
                             n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
                             rounding_p, bar, stats, lag, nnz, num_cu, s, variant,
-                            force_stream);
+                            force_stream, fx);
   default:
     return try_fused_any<4>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
                             n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
                             rounding_p, bar, stats, lag, nnz, num_cu, s, variant,
-                            force_stream);
+                            force_stream, fx);
   }
+}
+
+// The records around a cooperative SGD launch: phase 0 (before) makes
+// them, phase 1 (after a kernel that used them) adds the updates back into
+// the float embedding.
+template <int NC>
+static void fix_phase(int phase, float *emb, uint64_t n, FixFrame fx,
+                      const float *rounding_p, int num_cu, hipStream_t s) {
+  const unsigned grid = (unsigned)std::max<uint64_t>(
+      1, std::min<uint64_t>((n + 255) / 256, 8 * (uint64_t)num_cu));
+  if (phase == 0)
+    k_fix_in<NC><<<grid, 256, 0, s>>>(emb, n, fx);
+  else
+    k_fix_out<NC><<<grid, 256, 0, s>>>(emb, n, fx, rounding_p);
+}
+
+hipError_t launch_fix(int nc, int phase, float *emb, uint64_t n, FixFrame fx,
+                      const float *rounding_p, int num_cu, hipStream_t s) {
+  if (!fx.rec || n == 0)
+    return hipSuccess;
+  switch (nc) {
+  case 1:
+    fix_phase<1>(phase, emb, n, fx, rounding_p, num_cu, s);
+    break;
+  case 2:
+    fix_phase<2>(phase, emb, n, fx, rounding_p, num_cu, s);
+    break;
+  case 3:
+    fix_phase<3>(phase, emb, n, fx, rounding_p, num_cu, s);
+    break;
+  default:
+    fix_phase<4>(phase, emb, n, fx, rounding_p, num_cu, s);
+  }
+  return hipGetLastError();
 }
 
 #define GM_SGD_FUSED(NC, EPT)                                                  \

@@ -196,14 +196,22 @@ bool sgd_launches(int c, float *e, const int32_t *head, const int32_t *tail,
                   float gamma, float a, float b, float nsr_inv, uint64_t seed,
                   const float *rounding, unsigned *gbar,
                   unsigned long long *st, uint64_t cap, int num_cu,
-                  hipStream_t s, int sgd_mode, int *variant, bool verbose) {
+                  hipStream_t s, int sgd_mode, int *variant, bool verbose,
+                  FixFrame fx) {
   if (sgd_mode != SGD_PER_EPOCH) {
+    // the per-wave and streaming kernels run on the fixed-point copy (made
+    // here, folded back below when one of them ran)
+    if (fx.rec)
+      HIPC(launch_fix(c, 0, e, nrows, fx, rounding, num_cu, s));
     const hipError_t fe = launch_sgd_fused(
         c, e, head, tail, nnz_p, nrows, eps, nxt, nneg, n_epochs, alpha0,
         gamma, a, b, nsr_inv, seed, rounding, gbar, st, SGD_LAG, cap, num_cu,
-        s, variant, sgd_mode == SGD_STREAM);
-    if (fe == hipSuccess)
+        s, variant, sgd_mode == SGD_STREAM, fx);
+    if (fe == hipSuccess) {
+      if (fx.rec && *variant != 0)
+        HIPC(launch_fix(c, 1, e, nrows, fx, rounding, num_cu, s));
       return true;
+    }
     // too many edges for the resident grid, or cooperative launch not
     // available in this context: one launch per epoch instead
     (void)hipGetLastError();
@@ -369,6 +377,11 @@ struct Engine {
   uint32_t *pidx = nullptr;
   float *pdist = nullptr;
   float *emb = nullptr;
+  // the cooperative SGD kernels' row records (FixFrame): N records of
+  // 2 (c, or 4 for c = 3) words; GPUMAP_AMD_SGD_FIX=0 keeps the kernels on
+  // the float embedding
+  int32_t *fxp = nullptr;
+  bool sgd_fix = true;
   uint8_t *flag = nullptr;
   uint32_t *list = nullptr;
   float *pmax = nullptr;      // per-row pool maximum distance (inf if not full)
@@ -489,6 +502,8 @@ struct Engine {
     update_queue.in.assign(N, 0);
     if (const char *v = std::getenv("GPUMAP_AMD_LBM_MAX_BYTES"))
       lbm_max_bytes = std::strtoull(v, nullptr, 10);
+    if (const char *v = std::getenv("GPUMAP_AMD_SGD_FIX"))
+      sgd_fix = v[0] != '0';
     try {
       allocate();
     } catch (...) {
@@ -519,6 +534,7 @@ struct Engine {
     pdist = dalloc<float>(nk);
     HIPC(hipMemsetD32Async((hipDeviceptr_t)pdist, 0x7f7fffff, nk, s));
     emb = dalloc<float>(N * c);
+    fxp = dalloc<int32_t>(N * 2 * (c == 3 ? 4 : c));
     HIPC(hipMemsetAsync(emb, 0, N * c * sizeof(float), s));
     flag = dalloc<uint8_t>(N);
     HIPC(hipMemsetAsync(flag, 0, N, s));
@@ -662,7 +678,7 @@ struct Engine {
         (void **)&wflags, (void **)&head,    (void **)&tail,    (void **)&wout,
         (void **)&eps,    (void **)&nxt,     (void **)&nneg,    (void **)&snap,
         (void **)&rounding, (void **)&deg,   (void **)&maxdeg,  (void **)&dbatch,
-        (void **)&bmark,  (void **)&isc};
+        (void **)&bmark,  (void **)&isc,     (void **)&fxp};
     for (void **q : ptrs) {
       if (*q)
         (void)hipFree(*q);
@@ -1298,7 +1314,8 @@ struct Engine {
     const bool fused = sgd_launches(
         c, e, head, tail, scal + S_NNZ, (uint32_t)nrows, eps, nxt, nneg,
         n_epochs, alpha0, gamma, a, b, nsr_inv, seed, rounding, gbar, st, cap,
-        num_cu, s, sgd_mode, &sgd_variant, p.verbose != 0);
+        num_cu, s, sgd_mode, &sgd_variant, p.verbose != 0,
+        FixFrame{sgd_fix ? fxp : nullptr});
     if (fused) {
       fused_used = true;
       if (sgd_variant == 2)

@@ -208,10 +208,18 @@ struct SgdBatch { // passed by value as a kernel argument
 hipError_t launch_sgd_batch(int nc, const SgdBatch &batch, unsigned waves,
                             unsigned *bar, int lag, int num_cu,
                             hipStream_t s);
+// The cooperative SGD kernels' row records (embed_kernels.hip): n records
+// of 2 F words (F = NC, or 4 for NC = 3): the row's coordinates at the
+// launch's start, then its updates as integer multiples of the rounding
+// grid.  rec == nullptr: the kernels update the float embedding.
+struct FixFrame {
+  int32_t *rec;
+};
 // cooperative launch of k_sgd_waves (*variant 2), k_sgd_fused (0) or, for
 // edge sets larger than its resident grid or with force_stream,
 // k_sgd_stream (1, counting its work at stats + 8) (embed_kernels.hip);
-// returns the launch error when none can be launched
+// waves and stream run on fx's copy when it has one; returns the launch
+// error when none can be launched
 hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
                             const int32_t *tail, const uint32_t *nnz_p,
                             uint32_t n, const float *eps, float *nxt,
@@ -220,7 +228,11 @@ hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
                             uint64_t seed, const float *rounding_p,
                             unsigned *bar, unsigned long long *stats, int lag,
                             uint64_t nnz, int num_cu, hipStream_t s,
-                            int *variant, bool force_stream = false);
+                            int *variant, bool force_stream, FixFrame fx);
+// the fixed-point copy's kernels around a cooperative SGD launch (phase 0
+// before it, phase 1 after a kernel that used the copy)
+hipError_t launch_fix(int nc, int phase, float *emb, uint64_t n, FixFrame fx,
+                      const float *rounding_p, int num_cu, hipStream_t s);
 // engine.hip: holds the process's device lock exclusively (no session's
 // device work is in flight meanwhile) and the cooperative-launch mutex
 struct ExclusiveDevice {
@@ -242,7 +254,7 @@ bool sgd_launches(int c, float *e, const int32_t *head, const int32_t *tail,
                   float gamma, float a, float b, float nsr_inv, uint64_t seed,
                   const float *rounding, unsigned *gbar,
                   unsigned long long *st, uint64_t cap, int num_cu,
-                  hipStream_t s, int sgd_mode, int *variant, bool verbose);
+                  hipStream_t s, int sgd_mode, int *variant, bool verbose, FixFrame fx);
 __global__ void k_reposition(const uint32_t *pidx, const float *pdist,
                              uint64_t n, int k, int euclid, const double *sigma,
                              const double *rho, const double *mean,

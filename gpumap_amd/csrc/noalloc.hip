@@ -4,9 +4,9 @@
 // loops, harnesses that forbid hipMalloc.  Inputs and outputs are the
 // caller's device arrays; every buffer the kernels need beyond them (the
 // per-edge schedule, head degrees, the rounding grid, the cooperative
-// kernels' barrier words and wave slots, counts and flags) lives in the
-// file-scope device array `scratch`, of which each device holds one copy
-// for the life of the process.  Nothing here calls hipMalloc, hipFree or an
+// kernels' barrier words and wave slots, their row records, counts and
+// flags) lives in the file-scope device array `scratch`, of which each
+// device holds one copy for the life of the process.  Nothing here calls hipMalloc, hipFree or an
 // allocating library routine.
 #include "../../include/gpumap_c.h"
 #include "kernels.hpp"
@@ -21,14 +21,16 @@ namespace gm {
 namespace {
 
 // scratch layout (bytes): eps, nxt, nneg (4 B per edge each), deg (4 B per
-// row), then the barrier words and wave slots, then the control words
+// row), the SGD's row records (32 B per row), then the barrier words and
+// wave slots, then the control words
 constexpr uint64_t NA_EDGES = GM_SGD_NOALLOC_MAX_EDGES;
 constexpr uint64_t NA_ROWS = GM_SGD_NOALLOC_MAX_ROWS;
 constexpr uint64_t OFF_EPS = 0;
 constexpr uint64_t OFF_NXT = OFF_EPS + 4 * NA_EDGES;
 constexpr uint64_t OFF_NNEG = OFF_NXT + 4 * NA_EDGES;
 constexpr uint64_t OFF_DEG = OFF_NNEG + 4 * NA_EDGES;
-constexpr uint64_t OFF_BAR = OFF_DEG + 4 * NA_ROWS;
+constexpr uint64_t OFF_FXP = OFF_DEG + 4 * NA_ROWS;
+constexpr uint64_t OFF_BAR = OFF_FXP + 32 * NA_ROWS;
 constexpr uint64_t BAR_WORDS = 4 + SGD_MAX_WAVES;
 constexpr uint64_t OFF_CTL = (OFF_BAR + 4 * BAR_WORDS + 255) / 256 * 256;
 // control words: [0] edge count, [1] maximum weight (float bits), [2] an
@@ -159,6 +161,7 @@ extern "C" int gm_sgd_noalloc(float *emb, uint64_t n, int32_t c,
     uint32_t *ctl = reinterpret_cast<uint32_t *>(base + OFF_CTL);
     float *rounding = reinterpret_cast<float *>(ctl + 3);
     int *maxdeg = reinterpret_cast<int *>(ctl + 4);
+    const FixFrame fx{reinterpret_cast<int32_t *>(base + OFF_FXP)};
     hipStream_t s = nullptr; // the null stream
     // no session's kernels (or another call's) run on the device meanwhile:
     // the cooperative kernels need every CU, and the scratch is shared
@@ -199,7 +202,7 @@ extern "C" int gm_sgd_noalloc(float *emb, uint64_t n, int32_t c,
         per_epoch == 1   ? SGD_PER_EPOCH
         : per_epoch == 2 ? SGD_STREAM
                          : SGD_AUTO,
-        &variant, false);
+        &variant, false, fx);
     uint32_t timeout = 0;
     if (coop)
       na_check(hipMemcpyAsync(&timeout, bar + 2, 4, hipMemcpyDeviceToHost, s),

@@ -45,6 +45,7 @@ def _allocations(n: int, d: int, k: int, c: int, knn_mode: int,
              n * _POOL_W * 8,                 # pools
              nk * 4, nk * 4,                  # public ids, distances
              n * c * 4, n * c * 4,            # embedding, its snapshot
+             n * (4 if c == 3 else c) * 8,    # the SGD's row records
              n, n, n * 4, n * 4, n * 4,       # flags, marks, list, pmax, deg
              (max(2 * nk, n) // 1024 + 2) * 4,
              n * _POOL_W * 4, n * _POOL_W * 4,        # sampled lists

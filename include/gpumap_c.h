@@ -222,9 +222,9 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
 /* gm_debug_sgd without any device allocation: the same SGD with the same
  * kernels and launch shapes, on the caller's device arrays (emb n x c float32,
  * updated in place; head, tail int32 and w float32 of nnz COO slots, all in
- * device memory), with the schedule, head degrees, rounding grid and the
- * cooperative kernels' barrier words in one static device array per device
- * (about 16 MiB, held until the process ends).  No counterpart in the
+ * device memory), with the schedule, head degrees, rounding grid, the
+ * cooperative kernels' barrier words and row records in one static device
+ * array per device (about 48 MiB, held until the process ends).  No counterpart in the
  * reference; it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Refuses (GM_ERR_INVALID) n above
