@@ -1012,6 +1012,36 @@ __device__ inline FixStore<NC> fix_store(int32_t *rec, float rounding) {
   return FixStore<NC>{FixRows<NC>{rec, q}, rec, 1.0f / q};
 }
 
+// The batched kernel's store for the float embedding: row updates by the
+// branching seg_add / seg2_add (agg_add), which issue only the atomics a
+// wave needs.
+template <int NC> struct FloatStoreBranchy {
+  EmbRows<NC> rows;
+  float *emb;
+  __device__ void add(const Agg &a, int32_t hh, int32_t tt, float gh[NC],
+                      float gt[NC]) const {
+    agg_add<NC>(emb, a, hh, tt, gh, gt);
+  }
+};
+
+// sgd_gather on any row store: only the samples' loads, in a branch
+template <int NC, typename Rows>
+__device__ inline void sgd_gather_r(const Rows &rw, int32_t j, int32_t kk,
+                                    uint32_t n, float ep_neg, float nn,
+                                    int epoch, const uint32_t draw[8],
+                                    EdgeLoads<NC> &L) {
+  L.n_neg = (int)(((float)epoch - nn) / ep_neg);
+  const int cnt = min(8, L.n_neg);
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (q < cnt) {
+      L.r[q] = __umulhi(draw[q], n);
+      rw.load(L.r[q], L.neg[q]);
+    }
+  rw.load((uint64_t)j, L.cur);
+  rw.load((uint64_t)kk, L.oth);
+}
+
 // every row's record: its coordinates, no updates yet
 template <int NC>
 __global__ __launch_bounds__(256) void k_fix_in(const float *__restrict__ emb,
@@ -1475,9 +1505,9 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
 // samples), so each epoch issues only the memory operations it needs:
 // the slot reads and the sampling lanes' gathers at the start of the epoch,
 // the row updates (global atomics, seg_add / seg2_add) at its end.
-template <int NC>
+template <int NC, typename Store>
 __device__ __attribute__((always_inline)) inline void sgd_waves_body_batch(
-    float *__restrict__ emb, const int32_t *__restrict__ head,
+    const Store &st, const int32_t *__restrict__ head,
     const int32_t *__restrict__ tail, uint64_t nnz, uint32_t n,
     const float *__restrict__ eps, int n_epochs, float alpha0, float gamma,
     float a, float b, float nsr_inv, uint64_t seed, float rounding,
@@ -1519,16 +1549,18 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body_batch(
     const float ep_neg = ep * nsr_inv;
     EdgeLoads<NC> L;
     if (act)
-      sgd_gather<NC>(emb, hh, tt, n, ep_neg, nn, epoch, dr, L);
+      sgd_gather_r<NC>(st.rows, hh, tt, n, ep_neg, nn, epoch, dr, L);
     neg_draws(es + 1, e, dr);
     float gh[NC], gt[NC];
 #pragma unroll
     for (int d = 0; d < NC; ++d)
       gh[d] = gt[d] = 0.0f;
-    if (act)
-      sgd_update<NC>(emb, hh, e, n, ep, nx, nn, ep_neg, alpha, gamma, a, b,
-                     es, rounding, L, gh, gt, c_pos, c_neg);
-    agg_add<NC>(emb, ag, hh, tt, gh, gt);
+    if (act) {
+      sgd_apply_rows<NC>(st.rows, hh, e, n, ep_neg, alpha, gamma, a, b, es,
+                         rounding, L, gh, gt, c_pos, c_neg);
+      advance_schedule(ep, ep_neg, L.n_neg, nx, nn);
+    }
+    st.add(ag, hh, tt, gh, gt);
     alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
     // boundary
     if (lane == 0)
@@ -1610,10 +1642,21 @@ __global__ __launch_bounds__(64) void k_sgd_waves_multi(SgdBatch batch,
   const SgdJob &J = batch.job[j];
   if (b >= J.wave0 + J.nwaves)
     return; // not part of any job (never launched so)
-  sgd_waves_body_batch<NC>(J.emb, J.head, J.tail, *J.nnz_p, J.n, J.eps, J.n_epochs,
-                     J.alpha0, J.gamma, J.a, J.b, J.nsr_inv, J.seed,
-                     *J.rounding_p, bar + 4 + J.wave0, J.nwaves,
-                     b - J.wave0, bar + 2, J.stats, lag);
+  if (J.rec) {
+    const FixStore<NC> st = fix_store<NC>(J.rec, *J.rounding_p);
+    sgd_waves_body_batch<NC>(st, J.head, J.tail, *J.nnz_p, J.n, J.eps,
+                             J.n_epochs, J.alpha0, J.gamma, J.a, J.b,
+                             J.nsr_inv, J.seed, *J.rounding_p,
+                             bar + 4 + J.wave0, J.nwaves, b - J.wave0,
+                             bar + 2, J.stats, lag);
+  } else {
+    const FloatStoreBranchy<NC> st{EmbRows<NC>{J.emb}, J.emb};
+    sgd_waves_body_batch<NC>(st, J.head, J.tail, *J.nnz_p, J.n, J.eps,
+                             J.n_epochs, J.alpha0, J.gamma, J.a, J.b,
+                             J.nsr_inv, J.seed, *J.rounding_p,
+                             bar + 4 + J.wave0, J.nwaves, b - J.wave0,
+                             bar + 2, J.stats, lag);
+  }
 }
 
 // A slot's schedule as the streaming kernel reads it: all five fields in one

@@ -1416,7 +1416,7 @@ struct Engine {
                     scal + S_NNZ, eps,     rounding,
                     prof ? kstats : nullptr, 42ull, (uint32_t)n,
                     (int)p.max_epoch, p.alpha, 1.0f, p.a, p.b, nsr_inv, 0u,
-                    (unsigned)waves};
+                    (unsigned)waves, sgd_fix ? fxp : nullptr};
       return (uint32_t)nb;
     }
     if (before_own_sgd)
@@ -1677,6 +1677,13 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
       dones.v.emplace_back(es[i], ev);
     }
     HIPC(hipMemsetAsync(bar, 0, (4 + (size_t)waves) * sizeof(unsigned), bc.s));
+    // the members' row records (the kernel updates those, not the float
+    // embeddings), folded back after the launch
+    for (int i : members)
+      if (jobs[i].rec)
+        HIPC(launch_fix(c, 0, jobs[i].emb, jobs[i].n,
+                        FixFrame{jobs[i].rec}, jobs[i].rounding_p, bc.num_cu,
+                        bc.s));
     std::vector<Engine::Timed> tm;
     for (int i : members)
       if (es[i]->prof) {
@@ -1698,6 +1705,11 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
         HIPC(hipEventRecord(tm[ti].b, bc.s));
         es[i]->pending.push_back(tm[ti++]);
       }
+    for (int i : members)
+      if (jobs[i].rec)
+        HIPC(launch_fix(c, 1, jobs[i].emb, jobs[i].n,
+                        FixFrame{jobs[i].rec}, jobs[i].rounding_p, bc.num_cu,
+                        bc.s));
     HIPC(hipMemcpyAsync(bc.h_flag + nchunks, bar + 2, sizeof(unsigned),
                         hipMemcpyDeviceToHost, bc.s));
     hipEvent_t done = es[members[0]]->ev_get();

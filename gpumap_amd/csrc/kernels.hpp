@@ -195,6 +195,7 @@ struct SgdJob {
   int n_epochs;
   float alpha0, gamma, a, b, nsr_inv;
   unsigned wave0, nwaves;
+  int32_t *rec; // the session's row records (FixFrame), or nullptr
 };
 constexpr int SGD_MAX_JOBS = 16;
 struct SgdBatch { // passed by value as a kernel argument
