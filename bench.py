@@ -39,7 +39,8 @@ FP8_PEAK = 5.0e15          # MI355X dense FP8 MFMA, flop/s
 CHUNK = 500  # the workload's insertion chunk (set from WORKLOAD_CFG)
 METRIC = ("points embedded/s @recall@15≥0.95, BF16-MFMA vs FP32; "
           "p50/p99 run() latency")
-MODES = {"nndescent": 0, "exact_fp32": 1, "exact_bf16": 2, "exact_fp8": 3}
+MODES = {"nndescent": 0, "exact_fp32": 1, "exact_bf16": 2, "exact_fp8": 3,
+         "nndescent_bf16": 4}
 PRECISION = {
     0: "NN-Descent candidates and local join in FP32",
     1: "FP32",

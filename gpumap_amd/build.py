@@ -22,8 +22,8 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgpumap_amd.so")
 HIP_SOURCES = ["nnd_kernels.hip", "exact_kernels.hip", "mfma_kernels.hip",
-               "fp8_kernels.hip", "embed_kernels.hip", "engine.hip",
-               "noalloc.hip"]
+               "fp8_kernels.hip", "join_kernels.hip", "embed_kernels.hip",
+               "engine.hip", "noalloc.hip"]
 ARCH = os.environ.get("GPUMAP_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -163,6 +163,36 @@ __device__ inline bool cas64(uint64_t *p, uint64_t expect, uint64_t val) {
                                               __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Ascending sort of one value per lane over the full 64-lane wave.
+__device__ inline uint32_t wave_sort64(uint32_t x) {
+  const int l = __lane_id();
+#pragma unroll
+  for (int size = 2; size <= 64; size <<= 1)
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const uint32_t y = __shfl_xor(x, stride);
+      const bool up = (size == 64) || ((l & size) == 0);
+      const bool lower = (l & stride) == 0;
+      x = (lower == up) ? min(x, y) : max(x, y);
+    }
+  return x;
+}
+
+// Ascending sort of one value per lane within each 32-lane half.
+template <typename T> __device__ inline T half_sort32(T x) {
+  const int l = __lane_id() & 31;
+#pragma unroll
+  for (int size = 2; size <= 32; size <<= 1)
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const T y = __shfl_xor(x, stride, 32);
+      const bool up = (size == 32) || ((l & size) == 0);
+      const bool lower = (l & stride) == 0;
+      x = (lower == up) ? (x < y ? x : y) : (x < y ? y : x);
+    }
+  return x;
+}
+
 // Lock-free bounded top-W insert, executed by one 32-lane half of a wave
 // (lane = slot).  Contract (identical to the CPU restatement): reject a
 // duplicate id or a key not below the row maximum; otherwise replace the

@@ -78,6 +78,8 @@ std::string knn_mode_to_string(int32_t m) {
     return "exact_bf16";
   case GM_KNN_EXACT_FP8:
     return "exact_fp8";
+  case GM_KNN_NNDESCENT_BF16:
+    return "nndescent_bf16";
   default:
     return "exact_fp32";
   }
@@ -96,12 +98,14 @@ int32_t knn_mode_from_object(const py::object &v) {
       return GM_KNN_EXACT_BF16;
     if (s == "exact_fp8")
       return GM_KNN_EXACT_FP8;
+    if (s == "nndescent_bf16")
+      return GM_KNN_NNDESCENT_BF16;
     throw py::value_error("Unknown knn_mode '" + s +
                           "' -- expected 'nndescent', 'exact_fp32', "
-                          "'exact_bf16' or 'exact_fp8'");
+                          "'exact_bf16', 'exact_fp8' or 'nndescent_bf16'");
   }
   const int m = v.cast<int>();
-  if (m < GM_KNN_NNDESCENT || m > GM_KNN_EXACT_FP8)
+  if (m < GM_KNN_NNDESCENT || m > GM_KNN_NNDESCENT_BF16)
     throw py::value_error("Unknown knn_mode " + std::to_string(m));
   return m;
 }

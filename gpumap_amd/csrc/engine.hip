@@ -471,6 +471,10 @@ struct Engine {
   bool screened() const {
     return p.knn_mode == GM_KNN_EXACT_BF16 || p.knn_mode == GM_KNN_EXACT_FP8;
   }
+  bool nndescent() const {
+    return p.knn_mode == GM_KNN_NNDESCENT ||
+           p.knn_mode == GM_KNN_NNDESCENT_BF16;
+  }
   // a screen shape being timed for the tuner (read in insert_finish)
   struct TunePending {
     bool on = false;
@@ -568,6 +572,14 @@ struct Engine {
       }
       qctl = dalloc<unsigned>(4);
       HIPC(hipMemsetAsync(qctl, 0, 4 * sizeof(unsigned), s));
+    }
+    if (p.knn_mode == GM_KNN_NNDESCENT_BF16) {
+      // the local join's hi/lo operand store (32-feature chunks) and norms
+      Db = (D + 31) / 32 * 32;
+      Xs = dalloc<__bf16>(N * Db * 2);
+      HIPC(hipMemsetAsync(Xs, 0, N * Db * 2 * sizeof(__bf16), s));
+      nrm = dalloc<float>(N);
+      HIPC(hipMemsetAsync(nrm, 0, N * sizeof(float), s));
     }
     bcount = dalloc<uint32_t>(std::max<uint64_t>(2 * nk, N) / 1024 + 2);
     scal = dalloc<uint32_t>(S_COUNT);
@@ -751,6 +763,7 @@ struct Engine {
     totals[GM_KSTAT_JOIN].work[0] = c[0];
     totals[GM_KSTAT_JOIN].work[1] = c[1];
     totals[GM_KSTAT_JOIN].work[2] = c[2];
+    totals[GM_KSTAT_JOIN].work[3] = c[7];
     totals[GM_KSTAT_SGD].work[0] = c[3];
     totals[GM_KSTAT_SGD].work[1] = c[4];
     totals[GM_KSTAT_SGD].work[2] = c[KS_SGD_EE];
@@ -874,7 +887,11 @@ struct Engine {
       k_prep_rows<<<(unsigned)B, 64, 0, s>>>(X, Dp, D, b0, B);
       LAUNCH_CHECK();
     }
-    if (p.knn_mode == GM_KNN_NNDESCENT)
+    if (p.knn_mode == GM_KNN_NNDESCENT_BF16) {
+      k_join_prep<<<(unsigned)B, 64, 0, s>>>(X, Dp, Db, Xs, nrm, b0, B);
+      LAUNCH_CHECK();
+    }
+    if (nndescent())
       insert_nndescent(b0, B, window_start, seed);
     else
       insert_exact(b0, B);
@@ -955,6 +972,19 @@ struct Engine {
       LAUNCH_CHECK();
     }
     ev_stop(20);
+  }
+
+  // A of the BF16 screens' bound |d~ - d*| <= A s (derivation below, in
+  // insert_exact) for db padded features: norms, s and the fma, the hi/lo
+  // split, and the summation of BF16_GROUP_STEPS-step groups on the matrix
+  // core plus the round-to-nearest group adds
+  static double bf16_screen_A(uint64_t db) {
+    const double u24 = std::ldexp(1.0, -24);
+    const double nsteps_d = (double)db / 16.0;
+    const double gnorm = (8.0 * (double)((db + 511) / 512) + 6.0) * u24;
+    const double G = (double)BF16_GROUP_STEPS;
+    return gnorm + 3.02 * u24 + 3.01 * std::ldexp(1.0, -16) +
+           1.01 * (96.0 * G + std::ceil(nsteps_d / G) + 1.0) * u24;
   }
 
   // exact insertion by distance tiles (exact_kernels.hip, mfma_kernels.hip)
@@ -1041,9 +1071,7 @@ struct Engine {
       const double gnorm = (8.0 * (double)((Db + 511) / 512) + 6.0) * u24;
       double A;
       if (!f8) {
-        const double G = (double)BF16_GROUP_STEPS;
-        A = gnorm + 3.02 * u24 + 3.01 * std::ldexp(1.0, -16) +
-            1.01 * (96.0 * G + std::ceil(nsteps_d / G) + 1.0) * u24;
+        A = bf16_screen_A(Db);
       } else {
         const double G = (double)FP8_GROUP_STEPS;
         const double g = std::ldexp(1.0, -8), h = 1.0625 * std::ldexp(1.0, -10);
@@ -1169,7 +1197,7 @@ struct Engine {
   // update_kernel.cuh:913-1054); returns the fresh top-k edge count.
   uint64_t nnd_pass(std::vector<uint32_t> b, std::vector<uint32_t> &affected) {
     affected.clear();
-    if (p.knn_mode != GM_KNN_NNDESCENT)
+    if (!nndescent())
       return 0; // exact pools hold only old entries: nothing to join
     std::sort(b.begin(), b.end());
     b.erase(std::unique(b.begin(), b.end()), b.end());
@@ -1191,9 +1219,20 @@ struct Engine {
     LAUNCH_CHECK();
     const unsigned jgrid = (unsigned)std::min<uint64_t>(n, 2048);
     ev_start(GM_KSTAT_JOIN);
-    k_join<<<jgrid, 256, 0, s>>>(X, Dp, pool, fn, fo, fnc, foc, rev_new,
-                                 rev_old, owners, scal + S_OWNERS, flag, k,
-                                 kstats);
+    if (p.knn_mode == GM_KNN_NNDESCENT_BF16) {
+      // the BF16 screen's bound for Db features; cfac covers the canonical
+      // distance's own rounding, (D + 2) u relative
+      const float coef = (float)(1.25 * bf16_screen_A(Db));
+      const float cfac =
+          (float)(1.25 * (double)(D + 2) * std::ldexp(1.0, -24));
+      k_join_bf16<<<jgrid, 256, 0, s>>>(
+          X, Dp, Xs, Db, nrm, pool, fn, fo, fnc, foc, rev_new, rev_old, owners,
+          scal + S_OWNERS, flag, k, coef, cfac, kstats);
+    } else {
+      k_join<<<jgrid, 256, 0, s>>>(X, Dp, pool, fn, fo, fnc, foc, rev_new,
+                                   rev_old, owners, scal + S_OWNERS, flag, k,
+                                   kstats);
+    }
     LAUNCH_CHECK();
     ev_stop(1);
     k_join_reset<<<blocks_for(n * W, 256), 256, 0, s>>>(
@@ -1943,10 +1982,10 @@ int gm_create(const gm_params *params, gm_handle **out) {
       throw Fail(GM_ERR_INVALID, "n_features must be positive");
     if (q.metric != GM_EUCLIDEAN && q.metric != GM_COSINE)
       throw Fail(GM_ERR_INVALID, "metric must be 0 (euclidean) or 1 (cosine)");
-    if (q.knn_mode < GM_KNN_NNDESCENT || q.knn_mode > GM_KNN_EXACT_FP8)
+    if (q.knn_mode < GM_KNN_NNDESCENT || q.knn_mode > GM_KNN_NNDESCENT_BF16)
       throw Fail(GM_ERR_INVALID, "knn_mode must be 0 (nndescent), 1 "
-                                 "(exact_fp32), 2 (exact_bf16) or 3 "
-                                 "(exact_fp8)");
+                                 "(exact_fp32), 2 (exact_bf16), 3 "
+                                 "(exact_fp8) or 4 (nndescent_bf16)");
     if ((q.knn_mode == GM_KNN_EXACT_BF16 || q.knn_mode == GM_KNN_EXACT_FP8) &&
         q.n_instances >= (1ull << 30))
       throw Fail(GM_ERR_INVALID,

@@ -131,6 +131,18 @@ __global__ void k_bf16_select(const uint64_t *fkeys, const unsigned *fcnt,
                               uint64_t *pool, uint64_t b0,
                               unsigned long long *stats);
 
+// join_kernels.hip
+__global__ void k_join_prep(const float *X, uint64_t dp, uint64_t dj,
+                            __bf16 *Xj, float *nrm, uint64_t b0, uint64_t nb);
+__global__ void k_join_bf16(const float *X, uint64_t dp, const __bf16 *Xj,
+                            uint64_t dj, const float *nrm, uint64_t *pool,
+                            const uint32_t *fn, const uint32_t *fo,
+                            const int *fnc, const int *foc,
+                            const uint64_t *rev_new, const uint64_t *rev_old,
+                            const uint32_t *owners,
+                            const uint32_t *owner_count, uint8_t *flag, int k,
+                            float coef, float cfac, unsigned long long *stats);
+
 // embed_kernels.hip
 __global__ void k_mean_partial(const uint32_t *pidx, const float *pdist,
                                uint64_t n, int k, int euclid, double *psum,

@@ -13,8 +13,6 @@ namespace gm {
 
 namespace {
 
-__device__ inline int lane_id() { return __lane_id(); }
-
 // Bitonic sort of `len` (power of two) LDS elements in independent ascending
 // segments of `seg` elements; every thread of the block must call it.
 template <typename T>
@@ -33,36 +31,6 @@ __device__ void block_bitonic(T *a, int len, int seg) {
       }
       __syncthreads();
     }
-}
-
-// Ascending sort of one value per lane over the full 64-lane wave.
-__device__ inline uint32_t wave_sort64(uint32_t x) {
-  const int l = lane_id();
-#pragma unroll
-  for (int size = 2; size <= 64; size <<= 1)
-#pragma unroll
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      const uint32_t y = __shfl_xor(x, stride);
-      const bool up = (size == 64) || ((l & size) == 0);
-      const bool lower = (l & stride) == 0;
-      x = (lower == up) ? min(x, y) : max(x, y);
-    }
-  return x;
-}
-
-// Ascending sort of one 64-bit key per lane within each 32-lane half.
-__device__ inline uint64_t half_sort32(uint64_t x) {
-  const int l = lane_id() & 31;
-#pragma unroll
-  for (int size = 2; size <= 32; size <<= 1)
-#pragma unroll
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      const uint64_t y = __shfl_xor(x, stride, 32);
-      const bool up = (size == 32) || ((l & size) == 0);
-      const bool lower = (l & stride) == 0;
-      x = (lower == up) ? (x < y ? x : y) : (x < y ? y : x);
-    }
-  return x;
 }
 
 } // namespace

@@ -58,7 +58,7 @@ def _allocations(n: int, d: int, k: int, c: int, knn_mode: int,
     sizes += [256 * 8, 256 * 8, 8, 64, 8, 128, 4, 4, _SGD_SLOTS * 4]
     # the exact tiles' per-(row, split) top lists (FP32 and BF16 modes) or
     # the NN-Descent candidates
-    if knn_mode == 0:
+    if knn_mode in (0, 4):
         sizes += [b * _POOL_W * 4, b * 4, b * _CAND * 4, b * _CAND * 4, b * 4]
     else:
         sizes += [b * _MAX_SPLIT * _POOL_W * 8]
@@ -77,6 +77,9 @@ def _allocations(n: int, d: int, k: int, c: int, knn_mode: int,
                   b * cap * 4, 2 * b * 4, b * cap * 8]
         if n * b * 4 <= _LB_STORE_MAX:
             sizes += [n * b * 4]                          # lower-bound store
+    if knn_mode == 4:
+        # the local join's hi/lo operand store (32-feature chunks), norms
+        sizes += [n * ((d + 31) // 32 * 32) * 4, n * 4]
     return sizes
 
 
@@ -285,7 +288,7 @@ class Server:
         d = params.n_features or data_loader.n_features
         mode = getattr(params, "knn_mode", "exact_bf16")
         mode = {"nndescent": 0, "exact_fp32": 1, "exact_bf16": 2,
-                "exact_fp8": 3}.get(mode, mode)
+                "exact_fp8": 3, "nndescent_bf16": 4}.get(mode, mode)
         fp = session_footprint(n, d, params.n_neighbors, params.n_components,
                                int(mode), self.max_batch_rows)
         self.admission.acquire(fp, admission_timeout)

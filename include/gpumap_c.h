@@ -44,12 +44,17 @@ typedef enum gm_metric { GM_EUCLIDEAN = 0, GM_COSINE = 1 } gm_metric;
  *                      GM_KNN_EXACT_FP32)
  *   GM_KNN_EXACT_FP8   exact: FP8 E4M3 MFMA tiles (per-row power-of-two
  *                      scales, hi/lo split) screen with a proven, looser
- *                      bound; survivors re-ranked in FP32 (same graph) */
+ *                      bound; survivors re-ranked in FP32 (same graph)
+ *   GM_KNN_NNDESCENT_BF16  the reference's method with the local join's
+ *                      distance blocks on BF16 MFMA tiles, screened with
+ *                      the proven bound and re-ranked in FP32 (same graph
+ *                      as GM_KNN_NNDESCENT) */
 typedef enum gm_knn_mode {
   GM_KNN_NNDESCENT = 0,
   GM_KNN_EXACT_FP32 = 1,
   GM_KNN_EXACT_BF16 = 2,
-  GM_KNN_EXACT_FP8 = 3
+  GM_KNN_EXACT_FP8 = 3,
+  GM_KNN_NNDESCENT_BF16 = 4
 } gm_knn_mode;
 
 /* Mirrors gpumap::GPUMAPParams (cpp/include/gpumap/gpumap.cuh:15-36) with
@@ -246,7 +251,8 @@ int gm_sgd_noalloc(float *emb, uint64_t n, int32_t c, const int32_t *head,
  * session's stream bracket the timed kernels and device counters record their
  * algorithmic work:
  *   GM_KSTAT_JOIN        local join; work = {list members, distance pairs,
- *                        queued pool offers}
+ *                        queued pool offers, pairs re-ranked in FP32 after
+ *                        the BF16 screen (GM_KNN_NNDESCENT_BF16)}
  *   GM_KSTAT_SGD         SGD with a register-resident cooperative kernel
  *                        (one launch per embedding op: k_sgd_waves for up to
  *                        512 waves of edges, else k_sgd_fused) or, when

@@ -31,8 +31,9 @@ def _lockstep(x, chunk, k=15, metric=0, n_ops=None, embed=0, max_cycles=80,
     n_ops = n_ops or chunk
     e = capi.Engine(n, d, n_neighbors=k, n_components=c, metric=metric,
                     knn_mode=mode)
+    # the oracle's NN-Descent (0) for both NN-Descent joins, else exact (1)
     o = oracle.Oracle(n, d, n_neighbors=k, n_components=c, metric=metric,
-                      n_threads=8, knn_mode=min(mode, 1))
+                      n_threads=8, knn_mode=0 if mode in (0, 4) else 1)
     pos, cyc = 0, 0
     while True:
         ins = min(chunk, n - pos)
@@ -66,7 +67,9 @@ def test_philox_device_matches_published_vectors():
 
 
 MODES = [pytest.param(0, id="nndescent"), pytest.param(1, id="exact_fp32"),
-         pytest.param(2, id="exact_bf16"), pytest.param(3, id="exact_fp8")]
+         pytest.param(2, id="exact_bf16"), pytest.param(3, id="exact_fp8"),
+         pytest.param(4, id="nndescent_bf16")]
+EXACT_MODES = MODES[1:4]
 
 
 @pytest.mark.parametrize("mode", MODES)
@@ -100,7 +103,7 @@ def test_knn_bit_exact_tiny_batches_and_duplicates(mode):
     _lockstep(x, chunk=7, n_ops=40, max_cycles=200, mode=mode)
 
 
-@pytest.mark.parametrize("mode", MODES[1:])
+@pytest.mark.parametrize("mode", EXACT_MODES)
 def test_exact_mode_is_exact_knn(mode):
     # the exact pools give the FP64 brute-force neighbours (tie-aware)
     x = ev.gaussian10(n=3000, d=32)
@@ -108,6 +111,56 @@ def test_exact_mode_is_exact_knn(mode):
     idx, _ = e.knn()
     ei, ed = ev.exact_knn(x, 15)
     assert ev.recall_at_k(idx, x, ei, ed) == 1.0
+
+
+def test_nndescent_bf16_join_on_image_like_rows():
+    # 784 features (25 BF16 chunks, padding to 800) of the 70,000 x 784
+    # config's image-like rows: the BF16-screened local join keeps the
+    # oracle's NN-Descent graph every cycle, and its screen re-ranks fewer
+    # pairs than the join computes
+    x = ev.image_like(n=2500, d=784)
+    e, o = _lockstep(x, chunk=500, mode=4, check_every=2)
+    st = e.kernel_stats(0)
+    print(f"join: members {st.work[0]} pairs {st.work[1]} offers "
+          f"{st.work[2]} re-ranked {st.work[3]}", flush=True)
+    assert 0 < st.work[3] < st.work[1]
+
+
+def test_nndescent_bf16_equals_fp32_join_at_scale():
+    # 20,000 x 96 in 1,000-row chunks through the two joins on the device:
+    # equal public rows after every cycle, equal pools at the end
+    x = ev.gaussian10(n=20_000, d=96)
+    n, d = x.shape
+    e0 = capi.Engine(n, d, knn_mode=0)
+    e4 = capi.Engine(n, d, knn_mode=4)
+    for eng in (e0, e4):
+        eng.set_profiling(True)
+    pos, cyc = 0, 0
+    while True:
+        ins = min(1000, n - pos)
+        if ins:
+            for eng in (e0, e4):
+                eng.append(x[pos:pos + ins])
+            pos += ins
+        upd = 1000 if ins else n
+        r0, r4 = e0.run(ins, upd, 0), e4.run(ins, upd, 0)
+        for f in RESULT_FIELDS:
+            assert getattr(r0, f) == getattr(r4, f), (cyc, f)
+        i0, d0 = e0.knn()
+        i4, d4 = e4.knn()
+        assert np.array_equal(i0, i4), cyc
+        assert np.array_equal(d0.view(np.uint32), d4.view(np.uint32)), cyc
+        cyc += 1
+        if (r0.update_completed and pos == n) or cyc > 200:
+            break
+    assert r0.update_completed
+    assert np.array_equal(np.sort(e0.pool(), 1), np.sort(e4.pool(), 1))
+    s0, s4 = e0.kernel_stats(0), e4.kernel_stats(0)
+    assert s0.work[0] == s4.work[0] and s0.work[1] == s4.work[1]
+    print(f"join FP32 {s0.device_ms:.1f} ms, BF16 {s4.device_ms:.1f} ms; "
+          f"pairs {s4.work[1]}, re-ranked {s4.work[3]}", flush=True)
+    e0.close()
+    e4.close()
 
 
 def test_vmf_cosine_k30_shape_bf16():
@@ -436,7 +489,8 @@ def test_gaussian10_quality_and_exact_graph(mode):
     x = ev.gaussian10()
     n, d = x.shape
     e = capi.Engine(n, d, knn_mode=mode)
-    o = oracle.Oracle(n, d, n_threads=16, knn_mode=min(mode, 1))
+    o = oracle.Oracle(n, d, n_threads=16,
+                      knn_mode=0 if mode in (0, 4) else 1)
     pos = 0
     while pos < n:
         e.append(x[pos:pos + 500])
