@@ -50,8 +50,11 @@ PRECISION = {
     3: ("distance tiles FP8 E4M3 MFMA (per-row power-of-two scales, hi/lo "
         "split, FP32 accumulation) screened by a rigorous error bound and "
         "re-ranked in exact FP32; sigma/rho FP64; SGD FP32"),
+    4: ("NN-Descent candidates in FP32; local-join distance blocks BF16 MFMA "
+        "(hi/lo split, FP32 accumulation) screened by a rigorous error bound "
+        "and re-ranked in exact FP32; sigma/rho FP64; SGD FP32"),
 }
-DTYPE = {0: "f32", 1: "f32", 2: "bf16", 3: "fp8_e4m3"}
+DTYPE = {0: "f32", 1: "f32", 2: "bf16", 3: "fp8_e4m3", 4: "bf16"}
 
 WORKLOAD_DESC = {
     "image_like": "70,000x784 float32 image-like 10-cluster mixture, sparse "
@@ -160,12 +163,18 @@ def kernel_numbers(e, capi, d, c):
             "total_ms": ins.device_ms}
     js = e.kernel_stats(capi.KSTAT_JOIN)
     if js.launches:
-        members, offers = js.work[0], js.work[2]
+        members, pairs, offers = js.work[0], js.work[1], js.work[2]
         bytes_ = members * (4 * d + 8 * 32) + offers * 8 * 32
+        # distance work: one multiply-add per feature and pair of the lists
+        flops = 2.0 * d * pairs
         out["k_join"] = {
             "launches": int(js.launches), "avg_ms": js.device_ms / js.launches,
             "algo_bytes_per_launch": bytes_ / js.launches,
             "achieved_GBps": bytes_ / (js.device_ms * 1e-3) / 1e9,
+            "pairs": int(pairs), "offers": int(offers),
+            "pairs_reranked_fp32": int(js.work[3]),
+            "algo_flops_per_launch": flops / js.launches,
+            "achieved_TFLOPs": flops / (js.device_ms * 1e-3) / 1e12,
             "total_ms": js.device_ms}
     return out
 
@@ -692,6 +701,9 @@ def main():
         kern["insertion"]["mfma_frac_algorithmic"] = (
             kern["insertion"]["achieved_TFLOPs"] * 1e12 /
             (FP8_PEAK if mode == MODES["exact_fp8"] else BF16_PEAK))
+    if "k_join" in kern and mode == MODES["nndescent_bf16"]:
+        kern["k_join"]["mfma_frac_algorithmic"] = (
+            kern["k_join"]["achieved_TFLOPs"] * 1e12 / BF16_PEAK)
 
     if args.copy_probe:
         src = torch.empty(args.copy_probe << 18, dtype=torch.float32,
