@@ -55,7 +55,8 @@ class GmKernelStats(ctypes.Structure):
                 ("work", u64 * 4)]
 
 
-KSTAT_JOIN, KSTAT_SGD, KSTAT_INSERT, KSTAT_SGD_STREAM = 0, 1, 2, 3
+KSTAT_JOIN, KSTAT_SGD, KSTAT_INSERT, KSTAT_SGD_STREAM, KSTAT_GRAPH = (
+    0, 1, 2, 3, 4)
 KNN_NNDESCENT, KNN_EXACT_FP32, KNN_EXACT_BF16 = 0, 1, 2
 
 

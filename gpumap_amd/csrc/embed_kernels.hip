@@ -62,12 +62,12 @@ __device__ inline float o2f(uint32_t u) {
 // Global mean of the valid embed-time distances (reference
 // compute_mean_dist_kernel :82-128), in FP64 with a fixed reduction order.
 __global__ void k_mean_partial(const uint32_t *__restrict__ pidx,
-                               const float *__restrict__ pdist, uint64_t n,
+                               const float *__restrict__ pdist, NRows nr,
                                int k, int euclid, double *__restrict__ psum,
                                unsigned long long *__restrict__ pcnt) {
   __shared__ double s_s[256];
   __shared__ unsigned long long s_c[256];
-  const uint64_t total = n * (uint64_t)k;
+  const uint64_t total = nr.get() * (uint64_t)k;
   double s = 0.0;
   unsigned long long c = 0;
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void k_mean_final(
 // entry of the larger row is not placed on its own (bit 2) since its twin
 // holds it.  `inb` marks the batch rows (null: every row is one).
 __global__ void k_membership(const uint32_t *__restrict__ pidx,
-                             const float *__restrict__ pdist, uint64_t n,
+                             const float *__restrict__ pdist, NRows nr,
                              int k, int euclid, const double *__restrict__ sigma,
                              const double *__restrict__ rho,
                              const double *__restrict__ mean_p,
@@ -159,6 +159,7 @@ __global__ void k_membership(const uint32_t *__restrict__ pidx,
                              float *__restrict__ w, uint8_t *__restrict__ flags,
                              uint32_t *__restrict__ maxw_bits) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t n = nr.get();
   const double mean_dist = *mean_p;
   float wf = 0.0f;
   if (t < nb * (uint64_t)k) {
@@ -334,13 +335,14 @@ __device__ inline float rounding_of(int md, float alpha) {
 // :405-460): the last block to finish reads the maximum and writes the grid.
 // `done` counts finished blocks and is left at 0 for the next launch.
 __global__ __launch_bounds__(256) void k_max_int(const int *__restrict__ v,
-                                                 uint64_t n,
+                                                 NRows nr,
                                                  int *__restrict__ out,
                                                  float alpha,
                                                  float *__restrict__ rounding,
                                                  unsigned *__restrict__ done) {
   __shared__ int s_m[4];
   __shared__ bool s_last;
+  const uint64_t n = nr.get();
   int m = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -369,9 +371,10 @@ __global__ __launch_bounds__(256) void k_max_int(const int *__restrict__ v,
 
 // -------------------------------------------------------------- noise
 __global__ __launch_bounds__(256) void k_minmax_col0(
-    const float *__restrict__ emb, uint64_t n, int c, uint32_t *__restrict__ mn,
+    const float *__restrict__ emb, NRows nr, int c, uint32_t *__restrict__ mn,
     uint32_t *__restrict__ mx, int *__restrict__ deg_zero) {
   __shared__ uint32_t s_lo[4], s_hi[4];
+  const uint64_t n = nr.get();
   uint32_t lo = 0xffffffffu, hi = 0u;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
@@ -2071,7 +2074,7 @@ template __global__ void k_sgd_epoch<4>(float *, const int32_t *,
 // independent loads (entries past k - 1 read entry 0 and are skipped), and
 // sums them in neighbour order.
 __global__ void k_reposition(const uint32_t *__restrict__ pidx,
-                             const float *__restrict__ pdist, uint64_t n,
+                             const float *__restrict__ pdist, NRows nr,
                              int k, int euclid, const double *__restrict__ sigma,
                              const double *__restrict__ rho,
                              const double *__restrict__ mean_p,
@@ -2079,6 +2082,7 @@ __global__ void k_reposition(const uint32_t *__restrict__ pidx,
                              const float *__restrict__ snap,
                              float *__restrict__ emb, int c) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t n = nr.get();
   if (i >= n)
     return;
   if (in_batch[i]) { // a batch row: keeps its place; clear its mark
@@ -2127,6 +2131,15 @@ __global__ void k_reposition(const uint32_t *__restrict__ pidx,
       if (dd < c)
         emb[i * c + dd] = (float)(acc[dd] / wsum);
   }
+}
+
+// dst[0, n c) = src[0, n c): the embedding's snapshot for k_reposition
+__global__ void k_copy_rows(const float *__restrict__ src,
+                            float *__restrict__ dst, NRows nr, int c) {
+  const uint64_t total = nr.get() * (uint64_t)c;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 // ctr += nnz * n_epochs: the edge-epochs an SGD launch scans, counted on the

@@ -114,6 +114,7 @@ enum Scalar {
   S_MIN,
   S_MAX,
   S_NNZ,
+  S_NROWS, // the session's row count, for kernels replayed from a HIP graph
   S_COUNT
 };
 
@@ -382,6 +383,29 @@ struct Engine {
   // the float embedding
   int32_t *fxp = nullptr;
   bool sgd_fix = true;
+  // With GPUMAP_AMD_GRAPH=1 the embedding op's kernel sequences before and
+  // after its SGD launch run as HIP graphs: captured once (stream capture)
+  // and relaunched while the host values they depend on stay the same, the
+  // row count read from a device scalar; re-captured and patched into the
+  // executable graph (hipGraphExecUpdate) when they change.  The cooperative
+  // SGD launch stays outside the graphs, between them on the stream, with
+  // its timing events.  Off by default: on the 70,000 x 784 op plan the
+  // graph launches measured 3% slower per session than the direct launches.
+  bool use_graph = false;
+  hipGraphExec_t gx_pre = nullptr, gx_post = nullptr;
+  uint64_t graph_replays = 0, graph_instantiations = 0;
+  // the host values a captured sequence depends on (its kernels read the
+  // row count from S_NROWS and the batch from dbatch): an op with the same
+  // key relaunches the executable graph as it is
+  struct GraphKey {
+    uint64_t nb = 0, sweeps = 0;
+    bool partial = false, setup = false, valid = false;
+    bool operator==(const GraphKey &o) const {
+      return nb == o.nb && sweeps == o.sweeps && partial == o.partial &&
+             setup == o.setup && valid == o.valid;
+    }
+  };
+  GraphKey gk_pre, gk_post;
   uint8_t *flag = nullptr;
   uint32_t *list = nullptr;
   float *pmax = nullptr;      // per-row pool maximum distance (inf if not full)
@@ -508,6 +532,8 @@ struct Engine {
       lbm_max_bytes = std::strtoull(v, nullptr, 10);
     if (const char *v = std::getenv("GPUMAP_AMD_SGD_FIX"))
       sgd_fix = v[0] != '0';
+    if (const char *v = std::getenv("GPUMAP_AMD_GRAPH"))
+      use_graph = v[0] != '0';
     try {
       allocate();
     } catch (...) {
@@ -673,6 +699,11 @@ struct Engine {
     for (hipEvent_t ev : ev_free)
       (void)hipEventDestroy(ev);
     ev_free.clear();
+    for (hipGraphExec_t *gx : {&gx_pre, &gx_post}) {
+      if (*gx)
+        (void)hipGraphExecDestroy(*gx);
+      *gx = nullptr;
+    }
     // every device pointer is freed once and cleared, so that release() may
     // run again (from the destructor after an early gm_release)
     void **ptrs[] = {
@@ -774,6 +805,8 @@ struct Engine {
     totals[GM_KSTAT_INSERT].work[0] = ins_pairs;
     totals[GM_KSTAT_INSERT].work[1] = c[5];
     totals[GM_KSTAT_INSERT].work[2] = c[6];
+    totals[GM_KSTAT_GRAPH].launches = graph_replays;
+    totals[GM_KSTAT_GRAPH].work[0] = graph_instantiations;
   }
 
   void log(const char *msg) const {
@@ -1291,9 +1324,11 @@ struct Engine {
   // Builds the trimmed COO of the batch rows in dbatch[0, nb); leaves nnz in
   // scal[S_NNZ] and the maximum weight in scal[S_MAXW].  A batch of fewer
   // than n rows is marked in bmark (k_reposition, or the caller, clears it).
-  void build_fuzzy(uint64_t nb) {
+  // nr: the row count as the kernels read it (a value, or S_NROWS when the
+  // launches are captured into a graph)
+  void build_fuzzy(uint64_t nb, NRows nr) {
     uint8_t *mark = nb < n ? bmark : nullptr;
-    k_mean_partial<<<NPART, 256, 0, s>>>(pidx, pdist, n, k, euclid ? 1 : 0,
+    k_mean_partial<<<NPART, 256, 0, s>>>(pidx, pdist, nr, k, euclid ? 1 : 0,
                                          psum, pcnt);
     LAUNCH_CHECK();
     static_assert(S_MIN == S_MAXW + 1 && S_MAX == S_MAXW + 2,
@@ -1303,7 +1338,7 @@ struct Engine {
     LAUNCH_CHECK();
     const uint64_t nd = nb * (uint64_t)k;
     k_membership<<<blocks_for(nd, 256), 256, 0, s>>>(
-        pidx, pdist, n, k, euclid ? 1 : 0, sigma, rho, mean, dbatch, nb, mark,
+        pidx, pdist, nr, k, euclid ? 1 : 0, sigma, rho, mean, dbatch, nb, mark,
         w, wflags, scal + S_MAXW);
     LAUNCH_CHECK();
     const unsigned nblk = blocks_for(nd, 1024);
@@ -1326,28 +1361,36 @@ struct Engine {
   // by the caller's k_minmax_col0 on the embedding path; k_sgd_init clears
   // maxdeg and the barrier words, and k_max_int counts its finished blocks in
   // gbar[3] (unused by the SGD kernels).
+  // dev_rows: the row count read from S_NROWS (a captured launch; the grid
+  // then covers the capacity)
   void sgd_setup(uint64_t nrows, uint64_t cap, int n_epochs, float nsr_inv,
-                 float alpha0, bool deg_zeroed) {
+                 float alpha0, bool deg_zeroed, bool dev_rows = false) {
     if (!deg_zeroed)
       HIPC(hipMemsetAsync(deg, 0, nrows * sizeof(int), s));
     k_sgd_init<<<blocks_for(cap, 256), 256, 0, s>>>(
         wout, head, scal + S_NNZ, scal + S_MAXW, n_epochs, nsr_inv, eps, nxt,
         nneg, deg, maxdeg, gbar, 4 + SGD_MAX_WAVES);
     LAUNCH_CHECK();
-    k_max_int<<<std::min<unsigned>(blocks_for(nrows, 256), 64), 256, 0, s>>>(
-        deg, nrows, maxdeg, alpha0, rounding, gbar + 3);
+    k_max_int<<<std::min<unsigned>(blocks_for(dev_rows ? N : nrows, 256), 64),
+                256, 0, s>>>(deg,
+                             dev_rows ? NRows{scal + S_NROWS, 0}
+                                      : NRows{nullptr, nrows},
+                             maxdeg, alpha0, rounding, gbar + 3);
     LAUNCH_CHECK();
   }
 
   // sgd_mode: SGD_AUTO (the kernel that fits), SGD_PER_EPOCH (one launch per
   // epoch) or SGD_STREAM (the streaming cooperative kernel at any size)
+  // set_up: the schedule was prepared already (sgd_setup)
   void run_sgd(float *e, uint64_t nrows, uint64_t cap, int n_epochs, int nsr,
                float a, float b, float alpha0, float gamma, uint64_t seed,
-               int sgd_mode = 0, bool deg_zeroed = false) {
+               int sgd_mode = 0, bool deg_zeroed = false,
+               bool set_up = false) {
     if (cap == 0 || n_epochs <= 0)
       return;
     const float nsr_inv = 1.0f / (float)nsr;
-    sgd_setup(nrows, cap, n_epochs, nsr_inv, alpha0, deg_zeroed);
+    if (!set_up)
+      sgd_setup(nrows, cap, n_epochs, nsr_inv, alpha0, deg_zeroed);
     unsigned long long *st = prof ? kstats : nullptr;
     ev_start(GM_KSTAT_SGD);
     const bool fused = sgd_launches(
@@ -1369,6 +1412,60 @@ struct Engine {
           kstats + (fused && sgd_variant == 1 ? KS_STREAM_EE : KS_SGD_EE));
       LAUNCH_CHECK();
     }
+  }
+
+  // Enqueue `body`'s launches on the stream: directly, or as the executable
+  // graph `gx`.  While the host values the launches depend on (`key`) stay
+  // those `gx` was made with, `gx` is relaunched as it is; otherwise `body`
+  // is captured and patched into `gx` (hipGraphExecUpdate), or instantiated
+  // anew when its shape changed.
+  template <typename F>
+  void as_graph(hipGraphExec_t &gx, GraphKey &last, const GraphKey &key,
+                F &&body) {
+    if (!use_graph) {
+      body();
+      return;
+    }
+    if (gx && last == key) {
+      HIPC(hipGraphLaunch(gx, s));
+      ++graph_replays;
+      return;
+    }
+    last = GraphKey{};
+    hipGraph_t g = nullptr;
+    HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      body();
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &g);
+      if (g)
+        (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPC(hipStreamEndCapture(s, &g));
+    bool ok = false;
+    if (gx) {
+      hipGraphNode_t err_node = nullptr;
+      hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+      ok = hipGraphExecUpdate(gx, g, &err_node, &res) == hipSuccess &&
+           res == hipGraphExecUpdateSuccess;
+      if (!ok) {
+        (void)hipGetLastError();
+        (void)hipGraphExecDestroy(gx);
+        gx = nullptr;
+      }
+    }
+    if (!gx) {
+      const hipError_t e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPC(e);
+      ++graph_instantiations;
+    } else {
+      (void)hipGraphDestroy(g);
+    }
+    last = key;
+    HIPC(hipGraphLaunch(gx, s));
+    ++graph_replays;
   }
 
   // -------------------------------------------------------------- embed
@@ -1426,31 +1523,47 @@ struct Engine {
     }
     const uint64_t nb = batch.size();
     std::memcpy(h_list, batch.data(), nb * sizeof(uint32_t));
-    HIPC(hipMemcpyAsync(dbatch, h_list, nb * sizeof(uint32_t),
-                        hipMemcpyHostToDevice, s));
-    build_fuzzy(nb);
     if (embedding_sweeps_done > 2)
       embedding_completed = update_completed;
-
-    // noise (reference add_embedding_noise, embedding_kernel.cuh:953-1007);
-    // the column-0 range accumulators were reset by k_mean_final, and the
-    // range pass also clears the SGD's head degrees
-    k_minmax_col0<<<std::min<unsigned>(blocks_for(n, 256), 64), 256, 0, s>>>(
-        emb, n, c, scal + S_MIN, scal + S_MAX, deg);
-    LAUNCH_CHECK();
-    k_noise<<<blocks_for(nb * c, 256), 256, 0, s>>>(
-        emb, c, dbatch, nb, scal + S_MIN, scal + S_MAX, 0.001f,
-        42ull ^ (uint64_t)embedding_sweeps_done);
-    LAUNCH_CHECK();
     // the launch grids are sized for the most edges the batch can give (the
     // union keeps at most both directions of each of its nb k entries); the
     // kernels read the edge count itself from the device
     const uint64_t cap = 2 * nb * (uint64_t)k;
     const uint64_t waves = (cap + 63) / 64;
+    const bool batched =
+        job && cap > 0 && p.max_epoch > 0 && waves <= SGD_MAX_WAVES;
+    const bool own_sgd = !batched && cap > 0 && p.max_epoch > 0;
+    const float nsr_inv = 1.0f / (float)p.negative_sample_rate;
+    // with graphs the kernels read the row count from S_NROWS (copied from
+    // pinned memory by the sequence's first node) and cover the capacity
+    const NRows nr = use_graph ? NRows{scal + S_NROWS, 0} : NRows{nullptr, n};
+    const uint64_t gn = use_graph ? N : n;
+    h_scal[16] = (uint32_t)n;
+    const GraphKey key{nb, embedding_sweeps_done, nb < n, batched || own_sgd,
+                       true};
+    as_graph(gx_pre, gk_pre, key, [&] {
+      if (use_graph)
+        HIPC(hipMemcpyAsync(scal + S_NROWS, h_scal + 16, sizeof(uint32_t),
+                            hipMemcpyHostToDevice, s));
+      HIPC(hipMemcpyAsync(dbatch, h_list, nb * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
+      build_fuzzy(nb, nr);
+      // noise (reference add_embedding_noise, embedding_kernel.cuh:953-1007);
+      // the column-0 range accumulators were reset by k_mean_final, and the
+      // range pass also clears the SGD's head degrees
+      k_minmax_col0<<<std::min<unsigned>(blocks_for(gn, 256), 64), 256, 0,
+                      s>>>(emb, nr, c, scal + S_MIN, scal + S_MAX, deg);
+      LAUNCH_CHECK();
+      k_noise<<<blocks_for(nb * c, 256), 256, 0, s>>>(
+          emb, c, dbatch, nb, scal + S_MIN, scal + S_MAX, 0.001f,
+          42ull ^ (uint64_t)embedding_sweeps_done);
+      LAUNCH_CHECK();
+      if (batched || own_sgd)
+        sgd_setup(n, cap, (int)p.max_epoch, nsr_inv, p.alpha, true,
+                  use_graph);
+    });
     pend_nb = nb;
-    if (job && cap > 0 && p.max_epoch > 0 && waves <= SGD_MAX_WAVES) {
-      const float nsr_inv = 1.0f / (float)p.negative_sample_rate;
-      sgd_setup(n, cap, (int)p.max_epoch, nsr_inv, p.alpha, true);
+    if (batched) {
       *job = SgdJob{emb,     head,         tail,
                     scal + S_NNZ, eps,     rounding,
                     prof ? kstats : nullptr, 42ull, (uint32_t)n,
@@ -1461,7 +1574,7 @@ struct Engine {
     if (before_own_sgd)
       before_own_sgd();
     run_sgd(emb, n, cap, (int)p.max_epoch, (int)p.negative_sample_rate, p.a,
-            p.b, p.alpha, 1.0f, 42ull, SGD_AUTO, true);
+            p.b, p.alpha, 1.0f, 42ull, SGD_AUTO, true, true);
     if (fused_used)
       HIPC(hipMemcpyAsync(h_scal + 12, gbar + 2, 4, hipMemcpyDeviceToHost, s));
     return (uint32_t)nb;
@@ -1475,12 +1588,20 @@ struct Engine {
     const uint64_t nb = pend_nb;
     pend_nb = 0;
     if (nb > 0 && nb < n) {
-      HIPC(hipMemcpyAsync(snap, emb, n * c * sizeof(float),
-                          hipMemcpyDeviceToDevice, s));
-      k_reposition<<<blocks_for(n, 128), 128, 0, s>>>(
-          pidx, pdist, n, k, euclid ? 1 : 0, sigma, rho, mean, bmark, snap,
-          emb, c);
-      LAUNCH_CHECK();
+      // the row count from S_NROWS, set by the op's first sequence
+      const NRows nr =
+          use_graph ? NRows{scal + S_NROWS, 0} : NRows{nullptr, n};
+      const uint64_t gn = use_graph ? N : n;
+      as_graph(gx_post, gk_post, GraphKey{0, 0, true, false, true}, [&] {
+        k_copy_rows<<<(unsigned)std::min<uint64_t>(blocks_for(gn * c, 256),
+                                                   8 * (uint64_t)num_cu),
+                      256, 0, s>>>(emb, snap, nr, c);
+        LAUNCH_CHECK();
+        k_reposition<<<blocks_for(gn, 128), 128, 0, s>>>(
+            pidx, pdist, nr, k, euclid ? 1 : 0, sigma, rho, mean, bmark, snap,
+            emb, c);
+        LAUNCH_CHECK();
+      });
     }
   }
 
@@ -2254,6 +2375,7 @@ int gm_reset_kernel_stats(gm_handle *h) {
     e->fold_events();
     e->sgd_waves_launches = 0;
     e->ins_pairs = 0;
+    e->graph_replays = e->graph_instantiations = 0;
     HIPC(hipMemsetAsync(e->kstats, 0, 16 * sizeof(unsigned long long), e->s));
     HIPC(hipStreamSynchronize(e->s));
   });
@@ -2322,7 +2444,7 @@ int gm_debug_fuzzy(gm_handle *h, const uint32_t *batch, uint64_t nb,
     }
     std::shared_lock<gm::DevLock> dl(gm::g_dev_mu);
     HIPC(hipMemcpyAsync(e->dbatch, batch, nb * 4, hipMemcpyHostToDevice, e->s));
-    e->build_fuzzy(nb);
+    e->build_fuzzy(nb, gm::NRows{nullptr, e->n});
     if (nb < e->n) // no reposition follows to clear the batch marks
       HIPC(hipMemsetAsync(e->bmark, 0, e->n, e->s));
     uint32_t cnt = 0;

@@ -5,6 +5,14 @@
 
 namespace gm {
 
+// A row count that the kernels read at run time: a device scalar (kernels
+// captured into a HIP graph replay while the session grows) or a value.
+struct NRows {
+  const uint32_t *p;
+  uint64_t v;
+  __device__ uint64_t get() const { return p ? (uint64_t)*p : v; }
+};
+
 // nnd_kernels.hip
 __global__ void k_prep_rows(float *X, uint64_t dp, uint64_t d, uint64_t b0,
                             uint64_t nb);
@@ -145,13 +153,13 @@ __global__ void k_join_bf16(const float *X, uint64_t dp, const __bf16 *Xj,
 
 // embed_kernels.hip
 __global__ void k_mean_partial(const uint32_t *pidx, const float *pdist,
-                               uint64_t n, int k, int euclid, double *psum,
+                               NRows n, int k, int euclid, double *psum,
                                unsigned long long *pcnt);
 __global__ void k_mean_final(const double *psum, const unsigned long long *pcnt,
                              int nparts, double *mean, uint32_t *reset3,
                              const uint32_t *batch, uint64_t nb, uint8_t *mark);
 __global__ void k_membership(const uint32_t *pidx, const float *pdist,
-                             uint64_t n, int k, int euclid,
+                             NRows n, int k, int euclid,
                              const double *sigma, const double *rho,
                              const double *mean, const uint32_t *batch,
                              uint64_t nb, const uint8_t *inb, float *w,
@@ -169,9 +177,9 @@ __global__ void k_sgd_init(const float *w, const int32_t *head,
                            int n_epochs, float nsr_inv, float *eps, float *nxt,
                            float *nneg, int *deg, int *maxdeg, unsigned *bar,
                            unsigned nbar);
-__global__ void k_max_int(const int *v, uint64_t n, int *out, float alpha,
+__global__ void k_max_int(const int *v, NRows n, int *out, float alpha,
                           float *rounding, unsigned *done);
-__global__ void k_minmax_col0(const float *emb, uint64_t n, int c,
+__global__ void k_minmax_col0(const float *emb, NRows n, int c,
                               uint32_t *mn, uint32_t *mx, int *deg_zero);
 __global__ void k_noise(float *emb, int c, const uint32_t *batch, uint64_t nb,
                         const uint32_t *mn, const uint32_t *mx,
@@ -269,10 +277,11 @@ bool sgd_launches(int c, float *e, const int32_t *head, const int32_t *tail,
                   unsigned long long *st, uint64_t cap, int num_cu,
                   hipStream_t s, int sgd_mode, int *variant, bool verbose, FixFrame fx);
 __global__ void k_reposition(const uint32_t *pidx, const float *pdist,
-                             uint64_t n, int k, int euclid, const double *sigma,
+                             NRows n, int k, int euclid, const double *sigma,
                              const double *rho, const double *mean,
                              uint8_t *in_batch, const float *snap,
                              float *emb, int c);
+__global__ void k_copy_rows(const float *src, float *dst, NRows n, int c);
 __global__ void k_add_edge_epochs(const uint32_t *nnz_p, int n_epochs,
                                   unsigned long long *ctr);
 

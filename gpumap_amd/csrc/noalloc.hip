@@ -192,8 +192,8 @@ extern "C" int gm_sgd_noalloc(float *emb, uint64_t n, int32_t c,
         w, head, ctl, ctl + 1, n_epochs, nsr_inv, eps, nxt, nneg, deg, maxdeg,
         bar, (unsigned)BAR_WORDS);
     na_check(hipGetLastError(), "k_sgd_init");
-    k_max_int<<<grid_for(n, 256, 64), 256, 0, s>>>(deg, n, maxdeg, alpha0,
-                                                   rounding, bar + 3);
+    k_max_int<<<grid_for(n, 256, 64), 256, 0, s>>>(
+        deg, NRows{nullptr, n}, maxdeg, alpha0, rounding, bar + 3);
     na_check(hipGetLastError(), "k_max_int");
     int variant = 0;
     const bool coop = sgd_launches(

@@ -267,13 +267,19 @@ int gm_sgd_noalloc(float *emb, uint64_t n, int32_t c, const int32_t *head,
  *                        tiles after a survivor-list overflow}
  *   GM_KSTAT_SGD_STREAM  SGD with the streaming cooperative kernel (edge sets
  *                        larger than the register-resident grid); work as
- *                        GM_KSTAT_SGD */
+ *                        GM_KSTAT_SGD
+ *   GM_KSTAT_GRAPH       the embedding op's kernel sequences run as HIP
+ *                        graphs (counted with or without profiling);
+ *                        launches = graph launches, work[0] = executable
+ *                        graphs instantiated (the others were updated in
+ *                        place), device_ms unused */
 typedef enum gm_kstat {
   GM_KSTAT_JOIN = 0,
   GM_KSTAT_SGD = 1,
   GM_KSTAT_INSERT = 2,
   GM_KSTAT_SGD_STREAM = 3,
-  GM_KSTAT_COUNT = 4
+  GM_KSTAT_GRAPH = 4,
+  GM_KSTAT_COUNT = 5
 } gm_kstat;
 
 typedef struct gm_kernel_stats {

@@ -45,11 +45,15 @@ def test_initial_positions_match_oracle(mode):
                                    atol=1e-5, err_msg=f"after row {s + 400}")
 
 
+@pytest.mark.parametrize("graph", [True, False], ids=["graph", "direct"])
 @pytest.mark.parametrize("metric", [0, 1])
-def test_noise_and_reposition_match_oracle(metric):
+def test_noise_and_reposition_match_oracle(metric, graph, monkeypatch):
     # max_epoch = 0: every embedding op is the noise on its batch and the
     # reposition of the other rows, deterministic on both sides; partial ops
-    # (cursor sweeps, reposition active) and full sweeps, new-row queues
+    # (cursor sweeps, reposition active) and full sweeps, new-row queues.
+    # The op's kernel sequences run as HIP graphs (updated in place from op
+    # to op, re-instantiated when the batch size changes) or directly.
+    monkeypatch.setenv("GPUMAP_AMD_GRAPH", "1" if graph else "0")
     x = ev.gaussian10(n=3000, d=24, index=36)
     n, d = x.shape
     e = capi.Engine(n, d, knn_mode=1, max_epoch=0, metric=metric)
@@ -74,6 +78,12 @@ def test_noise_and_reposition_match_oracle(metric):
         assert r1.embedding_completed == r2.embedding_completed
         np.testing.assert_allclose(e.embedding(), o.embedding(), rtol=1e-5,
                                    atol=2e-5, err_msg=f"cycle {cyc}")
+    gs = e.kernel_stats(capi.KSTAT_GRAPH)
+    if graph:
+        # two sequences per op with a reposition, fewer instantiations
+        assert gs.launches >= len(plan) and 0 < gs.work[0] < gs.launches
+    else:
+        assert gs.launches == 0
 
 
 def _conflict_free_case(n=1 << 16, seed0=100):
