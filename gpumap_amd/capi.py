@@ -164,9 +164,10 @@ def philox(ctr, key):
 
 
 def sgd(emb, head, tail, w, n_epochs, nsr, a, b, alpha0=1.0, gamma=1.0, seed=42,
-        per_epoch=False, stream=False):
+        per_epoch=False, stream=False, pull=False):
     """gm_debug_sgd: all epochs in one launch of the kernel that fits, one
-    launch per epoch (per_epoch) or the streaming kernel (stream)."""
+    launch per epoch (per_epoch), the streaming kernel (stream) or the
+    row-owner kernel (pull)."""
     emb = np.ascontiguousarray(emb, np.float32).copy()
     head = np.ascontiguousarray(head, np.int32)
     tail = np.ascontiguousarray(tail, np.int32)
@@ -174,7 +175,8 @@ def sgd(emb, head, tail, w, n_epochs, nsr, a, b, alpha0=1.0, gamma=1.0, seed=42,
     check(lib().gm_debug_sgd(_p(emb, f32), emb.shape[0], emb.shape[1],
                              _p(head, i32), _p(tail, i32), _p(w, f32), len(w),
                              n_epochs, nsr, a, b, alpha0, gamma, seed,
-                             2 if stream else 1 if per_epoch else 0))
+                             3 if pull else 2 if stream else
+                             1 if per_epoch else 0))
     return emb
 
 

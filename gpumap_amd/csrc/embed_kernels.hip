@@ -1793,6 +1793,393 @@ __global__ __launch_bounds__(1024) void k_sgd_stream(
   }
 }
 
+// --------------------------------------------------- row-owner SGD
+// For edge sets beyond the resident grid (the full sweeps).  Every COO slot
+// (h, t) is listed twice, as an entry of row h (side 0) and of row t (side
+// 1), the entries grouped by row, each entry with its own copy of the
+// slot's schedule.  An epoch reads every row from the positions the
+// previous epoch left (a Jacobi step: no row moves while an epoch reads),
+// each entry computes its row's share of its slot -- on side 0 the head's
+// positive step and negative samples (the reference's head update,
+// embedding_kernel.cuh:509-589), on side 1 the tail's positive step, from
+// the same positions the head side reads -- and each run of a row's
+// entries within a wave adds its sum with one atomic.  The rows are read
+// from one of two record buffers, the records (B0) and PullFrame::d1 (B1, a
+// copy of the records' p0 with its own delta words), one load per row.
+// Epoch t reads B(t mod 2) and
+// adds into B(t + 1 mod 2) the run's sums of epochs t - 1 and t, so that
+// B(t + 1 mod 2) = D(t + 1), the sum of all updates of epochs 0 .. t (both
+// buffers start at zero).  The sums are integers in units of q, exact in
+// any order: the result does not depend on the order of the entries or of
+// the atomics.  A full grid barrier with agent-scope release and acquire
+// separates the epochs.
+
+// row-major counting of each slot under both its rows
+__global__ void k_pull_count(const int32_t *__restrict__ head,
+                             const int32_t *__restrict__ tail,
+                             const uint32_t *__restrict__ nnz_p,
+                             uint32_t *__restrict__ cnt) {
+  const uint64_t nnz = *nnz_p;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const int32_t h = head[e];
+    if (h < 0)
+      continue; // an empty slot of the twin-paired COO
+    atomicAdd(&cnt[h], 1u);
+    atomicAdd(&cnt[tail[e]], 1u);
+  }
+}
+
+// sum of cnt over each block of 1024 rows
+__global__ __launch_bounds__(1024) void k_pull_bsum(
+    const uint32_t *__restrict__ cnt, uint64_t n, uint32_t *__restrict__ bsum) {
+  __shared__ uint32_t s_w[16];
+  const uint64_t r = (uint64_t)blockIdx.x * 1024 + threadIdx.x;
+  uint32_t v = r < n ? cnt[r] : 0u;
+  for (int off = 32; off > 0; off >>= 1)
+    v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0)
+    s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < 16; ++w)
+      t += s_w[w];
+    bsum[blockIdx.x] = t;
+  }
+}
+
+// off[r] = bsum[block] (exclusive, scanned) + exclusive scan within the block
+__global__ __launch_bounds__(1024) void k_pull_offsets(
+    const uint32_t *__restrict__ cnt, uint64_t n,
+    const uint32_t *__restrict__ bsum, uint32_t *__restrict__ off) {
+  __shared__ uint32_t s[1024];
+  const uint64_t r = (uint64_t)blockIdx.x * 1024 + threadIdx.x;
+  const uint32_t x = r < n ? cnt[r] : 0u;
+  s[threadIdx.x] = x;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const uint32_t y = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += y;
+    __syncthreads();
+  }
+  if (r < n)
+    off[r] = bsum[blockIdx.x] + s[threadIdx.x] - x;
+}
+
+// the entries: slot e under its head (side 0) and its tail (side 1), placed
+// by each row's cursor (cnt, zeroed again); the order within a row does not
+// change the result
+__global__ void k_pull_fill(const int32_t *__restrict__ head,
+                            const int32_t *__restrict__ tail,
+                            const uint32_t *__restrict__ nnz_p,
+                            const float *__restrict__ eps,
+                            const float *__restrict__ nxt,
+                            const float *__restrict__ nneg, PullFrame pf) {
+  const uint64_t nnz = *nnz_p;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const int32_t h = head[e];
+    if (h < 0)
+      continue;
+    const int32_t t = tail[e];
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int32_t row = side ? t : h;
+      const uint32_t at = pf.off[row] + atomicAdd(&pf.cnt[row], 1u);
+      pf.erc[at] = make_int2(row, side ? h : t);
+      pf.esch[at] = make_float4(
+          __uint_as_float((uint32_t)e | (side ? 0x80000000u : 0u)), eps[e],
+          nxt[e], nneg[e]);
+    }
+  }
+}
+
+__global__ void k_fill_i32(int32_t *__restrict__ p, uint64_t n, int32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// d1 = the records' p0 with zero deltas (before the launch), or the
+// records' deltas = d1's (after an odd number of epochs, when the final
+// sums are in d1)
+template <int NC>
+__global__ void k_pull_copy(int32_t *__restrict__ d1,
+                            int32_t *__restrict__ rec, uint64_t n, int after) {
+  constexpr int F = fix_stride<NC>();
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (uint64_t)gridDim.x * blockDim.x)
+#pragma unroll
+    for (int d = 0; d < F; ++d) {
+      if (after)
+        rec[r * (2 * F) + F + d] = d1[r * (2 * F) + F + d];
+      else {
+        d1[r * (2 * F) + d] = rec[r * (2 * F) + d];
+        d1[r * (2 * F) + F + d] = 0;
+      }
+    }
+}
+
+// full grid barrier between epochs: every wave's row updates complete
+// before the block arrives (each wave waits for its own memory operations;
+// the arriving thread's release fence then covers the CU's caches), every
+// row read after the block leaves (the leaving thread's acquire fence
+// invalidates the CU's and the XCD's cached lines before any wave reads)
+__device__ inline void pull_barrier(unsigned *bar, unsigned nblocks,
+                                    int epoch, unsigned &seen) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_s_barrier();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned target = (unsigned)(epoch + 1) * nblocks;
+    unsigned spins = 0;
+    while (seen < target) {
+      seen = bar_peek(bar);
+      if (seen >= target)
+        break;
+      __builtin_amdgcn_s_sleep(1);
+      // bounded: a timed-out wait anywhere in the launch ends the others
+      if ((++spins & 1023u) == 0 &&
+          (spins > (1u << 22) || bar_peek(bar + 2))) {
+        __hip_atomic_store(bar + 2, 1u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __builtin_amdgcn_s_barrier();
+}
+
+template <int NC>
+__global__ __launch_bounds__(1024) void k_sgd_pull(
+    PullFrame pf, int32_t *__restrict__ rec, const uint32_t *__restrict__ nent_p,
+    uint32_t n, int n_epochs, float alpha0, float gamma, float a, float b,
+    float nsr_inv, uint64_t seed, const float *__restrict__ rounding_p,
+    unsigned *__restrict__ bar, unsigned long long *__restrict__ stats) {
+  constexpr int F = fix_stride<NC>();
+  const uint64_t nent = *nent_p;
+  const float rounding = *rounding_p;
+  const float q = ldexpf(rounding, -24); // a power of two, as is 1 / q
+  const float qinv = 1.0f / q;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const int lane = threadIdx.x & 63;
+  // whole waves iterate together (the runs are wave operations)
+  const uint64_t w0 = (uint64_t)blockIdx.x * blockDim.x +
+                      (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x - lane);
+  unsigned long long c_pos = 0, c_neg = 0;
+  unsigned seen = 0;
+  float alpha = alpha0;
+  for (int epoch = 0; epoch < n_epochs; ++epoch) {
+    const bool odd = epoch & 1;
+    const FixRows<NC> rw{odd ? pf.d1 : rec, q};
+    int32_t *const wb = (odd ? rec : pf.d1) + F; // delta words, stride 2F
+    const uint64_t es = seed + (uint64_t)epoch;
+    // the walk is pipelined: the next entry's words (and its run's sum of
+    // the previous epoch) load while this entry works
+    uint64_t base = w0;
+    int2 rc_c = make_int2(0, 0), rc_n;
+    float4 sc_c = make_float4(0.f, 0.f, 0.f, 0.f), sc_n;
+    int32_t pc_c[NC], pc_n[NC];
+    auto load_entry = [&](uint64_t e0, int2 &rc, float4 &sc,
+                          int32_t (&pc)[NC]) {
+      const uint64_t ei = e0 < nent ? e0 : nent - 1;
+      rc = pf.erc[ei];
+      sc = pf.esch[ei];
+#pragma unroll
+      for (int d = 0; d < NC; ++d)
+        pc[d] = pf.epc[ei * NC + d];
+    };
+    if (base < nent)
+      load_entry(base + lane, rc_c, sc_c, pc_c);
+    for (; base < nent; base += stride) {
+      load_entry(base + stride + lane, rc_n, sc_n, pc_n);
+      const uint64_t e = base + lane;
+      const bool valid = e < nent;
+      const int32_t row = rc_c.x, oth = rc_c.y;
+      const uint32_t sl = __float_as_uint(sc_c.x);
+      const float ep = sc_c.y;
+      float nx = sc_c.z, nn = sc_c.w;
+      const bool tail_side = (sl >> 31) != 0;
+      const uint64_t slot = sl & 0x7fffffffu;
+      const bool act = valid && !(nx > (float)epoch);
+      const bool negs = act && !tail_side;
+      // the slot's head and tail, as the reference's edge reads them
+      const int32_t hh = tail_side ? oth : row, tt = tail_side ? row : oth;
+      const float ep_neg = ep * nsr_inv;
+      float gh[NC], gt[NC];
+#pragma unroll
+      for (int d = 0; d < NC; ++d)
+        gh[d] = gt[d] = 0.0f;
+      if (act) {
+        // only the sampled entries load: both rows, and on the head side
+        // the draws and rows of the first 8 negative samples
+        EdgeLoads<NC> L;
+        L.n_neg = negs ? (int)(((float)epoch - nn) / ep_neg) : 0;
+        rw.load((uint64_t)hh, L.cur);
+        rw.load((uint64_t)tt, L.oth);
+        if (L.n_neg > 0) {
+          uint32_t dr[8];
+          neg_draws(es, slot, dr);
+          const int cnt = min(8, L.n_neg);
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            L.r[k] = __umulhi(dr[k], n);
+            if (k < cnt)
+              rw.load(L.r[k], L.neg[k]);
+          }
+        }
+        unsigned long long cp = 0, cn = 0;
+        sgd_apply_rows<NC>(rw, hh, slot, n, ep_neg, alpha, gamma, a, b, es,
+                           rounding, L, gh, gt, cp, cn);
+        if (!tail_side) {
+          c_pos += cp;
+          c_neg += cn;
+        }
+        nx += ep;
+        if (L.n_neg > 0)
+          nn += (float)L.n_neg * ep_neg;
+        // the schedule back (slot and eps unchanged)
+        reinterpret_cast<float2 *>(pf.esch + e)[1] = make_float2(nx, nn);
+      }
+      // this entry's share of its row, in units of q
+      int32_t dx[NC];
+#pragma unroll
+      for (int d = 0; d < NC; ++d)
+        dx[d] = act ? (int32_t)rintf((tail_side ? gt[d] : gh[d]) * qinv) : 0;
+      // runs of equal rows over the wave (entries grouped by row): each
+      // run's sum into its first lane
+      const int32_t key = valid ? row : -1 - lane;
+      const int32_t prev = __shfl_up(key, 1, 64);
+      const bool start = lane == 0 || prev != key;
+      const uint64_t sm = __ballot(start);
+      const uint64_t after = lane == 63 ? 0ull : (sm >> (lane + 1));
+      const int nstart = after ? lane + 1 + __builtin_ctzll(after) : 64;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+        for (int d = 0; d < NC; ++d) {
+          const int32_t t = __shfl_down(dx[d], o, 64);
+          if (lane + o < nstart)
+            dx[d] += t;
+        }
+      }
+      if (start && valid) {
+        // the run's sums of the previous epoch and this one
+        int32_t tot[NC];
+#pragma unroll
+        for (int d = 0; d < NC; ++d) {
+          tot[d] = pc_c[d] + dx[d];
+          pf.epc[e * NC + d] = dx[d];
+        }
+        int32_t *p = wb + (uint64_t)row * (2 * F);
+#pragma unroll
+        for (int d = 0; d + 1 < NC; d += 2) {
+          const unsigned long long pk = fix_pack(tot[d], tot[d + 1]);
+          if (pk != 0ull)
+            __hip_atomic_fetch_add(reinterpret_cast<unsigned long long *>(p + d),
+                                   pk, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if constexpr (NC & 1) {
+          if (tot[NC - 1] != 0)
+            __hip_atomic_fetch_add(reinterpret_cast<unsigned *>(p + NC - 1),
+                                   (unsigned)tot[NC - 1], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      rc_c = rc_n;
+      sc_c = sc_n;
+#pragma unroll
+      for (int d = 0; d < NC; ++d)
+        pc_c[d] = pc_n[d];
+    }
+    alpha = alpha0 * (1.0f - (float)epoch / (float)n_epochs);
+    pull_barrier(bar, gridDim.x, epoch, seen);
+  }
+  flush_sgd_stats(stats, c_pos, c_neg);
+}
+
+template <int NC>
+static hipError_t launch_pull_nc(const PullFrame &pf, FixFrame fx,
+                                 const int32_t *head, const int32_t *tail,
+                                 const uint32_t *nnz_p, uint64_t cap,
+                                 uint32_t n, const float *eps,
+                                 const float *nxt, const float *nneg,
+                                 int n_epochs, float alpha0, float gamma,
+                                 float a, float b, float nsr_inv,
+                                 uint64_t seed, const float *rounding_p,
+                                 unsigned *bar, unsigned long long *stats,
+                                 int num_cu, hipStream_t s) {
+  static int per_cu = 0;
+  if (per_cu <= 0)
+    per_cu = blocks_per_cu(&k_sgd_pull<NC>, 1024);
+  if (per_cu <= 0 || !fx.rec || cap == 0 || cap > pf.slots || n > pf.rows)
+    return hipErrorInvalidConfiguration;
+  const unsigned g = (unsigned)std::max<uint64_t>(
+      1, std::min<uint64_t>((cap + 255) / 256, 16 * (uint64_t)num_cu));
+  const unsigned nb = (unsigned)((n + 1023) / 1024);
+  k_fill_i32<<<g, 256, 0, s>>>(reinterpret_cast<int32_t *>(pf.cnt), n + 1, 0);
+  k_pull_count<<<g, 256, 0, s>>>(head, tail, nnz_p, pf.cnt);
+  k_pull_bsum<<<nb, 1024, 0, s>>>(pf.cnt, n, pf.bsum);
+  k_scan_small<<<1, 1024, 0, s>>>(pf.bsum, nb, pf.off + n);
+  k_pull_offsets<<<nb, 1024, 0, s>>>(pf.cnt, n, pf.bsum, pf.off);
+  k_fill_i32<<<g, 256, 0, s>>>(reinterpret_cast<int32_t *>(pf.cnt), n + 1, 0);
+  k_pull_fill<<<g, 256, 0, s>>>(head, tail, nnz_p, eps, nxt, nneg, pf);
+  k_fill_i32<<<g, 256, 0, s>>>(pf.epc, 2 * cap * NC, 0);
+  k_pull_copy<NC><<<g, 256, 0, s>>>(pf.d1, fx.rec, n, 0);
+  const hipError_t le = hipGetLastError();
+  if (le != hipSuccess)
+    return le;
+  // the co-resident grid, no more blocks than the most entries need
+  const uint64_t blocks = std::min<uint64_t>((uint64_t)per_cu * num_cu,
+                                             (2 * cap + 1023) / 1024);
+  PullFrame pfa = pf;
+  int32_t *reca = fx.rec;
+  const uint32_t *nent_p = pf.off + n;
+  stats = stats ? stats + 8 : nullptr; // the streaming kernels' slots
+  void *args[] = {&pfa,  &reca,    &nent_p, &n,    &n_epochs,
+                  &alpha0, &gamma, &a,      &b,    &nsr_inv,
+                  &seed, &rounding_p, &bar, &stats};
+  const hipError_t e = hipLaunchCooperativeKernel(
+      reinterpret_cast<const void *>(&k_sgd_pull<NC>), dim3((unsigned)blocks),
+      dim3(1024), args, 0, s);
+  if (e != hipSuccess)
+    return e;
+  if (n_epochs & 1)
+    k_pull_copy<NC><<<g, 256, 0, s>>>(pf.d1, fx.rec, n, 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_sgd_pull(int nc, const PullFrame &pf, FixFrame fx,
+                           const int32_t *head, const int32_t *tail,
+                           const uint32_t *nnz_p, uint64_t cap, uint32_t n,
+                           const float *eps, const float *nxt,
+                           const float *nneg, int n_epochs, float alpha0,
+                           float gamma, float a, float b, float nsr_inv,
+                           uint64_t seed, const float *rounding_p,
+                           unsigned *bar, unsigned long long *stats,
+                           int num_cu, hipStream_t s) {
+#define GM_PULL(NC)                                                            \
+  return launch_pull_nc<NC>(pf, fx, head, tail, nnz_p, cap, n, eps, nxt, nneg, \
+                            n_epochs, alpha0, gamma, a, b, nsr_inv, seed,     \
+                            rounding_p, bar, stats, num_cu, s)
+  switch (nc) {
+  case 1:
+    GM_PULL(1);
+  case 2:
+    GM_PULL(2);
+  case 3:
+    GM_PULL(3);
+  default:
+    GM_PULL(4);
+  }
+#undef GM_PULL
+}
+
 // Host launchers of the cooperative SGD kernels, in the kernels' own
 // translation unit: the register-resident kernel with 1 or 2 edges per
 // thread when its co-resident grid holds every edge, else the streaming
@@ -1895,9 +2282,9 @@ static hipError_t try_fused_any(float *emb, const int32_t *head,
                                 unsigned *bar, unsigned long long *stats,
                                 int lag, uint64_t nnz, int num_cu,
                                 hipStream_t s, int *variant,
-                                bool force_stream, FixFrame fx) {
+                                int stream_mode, FixFrame fx) {
   *variant = 1;
-  if (force_stream)
+  if (stream_mode == SGD_STREAM_FORCE)
     return try_stream<NC>(emb, head, tail, nnz_p, n, eps, nxt, nneg, n_epochs,
                           alpha0, gamma, a, b, nsr_inv, seed, rounding_p, bar,
                           stats, lag, nnz, num_cu, s, fx);
@@ -1923,6 +2310,8 @@ static hipError_t try_fused_any(float *emb, const int32_t *head,
   GM_TRY(1)
   GM_TRY(2)
 #undef GM_TRY
+  if (stream_mode == SGD_STREAM_NONE)
+    return hipErrorInvalidConfiguration;
   *variant = 1;
   return try_stream<NC>(emb, head, tail, nnz_p, n, eps, nxt, nneg, n_epochs,
                         alpha0, gamma, a, b, nsr_inv, seed, rounding_p, bar,
@@ -1968,28 +2357,28 @@ hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
                             uint64_t seed, const float *rounding_p,
                             unsigned *bar, unsigned long long *stats, int lag,
                             uint64_t nnz, int num_cu, hipStream_t s,
-                            int *variant, bool force_stream, FixFrame fx) {
+                            int *variant, int stream_mode, FixFrame fx) {
   switch (nc) {
   case 1:
     return try_fused_any<1>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
                             n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
                             rounding_p, bar, stats, lag, nnz, num_cu, s, variant,
-                            force_stream, fx);
+                            stream_mode, fx);
   case 2:
     return try_fused_any<2>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
                             n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
                             rounding_p, bar, stats, lag, nnz, num_cu, s, variant,
-                            force_stream, fx);
+                            stream_mode, fx);
   case 3:
     return try_fused_any<3>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
                             n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
                             rounding_p, bar, stats, lag, nnz, num_cu, s, variant,
-                            force_stream, fx);
+                            stream_mode, fx);
   default:
     return try_fused_any<4>(emb, head, tail, nnz_p, n, eps, nxt, nneg,
                             n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
                             rounding_p, bar, stats, lag, nnz, num_cu, s, variant,
-                            force_stream, fx);
+                            stream_mode, fx);
   }
 }
 

@@ -181,11 +181,13 @@ void set_last_error(const char *msg) {
 
 // The SGD's launches on a prepared schedule (eps, nxt, nneg, the rounding
 // grid and zeroed barrier words, sgd_setup's work): all epochs in one
-// cooperative launch of the kernel that fits the edge set (SGD_AUTO), the
-// streaming cooperative kernel (SGD_STREAM), or one launch per epoch
-// (SGD_PER_EPOCH, and whenever no cooperative launch is possible).  Returns
-// true when a cooperative kernel ran (its timeout flag is gbar[2]); *variant
-// names it (2 per-wave slots, 0 resident counter grid, 1 streaming).
+// cooperative launch of the kernel that fits the edge set (SGD_AUTO: the
+// register-resident kernels, else the row-owner kernel when `pf` is given,
+// else the streaming kernel), the streaming cooperative kernel (SGD_STREAM),
+// the row-owner kernel (SGD_PULL), or one launch per epoch (SGD_PER_EPOCH,
+// and whenever no cooperative launch is possible).  Returns true when a
+// cooperative kernel ran (its timeout flag is gbar[2]); *variant names it
+// (2 per-wave slots, 0 resident counter grid, 1 streaming, 3 row-owner).
 // Epoch lag of the cooperative SGD: a block starts epoch e + 1 once every
 // block has finished epoch e - SGD_LAG (0 would be a full grid barrier per
 // epoch, the reference's one launch per epoch).  The Hogwild updates are
@@ -198,16 +200,37 @@ bool sgd_launches(int c, float *e, const int32_t *head, const int32_t *tail,
                   const float *rounding, unsigned *gbar,
                   unsigned long long *st, uint64_t cap, int num_cu,
                   hipStream_t s, int sgd_mode, int *variant, bool verbose,
-                  FixFrame fx) {
+                  FixFrame fx, const PullFrame *pf) {
   if (sgd_mode != SGD_PER_EPOCH) {
-    // the per-wave and streaming kernels run on the fixed-point copy (made
-    // here, folded back below when one of them ran)
+    // the per-wave, streaming and row-owner kernels run on the fixed-point
+    // copy (made here, folded back below when one of them ran)
     if (fx.rec)
       HIPC(launch_fix(c, 0, e, nrows, fx, rounding, num_cu, s));
-    const hipError_t fe = launch_sgd_fused(
-        c, e, head, tail, nnz_p, nrows, eps, nxt, nneg, n_epochs, alpha0,
-        gamma, a, b, nsr_inv, seed, rounding, gbar, st, SGD_LAG, cap, num_cu,
-        s, variant, sgd_mode == SGD_STREAM, fx);
+    const bool pull = pf && fx.rec && sgd_mode != SGD_STREAM;
+    hipError_t fe = hipErrorInvalidConfiguration;
+    if (sgd_mode != SGD_PULL)
+      fe = launch_sgd_fused(
+          c, e, head, tail, nnz_p, nrows, eps, nxt, nneg, n_epochs, alpha0,
+          gamma, a, b, nsr_inv, seed, rounding, gbar, st, SGD_LAG, cap,
+          num_cu, s, variant,
+          sgd_mode == SGD_STREAM ? SGD_STREAM_FORCE
+          : pull                 ? SGD_STREAM_NONE
+                                 : SGD_STREAM_FALLBACK,
+          fx);
+    if (fe == hipErrorInvalidConfiguration && pull) {
+      (void)hipGetLastError();
+      *variant = 3;
+      fe = launch_sgd_pull(c, *pf, fx, head, tail, nnz_p, cap, nrows, eps,
+                           nxt, nneg, n_epochs, alpha0, gamma, a, b, nsr_inv,
+                           seed, rounding, gbar, st, num_cu, s);
+      if (fe == hipErrorInvalidConfiguration) {
+        (void)hipGetLastError();
+        fe = launch_sgd_fused(c, e, head, tail, nnz_p, nrows, eps, nxt, nneg,
+                              n_epochs, alpha0, gamma, a, b, nsr_inv, seed,
+                              rounding, gbar, st, SGD_LAG, cap, num_cu, s,
+                              variant, SGD_STREAM_FORCE, fx);
+      }
+    }
     if (fe == hipSuccess) {
       if (fx.rec && *variant != 0)
         HIPC(launch_fix(c, 1, e, nrows, fx, rounding, num_cu, s));
@@ -383,6 +406,17 @@ struct Engine {
   // the float embedding
   int32_t *fxp = nullptr;
   bool sgd_fix = true;
+  // With GPUMAP_AMD_SGD_PULL=1, embedding ops whose edge sets exceed the
+  // register-resident kernels (PULL_MIN_SLOTS slots) run on the row-owner
+  // kernel: reproducible bit for bit from run to run (every epoch reads the
+  // positions the previous one left, integer sums), measured 10% faster
+  // than the streaming kernel on the 70,000 x 784 full sweep but 1.5x and
+  // 2.6x slower on the 200,000 x 300 (k = 30) and 2,000,000 x 128 (3-D)
+  // ones, so off by default.  Its scratch is allocated on first use, sized
+  // for a full sweep.
+  PullFrame pullf{};
+  bool sgd_pull = false;
+  static constexpr uint64_t PULL_MIN_SLOTS = 1u << 17;
   // With GPUMAP_AMD_GRAPH=1 the embedding op's kernel sequences before and
   // after its SGD launch run as HIP graphs: captured once (stream capture)
   // and relaunched while the host values they depend on stay the same, the
@@ -534,6 +568,8 @@ struct Engine {
       sgd_fix = v[0] != '0';
     if (const char *v = std::getenv("GPUMAP_AMD_GRAPH"))
       use_graph = v[0] != '0';
+    if (const char *v = std::getenv("GPUMAP_AMD_SGD_PULL"))
+      sgd_pull = v[0] != '0';
     try {
       allocate();
     } catch (...) {
@@ -721,13 +757,17 @@ struct Engine {
         (void **)&wflags, (void **)&head,    (void **)&tail,    (void **)&wout,
         (void **)&eps,    (void **)&nxt,     (void **)&nneg,    (void **)&snap,
         (void **)&rounding, (void **)&deg,   (void **)&maxdeg,  (void **)&dbatch,
-        (void **)&bmark,  (void **)&isc,     (void **)&fxp};
+        (void **)&bmark,  (void **)&isc,     (void **)&fxp,
+        (void **)&pullf.cnt, (void **)&pullf.off, (void **)&pullf.bsum,
+        (void **)&pullf.erc, (void **)&pullf.esch, (void **)&pullf.epc,
+        (void **)&pullf.d1};
     for (void **q : ptrs) {
       if (*q)
         (void)hipFree(*q);
       *q = nullptr;
     }
     part_cap = ins_cap = xb_cap = 0;
+    pullf.slots = pullf.rows = 0;
     if (h_stage)
       (void)hipHostFree(h_stage);
     if (h_list)
@@ -1191,6 +1231,32 @@ struct Engine {
     ev_stop(1);
   }
 
+  // the row-owner SGD's scratch for COOs of up to `slots` slots over the
+  // session's N rows
+  void ensure_pull(uint64_t slots) {
+    if (pullf.slots >= slots && pullf.rows >= N)
+      return;
+    for (void **q :
+         {(void **)&pullf.cnt, (void **)&pullf.off, (void **)&pullf.bsum,
+          (void **)&pullf.erc, (void **)&pullf.esch, (void **)&pullf.epc,
+          (void **)&pullf.d1}) {
+      if (*q)
+        (void)hipFree(*q);
+      *q = nullptr;
+    }
+    pullf.slots = pullf.rows = 0;
+    const uint64_t ent = 2 * slots;
+    pullf.cnt = dalloc<uint32_t>(N + 1);
+    pullf.off = dalloc<uint32_t>(N + 1);
+    pullf.bsum = dalloc<uint32_t>((N + 1023) / 1024 + 1);
+    pullf.erc = dalloc<int2>(ent);
+    pullf.esch = dalloc<float4>(ent);
+    pullf.epc = dalloc<int32_t>(ent * (uint64_t)c);
+    pullf.d1 = dalloc<int32_t>(N * 2 * (uint64_t)(c == 3 ? 4 : c));
+    pullf.slots = slots;
+    pullf.rows = N;
+  }
+
   // batch operand store, tau and survivor queue of the BF16 screening
   void ensure_bf16_batch(uint64_t B) {
     if (B <= xb_cap)
@@ -1392,24 +1458,31 @@ struct Engine {
     if (!set_up)
       sgd_setup(nrows, cap, n_epochs, nsr_inv, alpha0, deg_zeroed);
     unsigned long long *st = prof ? kstats : nullptr;
+    const bool want_pull =
+        sgd_fix && sgd_pull &&
+        (sgd_mode == SGD_PULL ||
+         (sgd_mode == SGD_AUTO && cap >= PULL_MIN_SLOTS));
+    if (want_pull)
+      ensure_pull(std::max<uint64_t>(cap, 2 * N * (uint64_t)k));
     ev_start(GM_KSTAT_SGD);
     const bool fused = sgd_launches(
         c, e, head, tail, scal + S_NNZ, (uint32_t)nrows, eps, nxt, nneg,
         n_epochs, alpha0, gamma, a, b, nsr_inv, seed, rounding, gbar, st, cap,
         num_cu, s, sgd_mode, &sgd_variant, p.verbose != 0,
-        FixFrame{sgd_fix ? fxp : nullptr});
+        FixFrame{sgd_fix ? fxp : nullptr}, want_pull ? &pullf : nullptr);
+    // the streaming and row-owner kernels count as the large-set SGD
+    const bool large = fused && (sgd_variant == 1 || sgd_variant == 3);
     if (fused) {
       fused_used = true;
       if (sgd_variant == 2)
         ++sgd_waves_launches;
-      if (prof && sgd_variant == 1)
+      if (prof && large)
         pending.back().which = GM_KSTAT_SGD_STREAM;
     }
     ev_stop(fused ? 1 : (uint64_t)n_epochs);
     if (prof) {
       k_add_edge_epochs<<<1, 64, 0, s>>>(
-          scal + S_NNZ, n_epochs,
-          kstats + (fused && sgd_variant == 1 ? KS_STREAM_EE : KS_SGD_EE));
+          scal + S_NNZ, n_epochs, kstats + (large ? KS_STREAM_EE : KS_SGD_EE));
       LAUNCH_CHECK();
     }
   }
@@ -2496,7 +2569,7 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
   gm_handle *h = nullptr;
   int rc = guarded([&] {
     if (n == 0 || c < 1 || c > 4 || nnz < 0 || nsr < 1 || per_epoch < 0 ||
-        per_epoch > 2)
+        per_epoch > 3)
       throw Fail(GM_ERR_INVALID, "invalid SGD arguments");
     // the SGD kernels address the embedding with 32-bit byte offsets (the
     // limit gm_create applies to n_instances * n_components)
@@ -2540,6 +2613,7 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
                  seed,
                  per_epoch == 1   ? gm::SGD_PER_EPOCH
                  : per_epoch == 2 ? gm::SGD_STREAM
+                 : per_epoch == 3 ? gm::SGD_PULL
                                   : gm::SGD_AUTO);
     HIPC(hipMemcpyAsync(emb, e->emb, n * c * 4, hipMemcpyDeviceToHost, e->s));
     HIPC(hipStreamSynchronize(e->s));

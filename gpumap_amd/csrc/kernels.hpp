@@ -236,9 +236,26 @@ hipError_t launch_sgd_batch(int nc, const SgdBatch &batch, unsigned waves,
 struct FixFrame {
   int32_t *rec;
 };
+// Scratch of the row-owner SGD kernel (k_sgd_pull, embed_kernels.hip) for
+// edge sets beyond the resident grid: every COO slot listed under both its
+// rows (an entry), the entries grouped by row, each with its own copy of the
+// slot's schedule; sized for up to `slots` COO slots and `rows` rows.
+struct PullFrame {
+  uint32_t *cnt;   // rows + 1: entries per row, then the fill cursors
+  uint32_t *off;   // rows + 1: first entry of each row, off[n] = entries
+  uint32_t *bsum;  // per 1024 rows: block sums of cnt
+  int2 *erc;       // entry's row, the slot's other row
+  // slot | side << 31 (side 1: the row is the slot's tail; as float bits),
+  // then the slot's schedule: eps, next sample, next negative sample (used
+  // on the head side)
+  float4 *esch;
+  int32_t *epc;    // per entry n_components ints: a run's previous sum
+  int32_t *d1;     // second record buffer: rows x 2 fix_stride words
+  uint64_t slots, rows;
+};
 // cooperative launch of k_sgd_waves (*variant 2), k_sgd_fused (0) or, for
-// edge sets larger than its resident grid or with force_stream,
-// k_sgd_stream (1, counting its work at stats + 8) (embed_kernels.hip);
+// edge sets larger than its resident grid or when forced, k_sgd_stream (1,
+// counting its work at stats + 8) (embed_kernels.hip);
 // waves and stream run on fx's copy when it has one; returns the launch
 // error when none can be launched
 hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
@@ -249,7 +266,24 @@ hipError_t launch_sgd_fused(int nc, float *emb, const int32_t *head,
                             uint64_t seed, const float *rounding_p,
                             unsigned *bar, unsigned long long *stats, int lag,
                             uint64_t nnz, int num_cu, hipStream_t s,
-                            int *variant, bool force_stream, FixFrame fx);
+                            int *variant, int stream_mode, FixFrame fx);
+// stream_mode of launch_sgd_fused: the streaming kernel as the fallback
+// (SGD_STREAM_FALLBACK), always (SGD_STREAM_FORCE) or never
+// (SGD_STREAM_NONE: hipErrorInvalidConfiguration instead)
+constexpr int SGD_STREAM_FALLBACK = 0, SGD_STREAM_FORCE = 1,
+              SGD_STREAM_NONE = 2;
+// the row-owner SGD (k_sgd_pull) on a COO of up to `cap` slots over n rows:
+// builds pf's entries, then one cooperative launch of all epochs on fx's
+// records; hipErrorInvalidConfiguration when pf or fx cannot hold it
+hipError_t launch_sgd_pull(int nc, const PullFrame &pf, FixFrame fx,
+                           const int32_t *head, const int32_t *tail,
+                           const uint32_t *nnz_p, uint64_t cap, uint32_t n,
+                           const float *eps, const float *nxt,
+                           const float *nneg, int n_epochs, float alpha0,
+                           float gamma, float a, float b, float nsr_inv,
+                           uint64_t seed, const float *rounding_p,
+                           unsigned *bar, unsigned long long *stats,
+                           int num_cu, hipStream_t s);
 // the fixed-point copy's kernels around a cooperative SGD launch (phase 0
 // before it, phase 1 after a kernel that used the copy)
 hipError_t launch_fix(int nc, int phase, float *emb, uint64_t n, FixFrame fx,
@@ -267,7 +301,7 @@ void set_last_error(const char *msg);
 
 // engine.hip: the SGD's launches on a prepared schedule (engine and the
 // allocation-free entry point share them)
-constexpr int SGD_AUTO = 0, SGD_PER_EPOCH = 1, SGD_STREAM = 2;
+constexpr int SGD_AUTO = 0, SGD_PER_EPOCH = 1, SGD_STREAM = 2, SGD_PULL = 3;
 constexpr int SGD_LAG = 1;
 bool sgd_launches(int c, float *e, const int32_t *head, const int32_t *tail,
                   const uint32_t *nnz_p, uint32_t nrows, const float *eps,
@@ -275,7 +309,8 @@ bool sgd_launches(int c, float *e, const int32_t *head, const int32_t *tail,
                   float gamma, float a, float b, float nsr_inv, uint64_t seed,
                   const float *rounding, unsigned *gbar,
                   unsigned long long *st, uint64_t cap, int num_cu,
-                  hipStream_t s, int sgd_mode, int *variant, bool verbose, FixFrame fx);
+                  hipStream_t s, int sgd_mode, int *variant, bool verbose,
+                  FixFrame fx, const PullFrame *pf = nullptr);
 __global__ void k_reposition(const uint32_t *pidx, const float *pdist,
                              NRows n, int k, int euclid, const double *sigma,
                              const double *rho, const double *mean,

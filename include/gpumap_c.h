@@ -215,8 +215,9 @@ int gm_debug_fuzzy(gm_handle *h, const uint32_t *batch, uint64_t n_batch,
  * (host), with the engine's kernels: per_epoch 0 runs all epochs in one
  * cooperative launch of the kernel that fits the edge set (one launch per
  * epoch when none can be launched), 1 one launch per epoch, 2 the streaming
- * cooperative kernel the engine uses for edge sets beyond the resident grid,
- * at any size.  A slot with head = tail = -1 is empty (the
+ * cooperative kernel, 3 the row-owner kernel the engine uses for edge sets
+ * beyond the resident grid, at any size.  A slot with head = tail = -1 is
+ * empty (the
  * engine's own COO keeps each edge's reverse in the slot after it, or an
  * empty slot). */
 int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
@@ -265,8 +266,9 @@ int gm_sgd_noalloc(float *emb, uint64_t n, int32_t c, const int32_t *head,
  *                        {(batch row, row) pairs, BF16 screening survivors
  *                        re-ranked in FP32, insertions redone by the FP32
  *                        tiles after a survivor-list overflow}
- *   GM_KSTAT_SGD_STREAM  SGD with the streaming cooperative kernel (edge sets
- *                        larger than the register-resident grid); work as
+ *   GM_KSTAT_SGD_STREAM  SGD of edge sets larger than the register-resident
+ *                        grid: the row-owner kernel (or, without its
+ *                        scratch, the streaming kernel); work as
  *                        GM_KSTAT_SGD
  *   GM_KSTAT_GRAPH       the embedding op's kernel sequences run as HIP
  *                        graphs (counted with or without profiling);

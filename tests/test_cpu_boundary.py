@@ -155,8 +155,9 @@ def test_no_cpu_fallback_without_gpu():
 
 def test_debug_sgd_rejects_unknown_mode_before_any_device_call():
     # gm_debug_sgd's mode: 0 the kernel that fits, 1 one launch per epoch,
-    # 2 the streaming kernel; anything else is refused by the argument check
-    # that runs before the engine touches the device
+    # 2 the streaming kernel, 3 the row-owner kernel; anything else is
+    # refused by the argument check that runs before the engine touches the
+    # device
     import ctypes
 
     import numpy as np
@@ -168,7 +169,7 @@ def test_debug_sgd_rejects_unknown_mode_before_any_device_call():
     tail = np.array([1, 0], np.int32)
     w = np.ones(2, np.float32)
     f32, i32 = ctypes.c_float, ctypes.c_int32
-    for mode in (-1, 3):
+    for mode in (-1, 4):
         rc = capi.lib().gm_debug_sgd(
             capi._p(emb, f32), 4, 2, capi._p(head, i32), capi._p(tail, i32),
             capi._p(w, f32), 2, 2, 1, ctypes.c_float(1.577),

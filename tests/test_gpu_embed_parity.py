@@ -121,16 +121,19 @@ def test_sgd_with_negative_samples_matches_oracle():
     np.testing.assert_array_equal(got[16:], emb[16:])
 
 
-@pytest.mark.parametrize("stream", [False, True])
-def test_sgd_cooperative_kernels_match_oracle(stream):
-    # the cooperative kernels (per-wave k_sgd_waves, streaming k_sgd_stream):
-    # k_sgd_waves reads an epoch's positions before the previous epoch's
-    # updates have landed and adds its own edge's last step to them, so on
-    # the conflict-free set both follow the oracle's trajectory
+@pytest.mark.parametrize("kernel,epochs", [("waves", 30), ("stream", 30),
+                                           ("pull", 30), ("pull", 29)])
+def test_sgd_cooperative_kernels_match_oracle(kernel, epochs):
+    # the cooperative kernels (per-wave k_sgd_waves, streaming k_sgd_stream,
+    # row-owner k_sgd_pull): k_sgd_waves reads an epoch's positions before the
+    # previous epoch's updates have landed and adds its own edge's last step
+    # to them, k_sgd_pull reads every row as the previous epoch left it, so
+    # on the conflict-free set all follow the oracle's trajectory (29 epochs:
+    # k_sgd_pull's sums end in its second buffer)
     emb, head, tail, w, seed, _ = _conflict_free_case()
-    want = oracle.sgd(emb, head, tail, w, 30, 5, 1.577, 0.895, seed=seed)
-    got = capi.sgd(emb, head, tail, w, 30, 5, 1.577, 0.895, seed=seed,
-                   stream=stream)
+    want = oracle.sgd(emb, head, tail, w, epochs, 5, 1.577, 0.895, seed=seed)
+    got = capi.sgd(emb, head, tail, w, epochs, 5, 1.577, 0.895, seed=seed,
+                   stream=kernel == "stream", pull=kernel == "pull")
     np.testing.assert_array_equal(got[16:], emb[16:])
     print(f"max |device - oracle| {np.abs(got[:16] - want[:16]).max():.3g}",
           flush=True)

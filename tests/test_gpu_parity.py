@@ -465,6 +465,43 @@ def test_sgd_stream_statistics_match_oracle():
     assert abs(tg - to) <= 0.01, (tg, to)
 
 
+def test_sgd_pull_equals_per_epoch_launches():
+    # the row-owner kernel (the engine's choice for edge sets beyond the
+    # resident grid) on the streaming test's conflict-free set: every edge
+    # takes one attraction step, computed by both its rows' entries
+    g = np.random.default_rng(16)
+    n = 1 << 20
+    emb = g.standard_normal((n, 2)).astype(np.float32)
+    h = np.arange(0, n, 2, dtype=np.int32)
+    t = h + 1
+    w = np.ones(h.size, np.float32)
+    pull = capi.sgd(emb, h, t, w, 2, 1, 1.577, 0.895, pull=True)
+    per = capi.sgd(emb, h, t, w, 2, 1, 1.577, 0.895, per_epoch=True)
+    np.testing.assert_array_equal(pull, per)
+    assert (pull != emb).any(axis=1).mean() > 0.99
+
+
+@pytest.mark.parametrize("c", [2, 3])
+def test_sgd_pull_statistics_match_oracle_and_repeat_exactly(c):
+    # the row-owner kernel on a real fuzzy graph with negative sampling: the
+    # layout's trustworthiness within 0.01 of the oracle's, and the same
+    # result bit for bit on a second run (every epoch reads the positions
+    # the previous one left, and the integer sums do not depend on order)
+    x = ev.gaussian10(n=2000, d=16)
+    e, o = _lockstep(x, chunk=500, mode=1, check_every=10)
+    batch = np.arange(2000, dtype=np.uint32)
+    h, t, w, _, _ = e.fuzzy(batch)
+    g = np.random.default_rng(8)
+    emb0 = g.uniform(-10, 10, size=(2000, c)).astype(np.float32)
+    out_g = capi.sgd(emb0, h, t, w, 200, 5, 1.577, 0.895, pull=True)
+    again = capi.sgd(emb0, h, t, w, 200, 5, 1.577, 0.895, pull=True)
+    np.testing.assert_array_equal(out_g, again)
+    out_o = oracle.sgd(emb0, h, t, w, 200, 5, 1.577, 0.895)
+    tg = ev.trustworthiness(x, out_g)
+    to = ev.trustworthiness(x, out_o)
+    assert abs(tg - to) <= 0.01, (tg, to)
+
+
 def test_sgd_many_epochs_statistics_match_oracle():
     # a real fuzzy graph, full schedule with negative sampling: Hogwild runs
     # differ in detail, so compare what the layout preserves
