@@ -217,12 +217,18 @@ bool sgd_launches(int c, float *e, const int32_t *head, const int32_t *tail,
           : pull                 ? SGD_STREAM_NONE
                                  : SGD_STREAM_FALLBACK,
           fx);
+    if (sgd_mode == SGD_PULL && !pull)
+      throw Fail(GM_ERR_HIP, "the row-owner SGD kernel needs its scratch and "
+                             "the row records");
     if (fe == hipErrorInvalidConfiguration && pull) {
       (void)hipGetLastError();
       *variant = 3;
       fe = launch_sgd_pull(c, *pf, fx, head, tail, nnz_p, cap, nrows, eps,
                            nxt, nneg, n_epochs, alpha0, gamma, a, b, nsr_inv,
                            seed, rounding, gbar, st, num_cu, s);
+      if (fe != hipSuccess && sgd_mode == SGD_PULL)
+        throw Fail(GM_ERR_HIP, "the row-owner SGD kernel could not be "
+                               "launched");
       if (fe == hipErrorInvalidConfiguration) {
         (void)hipGetLastError();
         fe = launch_sgd_fused(c, e, head, tail, nnz_p, nrows, eps, nxt, nneg,
@@ -1459,9 +1465,8 @@ struct Engine {
       sgd_setup(nrows, cap, n_epochs, nsr_inv, alpha0, deg_zeroed);
     unsigned long long *st = prof ? kstats : nullptr;
     const bool want_pull =
-        sgd_fix && sgd_pull &&
-        (sgd_mode == SGD_PULL ||
-         (sgd_mode == SGD_AUTO && cap >= PULL_MIN_SLOTS));
+        sgd_fix && (sgd_mode == SGD_PULL ||
+                    (sgd_pull && sgd_mode == SGD_AUTO && cap >= PULL_MIN_SLOTS));
     if (want_pull)
       ensure_pull(std::max<uint64_t>(cap, 2 * N * (uint64_t)k));
     ev_start(GM_KSTAT_SGD);
