@@ -129,8 +129,10 @@ def kernel_numbers(e, capi, d, c):
         bytes_ = pos_ * (24 + 36 * c)
         ms = ss.device_ms / ss.launches
         waves = int(ss.work[3])
-        name = ("k_sgd_waves" if waves == ss.launches else
-                "k_sgd_fused" if waves == 0 else "k_sgd_waves+k_sgd_fused")
+        per_wave = ("k_sgd_waves" if os.environ.get("GPUMAP_AMD_SGD_PIPE") ==
+                    "0" else "k_sgd_pipe")
+        name = (per_wave if waves == ss.launches else
+                "k_sgd_fused" if waves == 0 else per_wave + "+k_sgd_fused")
         out[name] = {
             "launches": int(ss.launches), "avg_ms": ms,
             "positive_samples": int(pos_), "negative_samples": int(neg_),

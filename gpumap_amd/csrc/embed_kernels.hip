@@ -508,12 +508,26 @@ __device__ inline void neg_step(float cur[NC], float g[NC], const float neg[NC],
 
 // Where the SGD reads rows: the float embedding, or its fixed-point copy
 // (FixRows, below); load() gives a row's NC coordinates as floats.
+// load_raw / decode split a load from its use: the raw words stay in
+// registers (the load in flight) until decode, so that a caller can issue
+// rows long before it needs them.
 template <int NC> struct EmbRows {
   const float *__restrict__ emb;
+  static constexpr int RAW = NC;
   __device__ void load(uint64_t row, float out[NC]) const {
 #pragma unroll
     for (int d = 0; d < NC; ++d)
       out[d] = emb[row * NC + d];
+  }
+  __device__ void load_raw(uint64_t row, int32_t out[RAW]) const {
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      out[d] = __float_as_int(emb[row * NC + d]);
+  }
+  __device__ void decode(const int32_t in[RAW], float out[NC]) const {
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      out[d] = __int_as_float(in[d]);
   }
 };
 
@@ -670,6 +684,12 @@ __device__ inline Seg seg_of(int32_t key) {
       m |= 1u << s;
   }
   return Seg{m, start};
+}
+
+// v of lane ^ 1 (a DPP quad permutation, with no LDS round trip)
+__device__ inline float lane_xor1(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xf,
+                                                 0xf, false));
 }
 
 // v from lane + 2^S of the same 16-lane row (0 past the row's end)
@@ -878,7 +898,7 @@ __device__ inline void agg_add_buf(__amdgpu_buffer_rsrc_t r, const Agg &a,
   float v1[NC], v2[NC];
 #pragma unroll
   for (int d = 0; d < NC; ++d) {
-    const float tot = gh[d] + __shfl_xor(gt[d], 1, 64);
+    const float tot = gh[d] + lane_xor1(gt[d]);
     v1[d] = a.paired ? tot : gt[d];
     v2[d] = a.paired ? 0.0f : gh[d];
   }
@@ -930,6 +950,7 @@ __device__ inline unsigned long long fix_pack(int32_t lo, int32_t hi) {
 template <int NC> struct FixRows {
   const int32_t *__restrict__ rec;
   float q;
+  static constexpr int RAW = 2 * fix_stride<NC>();
   __device__ void load(uint64_t row, float out[NC]) const {
     constexpr int F = fix_stride<NC>();
     const int32_t *r = rec + row * (2 * F);
@@ -938,6 +959,20 @@ template <int NC> struct FixRows {
 #pragma unroll
     for (int d = 0; d < NC; ++d)
       out[d] = __int_as_float(r[d]) + (float)D[d] * q;
+  }
+  __device__ void load_raw(uint64_t row, int32_t out[RAW]) const {
+    const int32_t *r = rec + row * RAW;
+#pragma unroll
+    for (int d = 0; d < RAW; ++d)
+      out[d] = r[d];
+  }
+  __device__ void decode(const int32_t in[RAW], float out[NC]) const {
+    constexpr int F = fix_stride<NC>();
+    int32_t D[NC];
+    fix_decode<NC>(in + F, D);
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      out[d] = __int_as_float(in[d]) + (float)D[d] * q;
   }
 };
 
@@ -978,7 +1013,7 @@ __device__ inline void agg_add_fix(int32_t *__restrict__ rec, float qinv,
   float v1[NC], v2[NC];
 #pragma unroll
   for (int d = 0; d < NC; ++d) {
-    const float tot = gh[d] + __shfl_xor(gt[d], 1, 64);
+    const float tot = gh[d] + lane_xor1(gt[d]);
     v1[d] = a.paired ? tot : gt[d];
     v2[d] = a.paired ? 0.0f : gh[d];
   }
@@ -988,6 +1023,65 @@ __device__ inline void agg_add_fix(int32_t *__restrict__ rec, float qinv,
   const bool w2 = seg_sum<NC, 0>(a.sh, v2);
   fix_row_add<NC>(rec, qinv, k1, w1, v1);
   fix_row_add<NC>(rec, qinv, hh, w2 && !a.paired, v2);
+}
+
+// fix_row_add without branches, for kernels that keep a fixed set of memory
+// operations in flight: buffer atomics on the records, a lane that adds
+// nothing passing an offset past the buffer's end (dropped by the
+// hardware).  Integer buffer atomics have no clang builtin; these are the
+// LLVM intrinsics (no-return forms when the result is unused).
+__device__ long long gm_buf_atomic_add_i64(long long v,
+                                           __amdgpu_buffer_rsrc_t r, int off,
+                                           int soff, int aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.atomic.add.i64");
+__device__ int gm_buf_atomic_add_i32(int v, __amdgpu_buffer_rsrc_t r, int off,
+                                     int soff, int aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.atomic.add.i32");
+
+template <int NC>
+__device__ inline __amdgpu_buffer_rsrc_t fix_rsrc(int32_t *rec, uint32_t n) {
+  // num_records in bytes; records stay below 2^31 bytes
+  return __builtin_amdgcn_make_buffer_rsrc(
+      rec, 0, (int)(n * (2u * fix_stride<NC>() * 4u)), 0x00020000);
+}
+
+template <int NC>
+__device__ inline void fix_row_add_buf(__amdgpu_buffer_rsrc_t r, float qinv,
+                                       int32_t row, bool wr,
+                                       const float v[NC]) {
+  constexpr uint32_t F = fix_stride<NC>();
+  const uint32_t off =
+      (wr && row >= 0) ? ((uint32_t)row * (2u * F) + F) * 4u : BUF_SKIP;
+  int32_t dx[NC];
+#pragma unroll
+  for (int d = 0; d < NC; ++d)
+    dx[d] = (int32_t)rintf(v[d] * qinv);
+#pragma unroll
+  for (int d = 0; d + 1 < NC; d += 2)
+    gm_buf_atomic_add_i64((long long)fix_pack(dx[d], dx[d + 1]), r,
+                          (int)(off + 4u * d), 0, 0);
+  if constexpr (NC & 1)
+    gm_buf_atomic_add_i32(dx[NC - 1], r, (int)(off + 4u * (NC - 1)), 0, 0);
+}
+
+// agg_add_fix with fix_row_add_buf: always the same atomics per call
+template <int NC>
+__device__ inline void agg_add_fix_buf(__amdgpu_buffer_rsrc_t r, float qinv,
+                                       const Agg &a, int32_t hh, int32_t tt,
+                                       float gh[NC], float gt[NC]) {
+  float v1[NC], v2[NC];
+#pragma unroll
+  for (int d = 0; d < NC; ++d) {
+    const float tot = gh[d] + lane_xor1(gt[d]);
+    v1[d] = a.paired ? tot : gt[d];
+    v2[d] = a.paired ? 0.0f : gh[d];
+  }
+  const Seg s1{a.paired ? a.sp.m : a.st.m, a.paired ? a.sp.start : a.st.start};
+  const int32_t k1 = a.paired ? a.kp : tt;
+  const bool w1 = seg_sum<NC, 0>(s1, v1);
+  const bool w2 = seg_sum<NC, 0>(a.sh, v2);
+  fix_row_add_buf<NC>(r, qinv, k1, w1, v1);
+  fix_row_add_buf<NC>(r, qinv, hh, w2 && !a.paired, v2);
 }
 
 // The rows and row updates of one SGD kernel: the float embedding (buffer
@@ -1013,6 +1107,22 @@ template <int NC>
 __device__ inline FixStore<NC> fix_store(int32_t *rec, float rounding) {
   const float q = ldexpf(rounding, -24); // a power of two, as is 1 / q
   return FixStore<NC>{FixRows<NC>{rec, q}, rec, 1.0f / q};
+}
+// FixStore with branch-free row updates (agg_add_fix_buf)
+template <int NC> struct FixStoreBuf {
+  FixRows<NC> rows;
+  __amdgpu_buffer_rsrc_t rr;
+  float qinv;
+  __device__ void add(const Agg &a, int32_t hh, int32_t tt, float gh[NC],
+                      float gt[NC]) const {
+    agg_add_fix_buf<NC>(rr, qinv, a, hh, tt, gh, gt);
+  }
+};
+template <int NC>
+__device__ inline FixStoreBuf<NC> fix_store_buf(int32_t *rec, float rounding,
+                                                uint32_t n) {
+  const float q = ldexpf(rounding, -24);
+  return FixStoreBuf<NC>{FixRows<NC>{rec, q}, fix_rsrc<NC>(rec, n), 1.0f / q};
 }
 
 // The batched kernel's store for the float embedding: row updates by the
@@ -1437,12 +1547,12 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body(
     // twin's total)
 #pragma unroll
     for (int d = 0; d < NC; ++d) {
-      const float twin_t = __shfl_xor(gt[d], 1, 64);
+      const float twin_t = lane_xor1(gt[d]);
       const float pair_h = gh[d] + twin_t;
       own2_h[d] = own1_h[d];
       own2_t[d] = own1_t[d];
       own1_h[d] = ag.paired ? pair_h : gh[d];
-      own1_t[d] = ag.paired ? __shfl_xor(pair_h, 1, 64) : gt[d];
+      own1_t[d] = ag.paired ? lane_xor1(pair_h) : gt[d];
       pgh[d] = gh[d];
       pgt[d] = gt[d];
     }
@@ -1628,6 +1738,565 @@ __global__ __launch_bounds__(64) void k_sgd_waves(
     sgd_waves_body<NC>(st, head, tail, *nnz_p, n, eps, n_epochs, alpha0,
                        gamma, a, b, nsr_inv, seed, *rounding_p, bar + 4,
                        gridDim.x, blockIdx.x, bar + 2, stats, lag);
+  }
+}
+
+// ------------------------------------------------------- pipelined SGD
+// The per-cycle kernel with each group of 64 edges on four waves of one
+// block (the four SIMDs of one CU) that hand work to each other through LDS
+// slots and counters, with no block barrier:
+//   chain wave  the epochs' arithmetic only (each edge's positive step and
+//               its sequential negative steps), inputs read from LDS;
+//   row wave    gathers the edge's own two rows three epochs ahead, hands
+//               them to the chain wave, adds the chain wave's
+//               contributions to the rows, and keeps the launch's blocks
+//               within the drift bound;
+//   draw waves  (two, for the even and the odd epochs) compute the
+//               negative-sample draws (Philox's 32-bit multiplies run at a
+//               quarter of the vector rate: one epoch's draws cost about as
+//               much as the chain wave's epoch), gather the sample rows and
+//               hand them to the chain wave.
+// With one wave per SIMD an epoch of the single-wave kernel costs that
+// wave's whole instruction stream plus the memory latency it cannot hide;
+// split, the chain wave's stream is about a third of it, and the scattered
+// row loads of the other waves overlap it.  A wave waits for another only
+// when it has run ahead of it.
+//
+// Row wave, tick k (k = -3 .. Ep; Ep = n_epochs rounded up to a multiple of
+// 4, the epochs from n_epochs on sample nothing):
+//   (b) once the chain wave has finished epoch k - 3, epoch k + 1's two
+//       rows (loads issued at tick k - 2) into chain slot (k + 1) mod 4;
+//   (c) epoch k + 3's two row loads issued (row 0 for a lane that does not
+//       sample), so that (b)'s wait counts exactly the loads after its own;
+//   (d) once the chain wave has finished epoch k - 1, its contributions
+//       (ring slot (k - 1) mod 4) added to the rows;
+//   (e) every fourth tick, the arrival: row updates of epochs < k issued;
+//   (f) every fourth tick (k = 0 mod 4) a read of the other blocks' arrival
+//       slots is issued, taken in at tick k + 3 (its wait then covers only
+//       operations issued at least two ticks earlier), and before (c) the
+//       drift bound: epoch k + 3's rows are read once every other block's
+//       slot shows k + 3 - lag (read again, at once, until it does).
+//       A slot read stays in flight for three ticks: waiting for it at
+//       once would wait for every operation issued before it.
+// Epoch g's own rows are loaded after the row updates of epoch g - 5 and
+// before those of epoch g - 4 (one wave, issue order), so the chain wave
+// adds its own pair's share of epochs g - 4 .. g - 1 to what they read and,
+// on a conflict-free edge set, follows the sequential trajectory exactly.
+// A draw wave loads epoch g's sample rows after the row wave's tick g - 3,
+// so after its drift bound for epoch g.  Other blocks' updates are at most
+// lag + 1 epochs old when read (a Hogwild read).  A block whose row
+// wave is at tick k' has stored at least k' - 4, so with a lag >= 7 the
+// row wave furthest behind never waits for another block and the launch
+// always progresses.  With four chain slots the row and draw waves hand
+// over an epoch while the chain wave is still two epochs before it, so the
+// chain wave waits only when they fall behind.  Every wait is bounded: a
+// time-out sets *timeout, and the wave's later waits are skipped.
+template <int NC> struct PipeSlot {
+  float cur[NC][64], oth[NC][64], neg[8][NC][64];
+  // n_neg when the lane samples (bits 0..15) and the usable negative
+  // samples (bits 16..23: q < min(8, n_neg) and not the head), else -1
+  int meta[64];
+  int qmax; // 1 + the highest usable sample slot over the wave (0: none)
+  int many; // some lane has more than 8 negative samples
+};
+
+// The chain wave's epoch: sgd_apply_rows without branches per lane.  Every
+// lane runs the positive step and the wave's negative-sample steps q <
+// qmax, and a lane keeps only the steps it takes (act, usable sample q):
+// the same operations as sgd_apply_rows on the lanes that take them.  At
+// d^2 = 0 the coefficient is computed and discarded (the select keeps 0,
+// as sgd_apply_rows' branch does).
+template <int NC, typename Rows>
+__device__ inline void pipe_apply(const Rows &rw, int32_t j, uint64_t e,
+                                  uint32_t n, float ep_neg, float alpha,
+                                  float gamma, float a, float b,
+                                  uint64_t seed, float rounding, bool act,
+                                  int n_neg, int vm, int qmax, bool many,
+                                  EdgeLoads<NC> &L, float gh[NC],
+                                  float gt[NC], unsigned &c_pos,
+                                  unsigned &c_neg) {
+  c_pos += act ? 1u : 0u;
+  c_neg += (unsigned)__popc((unsigned)vm);
+  float g[NC];
+  float d2 = 0.0f;
+#pragma unroll
+  for (int d = 0; d < NC; ++d) {
+    const float t = L.cur[d] - L.oth[d];
+    d2 += t * t;
+  }
+  const float pb = pow_pos(d2, b);
+  const float attr_c =
+      (-2.0f * a * b * pb) * __builtin_amdgcn_rcpf(d2 * (a * pb + 1.0f));
+  const float attr = d2 > 0.0f ? attr_c : 0.0f;
+#pragma unroll
+  for (int d = 0; d < NC; ++d) {
+    const float gp = fclip(attr * (L.cur[d] - L.oth[d])) * alpha;
+    g[d] = act ? gp : 0.0f;
+    L.cur[d] += g[d];
+    gt[d] = (rounding + -g[d]) - rounding;
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    if (q >= qmax)
+      break;
+    const bool take = (vm >> q) & 1;
+    float d2n = 0.0f;
+#pragma unroll
+    for (int d = 0; d < NC; ++d) {
+      const float t = L.cur[d] - L.neg[q][d];
+      d2n += t * t;
+    }
+    const float rep_c =
+        (2.0f * gamma * b) *
+        __builtin_amdgcn_rcpf((0.001f + d2n) * (a * pow_pos(d2n, b) + 1.0f));
+    const float rep = d2n > 0.0f ? rep_c : 0.0f;
+#pragma unroll
+    for (int d = 0; d < NC; ++d) {
+      const float gg = rep > 0.0f
+                           ? fclip(rep * (L.cur[d] - L.neg[q][d])) * alpha
+                           : 4.0f * alpha;
+      const float gk = take ? gg : 0.0f;
+      L.cur[d] += gk;
+      g[d] += gk;
+    }
+  }
+  if (many && act && n_neg > 8) {
+    // more than 8 negative samples (an edge skipped many epochs): the
+    // stream continues at Philox block 2
+    Rng rng(seed, (uint32_t)e, (uint32_t)(e >> 32), TAG_NEG);
+    rng.ctr[2] = 2u;
+    for (int p = 8; p < n_neg; ++p) {
+      const uint32_t r = __umulhi(rng.next(), n);
+      if ((int32_t)r == j)
+        continue;
+      ++c_neg;
+      float neg[NC];
+      rw.load(r, neg);
+      neg_step<NC>(L.cur, g, neg, alpha, gamma, a, b);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < NC; ++d)
+    gh[d] = (rounding + g[d]) - rounding;
+}
+struct PipeSync {
+  int c_done;    // epochs the chain wave has finished
+  int m_hand;    // epochs whose own rows are in their chain slot
+  int m_done;    // row-wave ticks finished, + 4 (tick k done: k + 4)
+  int w_hand[2]; // draw wave h: its last epoch in its chain slot, + 1
+};
+// the block's LDS state (one allocation per kernel, whichever row store)
+template <int NC> struct PipeShared {
+  PipeSlot<NC> ps[4];       // chain slots, epoch g in slot g mod 4
+  float ctr[4][2 * NC][64]; // the chain wave's contributions, g mod 4
+  PipeSync sy;
+};
+// the drift bound's default lag (GPUMAP_AMD_PIPE_LAG overrides it, at least
+// PIPE_LAG_MIN).  In steady state a slot read is up to six ticks old when
+// used and each slot trails its block by up to four ticks, so a small lag
+// repeats reads at once; a large one lets blocks drift apart, which was
+// slower too (70,000 x 784 per-cycle sets: 0.44, 0.37, 0.36, 0.39, 0.41 ms
+// per launch at lags 7, 8, 10, 12, 16)
+constexpr int PIPE_LAG = 10;
+constexpr int PIPE_LAG_MIN = 7;
+constexpr unsigned PIPE_SPIN_MAX = 1u << 22;
+
+// wait until *p >= target, a counter that another wave of the block raises
+__device__ inline void pipe_wait(const int *p, int target, bool &gave_up,
+                                 unsigned *__restrict__ timeout) {
+  if (gave_up)
+    return;
+  unsigned spins = 0;
+  while (__builtin_amdgcn_readfirstlane(__hip_atomic_load(
+             p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) < target) {
+    if (++spins > PIPE_SPIN_MAX) {
+      if ((threadIdx.x & 63) == 0)
+        __hip_atomic_store(timeout, 1u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      gave_up = true;
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  // the slot reads below stay after the counter read (compiler order; the
+  // wave has waited for the counter's value)
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// both counters, read together (one LDS round trip per poll)
+__device__ inline void pipe_wait2(const int *p1, int t1, const int *p2, int t2,
+                                  bool &gave_up,
+                                  unsigned *__restrict__ timeout) {
+  if (gave_up)
+    return;
+  unsigned spins = 0;
+  for (;;) {
+    const int v1 = __hip_atomic_load(p1, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+    const int v2 = __hip_atomic_load(p2, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (__builtin_amdgcn_readfirstlane(v1 >= t1 && v2 >= t2))
+      break;
+    if (++spins > PIPE_SPIN_MAX) {
+      if ((threadIdx.x & 63) == 0)
+        __hip_atomic_store(timeout, 1u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      gave_up = true;
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// The slot writes before it become visible no later than the counter: one
+// wave's LDS operations are performed in issue order, so only the
+// compiler's order needs holding (no wait for the writes).
+__device__ inline void pipe_post(int *p, int v) {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// the smallest of a read of the slots over the wave, slot w left out:
+// DPP row shifts within each 16-lane row, then the four rows' minima
+// (no LDS round trips)
+__device__ inline unsigned pipe_slots_min(const unsigned sv[8], unsigned nw,
+                                          unsigned w, int lane) {
+  unsigned m = ~0u;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const unsigned idx = (unsigned)lane + 64u * i;
+    m = min(m, (idx >= nw || idx == w) ? ~0u : sv[i]);
+  }
+  // row_shr:1, 2, 4, 8 (a lane without a source keeps its own value)
+  m = min(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x111, 0xf,
+                                                   0xf, false));
+  m = min(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x112, 0xf,
+                                                   0xf, false));
+  m = min(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x114, 0xf,
+                                                   0xf, false));
+  m = min(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x118, 0xf,
+                                                   0xf, false));
+  // lane 15 of each row holds its row's minimum
+  const unsigned r0 = __builtin_amdgcn_readlane(m, 15);
+  const unsigned r1 = __builtin_amdgcn_readlane(m, 31);
+  const unsigned r2 = __builtin_amdgcn_readlane(m, 47);
+  const unsigned r3 = __builtin_amdgcn_readlane(m, 63);
+  return min(min(r0, r1), min(r2, r3));
+}
+
+// the smallest arrival slot of the other blocks (slot w left out), read now
+__device__ inline unsigned pipe_slot_min(const unsigned *__restrict__ slots,
+                                         unsigned nw, unsigned w, int lane) {
+  unsigned sv[8];
+  poll_slots(slots, nw, lane, sv);
+  return pipe_slots_min(sv, nw, w, lane);
+}
+
+template <int NC, typename Store>
+__device__ __attribute__((always_inline)) inline void sgd_pipe_body(
+    const Store &st, const int32_t *__restrict__ head,
+    const int32_t *__restrict__ tail, uint64_t nnz, uint32_t n,
+    const float *__restrict__ eps, int n_epochs, float alpha0, float gamma,
+    float a, float b, float nsr_inv, uint64_t seed, float rounding,
+    unsigned *__restrict__ slots, unsigned nw, unsigned w,
+    unsigned *__restrict__ timeout, unsigned long long *__restrict__ stats,
+    int lag, PipeShared<NC> &sh) {
+  PipeSlot<NC>(&ps)[4] = sh.ps;
+  float(&ctr)[4][2 * NC][64] = sh.ctr;
+  PipeSync &sy = sh.sy;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint64_t e = (uint64_t)w * 64 + lane;
+  int32_t hh, tt;
+  float ep;
+  if (e < nnz && head[e] >= 0) {
+    hh = head[e];
+    tt = tail[e];
+    ep = eps[e];
+  } else {
+    hh = tt = -1 - lane;
+    ep = -1.0f;
+  }
+  const float ep_neg = ep * nsr_inv;
+  const int E = n_epochs;
+  const int Ep = (E + 3) & ~3;
+  // every edge's schedule, run by the row wave and both draw waves alike:
+  // whether epoch g samples, and its negative-sample count
+  float nx = ep > 0.0f ? ep : INFINITY, nn = ep * nsr_inv;
+  auto schedule = [&](int g, int &n_neg) {
+    const bool act = g < E && ep > 0.0f && !(nx > (float)g);
+    n_neg = act ? (int)(((float)g - nn) / ep_neg) : 0;
+    if (act)
+      advance_schedule(ep, ep_neg, n_neg, nx, nn);
+    return act;
+  };
+  if (threadIdx.x == 0)
+    sy = PipeSync{0, 0, 0, {0, 0}};
+  if (wv == 1) {
+    // the row wave's adds of ticks -3 .. 0 (epochs -4 .. -1) add zeros
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int d = 0; d < 2 * NC; ++d)
+        ctr[c][d][lane] = 0.0f;
+    __hip_atomic_store(slots + w, 0u, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  bool gave_up = false;
+  if (wv == 0) {
+    // ---------------------------------------------------------- chain
+    const Agg ag = agg_of(hh, tt);
+    unsigned c_pos = 0, c_neg = 0;
+    float own_h[4][NC], own_t[4][NC];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int d = 0; d < NC; ++d)
+        own_h[k][d] = own_t[k][d] = 0.0f;
+    float alpha = alpha0;
+    auto step = [&](int g, auto ring) {
+      constexpr int R = decltype(ring)::value; // g mod 4
+      pipe_wait2(&sy.m_hand, g + 1, &sy.w_hand[g & 1], g + 1, gave_up,
+                 timeout);
+      const PipeSlot<NC> &sl = ps[g & 3];
+      const int meta = sl.meta[lane];
+      const int qmax = __builtin_amdgcn_readfirstlane(sl.qmax);
+      const bool many = __builtin_amdgcn_readfirstlane(sl.many) != 0;
+      const bool act = meta >= 0;
+      EdgeLoads<NC> L;
+#pragma unroll
+      for (int d = 0; d < NC; ++d) {
+        L.cur[d] = sl.cur[d][lane] + own_h[0][d] + own_h[1][d] + own_h[2][d] +
+                   own_h[3][d];
+        L.oth[d] = sl.oth[d][lane] + own_t[0][d] + own_t[1][d] + own_t[2][d] +
+                   own_t[3][d];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+#pragma unroll
+        for (int d = 0; d < NC; ++d)
+          L.neg[q][d] = sl.neg[q][d][lane];
+      float gh[NC], gt[NC];
+      pipe_apply<NC>(st.rows, hh, e, n, ep_neg, alpha, gamma, a, b,
+                     seed + (uint64_t)g, rounding, act, act ? meta & 0xffff : 0,
+                     act ? (meta >> 16) & 0xff : 0, qmax, many, L, gh, gt,
+                     c_pos, c_neg);
+#pragma unroll
+      for (int d = 0; d < NC; ++d) {
+        ctr[R][d][lane] = gh[d];
+        ctr[R][NC + d][lane] = gt[d];
+        // this epoch's share of the pair's rows (paired: the head row gets
+        // the edge's head and the twin's tail contributions, the tail row
+        // the twin's total)
+        const float twin_t = lane_xor1(gt[d]);
+        const float pair_h = gh[d] + twin_t;
+        own_h[R][d] = ag.paired ? pair_h : gh[d];
+        own_t[R][d] = ag.paired ? lane_xor1(pair_h) : gt[d];
+      }
+      alpha = alpha0 * (1.0f - (float)g / (float)E);
+      pipe_post(&sy.c_done, g + 1);
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    for (int g = 0; g < Ep; g += 4) {
+      step(g, I0{});
+      step(g + 1, I1{});
+      step(g + 2, I2{});
+      step(g + 3, I3{});
+    }
+    flush_sgd_stats(stats, c_pos, c_neg);
+  } else if (wv == 1) {
+    // ------------------------------------------------------------ rows
+    const Agg ag = agg_of(hh, tt);
+    using Rows = std::remove_cv_t<std::remove_reference_t<decltype(st.rows)>>;
+    constexpr int RW = Rows::RAW;
+    // two epochs' own rows (cur, oth) as raw words, in flight from (c) to
+    // (b)
+    int32_t G0[2][RW], G1[2][RW];
+    int next_g = 0; // the next epoch whose schedule is to run
+    unsigned sv[8];    // a read of the slots in flight
+    unsigned smin = 0; // the smallest other slot at the last read taken in
+    unsigned spins = 0;
+    bool slots_gave_up = false;
+    // tick phase k mod 4 = Q: (e) at Q = 3, the slot read issued at Q = 0
+    // and taken in at Q = 3; before the loop Q = -1 (no slot work) and -2
+    // (the slot's starting value stored again, so that the loop is entered
+    // with the memory operations in flight that it repeats with)
+    auto tick = [&](int k, auto par, auto hb, auto hc, auto hd, auto phase) {
+      constexpr int P = decltype(par)::value;   // k mod 2
+      constexpr int Q = decltype(phase)::value; // k mod 4, or -1, -2
+      int32_t(&G)[2][RW] = P ? G1 : G0;
+      // (b)
+      if constexpr (decltype(hb)::value) {
+        if (k + 1 < Ep) {
+          pipe_wait(&sy.c_done, k - 2, gave_up, timeout);
+          PipeSlot<NC> &sl = ps[(k + 1) & 3];
+          float v[NC];
+          st.rows.decode(G[0], v);
+#pragma unroll
+          for (int d = 0; d < NC; ++d)
+            sl.cur[d][lane] = v[d];
+          st.rows.decode(G[1], v);
+#pragma unroll
+          for (int d = 0; d < NC; ++d)
+            sl.oth[d][lane] = v[d];
+          pipe_post(&sy.m_hand, k + 2);
+        }
+      }
+      // (f): the read issued at tick k - 3 taken in, and the drift bound
+      if constexpr (Q == 3)
+        smin = pipe_slots_min(sv, nw, w, lane);
+      if constexpr (decltype(hc)::value) {
+        if (k + 3 > lag) {
+          const unsigned target = (unsigned)(k + 3 - lag);
+          while (!slots_gave_up && smin < target) {
+            if ((++spins & 1023u) == 0 &&
+                (spins > (1u << 20) ||
+                 __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                     timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))) {
+              if (lane == 0)
+                __hip_atomic_store(timeout, 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+              slots_gave_up = true;
+              break;
+            }
+            smin = pipe_slot_min(slots, nw, w, lane);
+            if (smin < target)
+              __builtin_amdgcn_s_sleep(1);
+          }
+        }
+        // (c): the schedule through epoch k + 3
+        bool act = false;
+        for (; next_g <= k + 3; ++next_g) {
+          int n_neg;
+          act = schedule(next_g, n_neg);
+        }
+        st.rows.load_raw(act ? (uint64_t)hh : 0, G[0]);
+        st.rows.load_raw(act ? (uint64_t)tt : 0, G[1]);
+      }
+      // (d)
+      if constexpr (decltype(hd)::value) {
+        pipe_wait(&sy.c_done, k, gave_up, timeout);
+        float gh[NC], gt[NC];
+        const int c = (k - 1) & 3;
+#pragma unroll
+        for (int d = 0; d < NC; ++d) {
+          gh[d] = ctr[c][d][lane];
+          gt[d] = ctr[c][NC + d][lane];
+        }
+        st.add(ag, hh, tt, gh, gt);
+      }
+      // (e)
+      if constexpr (Q == 3 || Q == -2)
+        __hip_atomic_store(slots + w, Q == 3 ? (unsigned)k : 0u,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // (f): the slot read, issued after this tick's loads and updates
+      if constexpr (Q == 0)
+        poll_slots(slots, nw, lane, sv);
+      pipe_post(&sy.m_done, k + 4);
+    };
+    using B0 = std::integral_constant<int, 0>;
+    using B1 = std::integral_constant<int, 1>;
+    using T = std::integral_constant<bool, true>;
+    using F = std::integral_constant<bool, false>;
+    using QN = std::integral_constant<int, -1>;
+    using QS = std::integral_constant<int, -2>;
+    using Q0 = std::integral_constant<int, 0>;
+    using Q1 = std::integral_constant<int, 1>;
+    using Q2 = std::integral_constant<int, 2>;
+    using Q3 = std::integral_constant<int, 3>;
+    tick(-3, B1{}, F{}, T{}, T{}, QN{});
+    tick(-2, B0{}, F{}, T{}, T{}, QN{});
+    tick(-1, B1{}, T{}, T{}, T{}, QS{});
+    for (int k = 0; k < Ep; k += 4) {
+      tick(k, B0{}, T{}, T{}, T{}, Q0{});
+      tick(k + 1, B1{}, T{}, T{}, T{}, Q1{});
+      tick(k + 2, B0{}, T{}, T{}, T{}, Q2{});
+      tick(k + 3, B1{}, T{}, T{}, T{}, Q3{});
+    }
+    tick(Ep, B0{}, F{}, F{}, T{}, QN{});
+    __hip_atomic_store(slots + w, (unsigned)Ep + 8u, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    // ----------------------------------------------------------- draws
+    // draw wave h takes the epochs g = h mod 2 and runs the schedule of
+    // every epoch
+    const int h = wv - 2;
+    using Rows = std::remove_cv_t<std::remove_reference_t<decltype(st.rows)>>;
+    constexpr int RW = Rows::RAW;
+    for (int g = 0; g < Ep; ++g) {
+      int n_neg;
+      const bool act = schedule(g, n_neg);
+      if ((g & 1) != h)
+        continue;
+      uint32_t dr[8];
+      neg_draws(seed + (uint64_t)g, e, dr);
+      const int cnt = min(8, n_neg);
+      int vm = 0;
+      uint32_t r[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const uint32_t rq = __umulhi(dr[q], n);
+        r[q] = q < cnt ? rq : 0u;
+        vm |= (q < cnt && (int32_t)rq != hh) ? (1 << q) : 0;
+      }
+      // the sample rows are read after the row wave's drift bound for
+      // epoch g (its tick g - 3)
+      pipe_wait(&sy.m_done, g + 1, gave_up, timeout);
+      int32_t Gn[8][RW];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        st.rows.load_raw(r[q], Gn[q]);
+      // chain slot g mod 4 is free once the chain wave has finished g - 4
+      pipe_wait(&sy.c_done, g - 3, gave_up, timeout);
+      PipeSlot<NC> &sl = ps[g & 3];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        float v[NC];
+        st.rows.decode(Gn[q], v);
+#pragma unroll
+        for (int d = 0; d < NC; ++d)
+          sl.neg[q][d][lane] = v[d];
+      }
+      sl.meta[lane] = act ? (n_neg | (vm << 16)) : -1;
+      // wave-wide values, computed by every lane
+      int qmax = 0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        qmax = __ballot((vm >> q) & 1) ? q + 1 : qmax;
+      const int many = __ballot(act && n_neg > 8) ? 1 : 0;
+      if (lane == 0) {
+        sl.qmax = qmax;
+        sl.many = many;
+      }
+      pipe_post(&sy.w_hand[h], g + 1);
+    }
+  }
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_sgd_pipe(
+    float *__restrict__ emb, const int32_t *__restrict__ head,
+    const int32_t *__restrict__ tail, const uint32_t *__restrict__ nnz_p,
+    uint32_t n, const float *__restrict__ eps, int n_epochs, float alpha0,
+    float gamma, float a, float b, float nsr_inv, uint64_t seed,
+    const float *__restrict__ rounding_p, unsigned *__restrict__ bar,
+    unsigned long long *__restrict__ stats, int lag, FixFrame fx) {
+  __shared__ PipeShared<NC> sh;
+  if (fx.rec) {
+    const FixStoreBuf<NC> st = fix_store_buf<NC>(fx.rec, *rounding_p, n);
+    sgd_pipe_body<NC>(st, head, tail, *nnz_p, n, eps, n_epochs, alpha0, gamma,
+                      a, b, nsr_inv, seed, *rounding_p, bar + 4, gridDim.x,
+                      blockIdx.x, bar + 2, stats, lag, sh);
+  } else {
+    const FloatStore<NC> st{EmbRows<NC>{emb}, emb_rsrc<NC>(emb, n)};
+    sgd_pipe_body<NC>(st, head, tail, *nnz_p, n, eps, n_epochs, alpha0, gamma,
+                      a, b, nsr_inv, seed, *rounding_p, bar + 4, gridDim.x,
+                      blockIdx.x, bar + 2, stats, lag, sh);
   }
 }
 
@@ -2185,6 +2854,13 @@ hipError_t launch_sgd_pull(int nc, const PullFrame &pf, FixFrame fx,
 // thread when its co-resident grid holds every edge, else the streaming
 // kernel on the co-resident grid.  Returns hipSuccess, or
 // the launch error (the caller then launches one kernel per epoch).
+// GPUMAP_AMD_SGD_PIPE=0 keeps the per-cycle SGD on k_sgd_waves (read at
+// each launch)
+static bool sgd_pipe_enabled() {
+  const char *v = std::getenv("GPUMAP_AMD_SGD_PIPE");
+  return !(v && v[0] == '0');
+}
+
 template <typename K> static int blocks_per_cu(K kernel, int threads) {
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -2244,6 +2920,33 @@ static hipError_t try_waves(float *emb, const int32_t *head,
 }
 
 template <int NC>
+static hipError_t try_pipe(float *emb, const int32_t *head,
+                           const int32_t *tail, const uint32_t *nnz_p,
+                           uint32_t n, const float *eps, int n_epochs,
+                           float alpha0, float gamma, float a, float b,
+                           float nsr_inv, uint64_t seed,
+                           const float *rounding_p, unsigned *bar,
+                           unsigned long long *stats, uint64_t nnz,
+                           int num_cu, hipStream_t s, FixFrame fx) {
+  static int per_cu = 0;
+  if (per_cu <= 0)
+    per_cu = blocks_per_cu(&k_sgd_pipe<NC>, 256);
+  const uint64_t need = (nnz + 63) / 64;
+  if (need == 0 || need > SGD_MAX_WAVES ||
+      need > (uint64_t)per_cu * (uint64_t)num_cu)
+    return hipErrorInvalidConfiguration;
+  int lag = PIPE_LAG;
+  if (const char *v = std::getenv("GPUMAP_AMD_PIPE_LAG"))
+    lag = std::max(PIPE_LAG_MIN, std::atoi(v));
+  void *args[] = {&emb,     &head,   &tail,  &nnz_p, &n,       &eps,
+                  &n_epochs, &alpha0, &gamma, &a,     &b,       &nsr_inv,
+                  &seed,    &rounding_p, &bar, &stats, &lag,    &fx};
+  return hipLaunchCooperativeKernel(
+      reinterpret_cast<const void *>(&k_sgd_pipe<NC>), dim3((unsigned)need),
+      dim3(256), args, 0, s);
+}
+
+template <int NC>
 static hipError_t try_stream(float *emb, const int32_t *head,
                              const int32_t *tail, const uint32_t *nnz_p,
                              uint32_t n, const float *eps, float *nxt,
@@ -2288,6 +2991,16 @@ static hipError_t try_fused_any(float *emb, const int32_t *head,
     return try_stream<NC>(emb, head, tail, nnz_p, n, eps, nxt, nneg, n_epochs,
                           alpha0, gamma, a, b, nsr_inv, seed, rounding_p, bar,
                           stats, lag, nnz, num_cu, s, fx);
+  if (sgd_pipe_enabled()) {
+    // small edge sets: chain, row and draw waves (variant 4)
+    *variant = 4;
+    const hipError_t e =
+        try_pipe<NC>(emb, head, tail, nnz_p, n, eps, n_epochs, alpha0, gamma,
+                     a, b, nsr_inv, seed, rounding_p, bar, stats, nnz, num_cu,
+                     s, fx);
+    if (e != hipErrorInvalidConfiguration)
+      return e;
+  }
   *variant = 2;
   {
     // small edge sets: per-wave arrival slots, twice the counter kernels' lag

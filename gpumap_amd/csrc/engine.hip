@@ -187,7 +187,8 @@ void set_last_error(const char *msg) {
 // the row-owner kernel (SGD_PULL), or one launch per epoch (SGD_PER_EPOCH,
 // and whenever no cooperative launch is possible).  Returns true when a
 // cooperative kernel ran (its timeout flag is gbar[2]); *variant names it
-// (2 per-wave slots, 0 resident counter grid, 1 streaming, 3 row-owner).
+// (4 pipelined chain/row/draw waves, 2 per-wave slots, 0 resident counter grid, 1
+// streaming, 3 row-owner).
 // Epoch lag of the cooperative SGD: a block starts epoch e + 1 once every
 // block has finished epoch e - SGD_LAG (0 would be a full grid barrier per
 // epoch, the reference's one launch per epoch).  The Hogwild updates are
@@ -1479,7 +1480,7 @@ struct Engine {
     const bool large = fused && (sgd_variant == 1 || sgd_variant == 3);
     if (fused) {
       fused_used = true;
-      if (sgd_variant == 2)
+      if (sgd_variant == 2 || sgd_variant == 4)
         ++sgd_waves_launches;
       if (prof && large)
         pending.back().which = GM_KSTAT_SGD_STREAM;

@@ -255,13 +255,14 @@ int gm_sgd_noalloc(float *emb, uint64_t n, int32_t c, const int32_t *head,
  *                        queued pool offers, pairs re-ranked in FP32 after
  *                        the BF16 screen (GM_KNN_NNDESCENT_BF16)}
  *   GM_KSTAT_SGD         SGD with a register-resident cooperative kernel
- *                        (one launch per embedding op: k_sgd_waves for up to
- *                        512 waves of edges, else k_sgd_fused) or, when
+ *                        (one launch per embedding op: for up to 512 groups
+ *                        of 64 edges k_sgd_pipe, or k_sgd_waves with
+ *                        GPUMAP_AMD_SGD_PIPE=0, else k_sgd_fused) or, when
  *                        cooperative launch is unavailable, one launch per
  *                        epoch; work = {positive samples, negative samples,
  *                        COO slot-epochs scanned (edges and the empty slots
  *                        of the twin-paired COO), launches that ran
- *                        k_sgd_waves}
+ *                        k_sgd_pipe or k_sgd_waves}
  *   GM_KSTAT_INSERT      insertions (all kernels of one insertion); work =
  *                        {(batch row, row) pairs, BF16 screening survivors
  *                        re-ranked in FP32, insertions redone by the FP32

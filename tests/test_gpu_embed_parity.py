@@ -121,15 +121,20 @@ def test_sgd_with_negative_samples_matches_oracle():
     np.testing.assert_array_equal(got[16:], emb[16:])
 
 
-@pytest.mark.parametrize("kernel,epochs", [("waves", 30), ("stream", 30),
-                                           ("pull", 30), ("pull", 29)])
-def test_sgd_cooperative_kernels_match_oracle(kernel, epochs):
-    # the cooperative kernels (per-wave k_sgd_waves, streaming k_sgd_stream,
-    # row-owner k_sgd_pull): k_sgd_waves reads an epoch's positions before the
-    # previous epoch's updates have landed and adds its own edge's last step
-    # to them, k_sgd_pull reads every row as the previous epoch left it, so
-    # on the conflict-free set all follow the oracle's trajectory (29 epochs:
-    # k_sgd_pull's sums end in its second buffer)
+@pytest.mark.parametrize("kernel,epochs", [
+    ("pipe", 30), ("pipe", 29), ("pipe", 1), ("pipe", 5), ("waves", 30),
+    ("stream", 30), ("pull", 30), ("pull", 29)])
+def test_sgd_cooperative_kernels_match_oracle(kernel, epochs, monkeypatch):
+    # the cooperative kernels (pipelined k_sgd_pipe, per-wave k_sgd_waves,
+    # streaming k_sgd_stream, row-owner k_sgd_pull): k_sgd_pipe and
+    # k_sgd_waves read an epoch's positions before the last epochs' updates
+    # of their own edges have landed and add their own edges' steps of those
+    # epochs, k_sgd_pull reads every row as the previous epoch left it, so
+    # on the conflict-free set all follow the oracle's trajectory (29
+    # epochs: k_sgd_pull's sums end in its second buffer; k_sgd_pipe: 1 and
+    # 5 epochs are shorter than its pipeline, 29 and 30 end before its
+    # padding to a multiple of 4)
+    monkeypatch.setenv("GPUMAP_AMD_SGD_PIPE", "0" if kernel == "waves" else "1")
     emb, head, tail, w, seed, _ = _conflict_free_case()
     want = oracle.sgd(emb, head, tail, w, epochs, 5, 1.577, 0.895, seed=seed)
     got = capi.sgd(emb, head, tail, w, epochs, 5, 1.577, 0.895, seed=seed,
