@@ -1800,6 +1800,12 @@ template <int NC> struct PipeSlot {
   int many; // some lane has more than 8 negative samples
 };
 
+// fclip with one clamp instruction (v_med3: equal to min then max for every
+// finite value; a value that is not finite is 0, as in fclip)
+__device__ inline float fclip_med3(float v) {
+  return isfinite(v) ? __builtin_amdgcn_fmed3f(v, -4.0f, 4.0f) : 0.0f;
+}
+
 // The chain wave's epoch: sgd_apply_rows without branches per lane.  Every
 // lane runs the positive step and the wave's negative-sample steps q <
 // qmax, and a lane keeps only the steps it takes (act, usable sample q):
@@ -1830,7 +1836,7 @@ __device__ inline void pipe_apply(const Rows &rw, int32_t j, uint64_t e,
   const float attr = d2 > 0.0f ? attr_c : 0.0f;
 #pragma unroll
   for (int d = 0; d < NC; ++d) {
-    const float gp = fclip(attr * (L.cur[d] - L.oth[d])) * alpha;
+    const float gp = fclip_med3(attr * (L.cur[d] - L.oth[d])) * alpha;
     g[d] = act ? gp : 0.0f;
     L.cur[d] += g[d];
     gt[d] = (rounding + -g[d]) - rounding;
@@ -1853,7 +1859,7 @@ __device__ inline void pipe_apply(const Rows &rw, int32_t j, uint64_t e,
 #pragma unroll
     for (int d = 0; d < NC; ++d) {
       const float gg = rep > 0.0f
-                           ? fclip(rep * (L.cur[d] - L.neg[q][d])) * alpha
+                           ? fclip_med3(rep * (L.cur[d] - L.neg[q][d])) * alpha
                            : 4.0f * alpha;
       const float gk = take ? gg : 0.0f;
       L.cur[d] += gk;
@@ -2078,6 +2084,14 @@ __device__ __attribute__((always_inline)) inline void sgd_pipe_body(
 #pragma unroll
         for (int d = 0; d < NC; ++d)
           L.neg[q][d] = sl.neg[q][d][lane];
+      // every slot value read now, with one wait: left to itself the
+      // compiler moves each sample's read into its step (behind the step's
+      // uniform exit) and waits an LDS round trip per step
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+#pragma unroll
+        for (int d = 0; d < NC; ++d)
+          __asm__ volatile("" ::"v"(L.neg[q][d]));
       float gh[NC], gt[NC];
       pipe_apply<NC>(st.rows, hh, e, n, ep_neg, alpha, gamma, a, b,
                      seed + (uint64_t)g, rounding, act, act ? meta & 0xffff : 0,
