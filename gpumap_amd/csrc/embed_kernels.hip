@@ -2949,6 +2949,11 @@ static hipError_t try_pipe(float *emb, const int32_t *head,
   if (need == 0 || need > SGD_MAX_WAVES ||
       need > (uint64_t)per_cu * (uint64_t)num_cu)
     return hipErrorInvalidConfiguration;
+  // the row updates address the records through a buffer of at most 2^31
+  // bytes (larger sessions take k_sgd_waves, whose atomics use 64-bit
+  // addresses)
+  if (fx.rec && (uint64_t)n * (2u * fix_stride<NC>() * 4u) >= 0x80000000ull)
+    return hipErrorInvalidConfiguration;
   int lag = PIPE_LAG;
   if (const char *v = std::getenv("GPUMAP_AMD_PIPE_LAG"))
     lag = std::max(PIPE_LAG_MIN, std::atoi(v));
