@@ -86,14 +86,14 @@ def test_noise_and_reposition_match_oracle(metric, graph, monkeypatch):
         assert gs.launches == 0
 
 
-def _conflict_free_case(n=1 << 16, seed0=100):
+def _conflict_free_case(n=1 << 16, seed0=100, c=2):
     """Eight edges on rows 0..15 with relative weights in W_REL-like ratios,
-    positions spread wide; a seed whose negative samples never land on an
-    edge's rows (checked with the oracle's trace), so that the device's
-    Hogwild order cannot matter."""
+    positions spread wide (c components); a seed whose negative samples never
+    land on an edge's rows (checked with the oracle's trace), so that the
+    device's Hogwild order cannot matter."""
     g = np.random.default_rng(38)
-    emb = g.uniform(-10, 10, size=(n, 2)).astype(np.float32)
-    emb[:16] = g.uniform(-2, 2, size=(16, 2))
+    emb = g.uniform(-10, 10, size=(n, c)).astype(np.float32)
+    emb[:16] = g.uniform(-2, 2, size=(16, c))
     head = np.arange(0, 16, 2, dtype=np.int32)
     tail = head + 1
     w = np.array((W_REL * 2)[:8], np.float32)
@@ -140,6 +140,30 @@ def test_sgd_cooperative_kernels_match_oracle(kernel, epochs, monkeypatch):
     got = capi.sgd(emb, head, tail, w, epochs, 5, 1.577, 0.895, seed=seed,
                    stream=kernel == "stream", pull=kernel == "pull")
     np.testing.assert_array_equal(got[16:], emb[16:])
+    print(f"max |device - oracle| {np.abs(got[:16] - want[:16]).max():.3g}",
+          flush=True)
+    np.testing.assert_allclose(got[:16], want[:16], rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("c,fix", [(1, True), (3, True), (4, True),
+                                   (2, False), (3, False)])
+def test_sgd_pipelined_kernel_matches_oracle_in_every_width(c, fix,
+                                                            monkeypatch):
+    # k_sgd_pipe's other instantiations: 1, 3 and 4 components on the
+    # fixed-point records (64-bit buffer atomics per coordinate pair, a 32-bit
+    # one for an odd last coordinate) and the float embedding's buffer float
+    # atomics (GPUMAP_AMD_SGD_FIX=0); the conflict-free set, 30 epochs (20 in
+    # one dimension: past about epoch 20 this 1-D trajectory turns the
+    # device's few-ulp pow and reciprocal into 1.5e-3 in every device path,
+    # one launch per epoch included)
+    monkeypatch.setenv("GPUMAP_AMD_SGD_PIPE", "1")
+    monkeypatch.setenv("GPUMAP_AMD_SGD_FIX", "1" if fix else "0")
+    epochs = 20 if c == 1 else 30
+    emb, head, tail, w, seed, _ = _conflict_free_case(c=c)
+    want = oracle.sgd(emb, head, tail, w, epochs, 5, 1.577, 0.895, seed=seed)
+    got = capi.sgd(emb, head, tail, w, epochs, 5, 1.577, 0.895, seed=seed)
+    np.testing.assert_array_equal(got[16:], emb[16:])
+    assert np.abs(want[:16] - emb[:16]).max() > 0.5
     print(f"max |device - oracle| {np.abs(got[:16] - want[:16]).max():.3g}",
           flush=True)
     np.testing.assert_allclose(got[:16], want[:16], rtol=1e-4, atol=1e-4)
