@@ -1118,6 +1118,10 @@ template <int NC> struct FixStoreBuf {
     agg_add_fix_buf<NC>(rr, qinv, a, hh, tt, gh, gt);
   }
 };
+// whether n rows' records fit one buffer descriptor (below 2^31 bytes)
+template <int NC> __host__ __device__ inline bool fix_buf_fits(uint64_t n) {
+  return n * (2u * fix_stride<NC>() * 4u) < 0x80000000ull;
+}
 template <int NC>
 __device__ inline FixStoreBuf<NC> fix_store_buf(int32_t *rec, float rounding,
                                                 uint32_t n) {
@@ -2952,7 +2956,7 @@ static hipError_t try_pipe(float *emb, const int32_t *head,
   // the row updates address the records through a buffer of at most 2^31
   // bytes (larger sessions take k_sgd_waves, whose atomics use 64-bit
   // addresses)
-  if (fx.rec && (uint64_t)n * (2u * fix_stride<NC>() * 4u) >= 0x80000000ull)
+  if (fx.rec && !fix_buf_fits<NC>(n))
     return hipErrorInvalidConfiguration;
   int lag = PIPE_LAG;
   if (const char *v = std::getenv("GPUMAP_AMD_PIPE_LAG"))
