@@ -1905,9 +1905,10 @@ template <int NC> struct PipeShared {
 // PIPE_LAG_MIN).  In steady state a slot read is up to six ticks old when
 // used and each slot trails its block by up to four ticks, so a small lag
 // repeats reads at once; a large one lets blocks drift apart, which was
-// slower too (70,000 x 784 per-cycle sets: 0.44, 0.37, 0.36, 0.39, 0.41 ms
-// per launch at lags 7, 8, 10, 12, 16)
-constexpr int PIPE_LAG = 10;
+// slower too, and slots that move in steps of four make the lag's residue
+// mod 4 matter (70,000 x 784 per-cycle sets, ms per launch: lag 7 0.44,
+// 8 0.37, 9 0.335, 10 0.355, 11 0.42, 12 0.39, 16 0.41)
+constexpr int PIPE_LAG = 9;
 constexpr int PIPE_LAG_MIN = 7;
 constexpr unsigned PIPE_SPIN_MAX = 1u << 22;
 
