@@ -1446,6 +1446,29 @@ __device__ inline unsigned slot_min(const unsigned sv[8], unsigned nw,
 // SGD_POLL - 1 epochs, so the slowest wave never waits and every wave is at
 // most lag + 2 (SGD_POLL - 1) epochs ahead of it.  The wait is bounded
 // (*timeout is set, as epoch_end's bar[2]).
+// the smallest of eight values per lane over the wave: DPP row shifts within
+// each 16-lane row, then the four rows' minima (no LDS round trips)
+__device__ inline unsigned wave_min_dpp(const unsigned v[8]) {
+  unsigned m = v[0];
+#pragma unroll
+  for (int i = 1; i < 8; ++i)
+    m = min(m, v[i]);
+  // row_shr:1, 2, 4, 8 (a lane without a source keeps its own value)
+  m = min(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x111, 0xf,
+                                                   0xf, false));
+  m = min(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x112, 0xf,
+                                                   0xf, false));
+  m = min(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x114, 0xf,
+                                                   0xf, false));
+  m = min(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x118, 0xf,
+                                                   0xf, false));
+  // lane 15 of each row holds its row's minimum
+  return min(min(__builtin_amdgcn_readlane(m, 15),
+                 __builtin_amdgcn_readlane(m, 31)),
+             min(__builtin_amdgcn_readlane(m, 47),
+                 __builtin_amdgcn_readlane(m, 63)));
+}
+
 constexpr int SGD_POLL = 4;
 
 template <int NC, typename Store>
@@ -1687,14 +1710,7 @@ __device__ __attribute__((always_inline)) inline void sgd_waves_body_batch(
       const unsigned target = (unsigned)(epoch - lag + 1);
       unsigned spins = 0;
       for (;;) {
-        unsigned m = sv[0];
-#pragma unroll
-        for (int i = 1; i < 8; ++i)
-          m = min(m, sv[i]);
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
-          m = min(m, (unsigned)__shfl_xor((int)m, off, 64));
-        if (m >= target)
+        if (wave_min_dpp(sv) >= target)
           break;
         // bounded: a timed-out wait anywhere in the launch ends the others
         // within 1024 polls
