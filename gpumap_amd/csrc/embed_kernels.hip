@@ -1984,32 +1984,16 @@ __device__ inline void pipe_post(int *p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// the smallest of a read of the slots over the wave, slot w left out:
-// DPP row shifts within each 16-lane row, then the four rows' minima
-// (no LDS round trips)
+// the smallest of a read of the slots over the wave, slot w left out
 __device__ inline unsigned pipe_slots_min(const unsigned sv[8], unsigned nw,
                                           unsigned w, int lane) {
-  unsigned m = ~0u;
+  unsigned v[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const unsigned idx = (unsigned)lane + 64u * i;
-    m = min(m, (idx >= nw || idx == w) ? ~0u : sv[i]);
+    v[i] = (idx >= nw || idx == w) ? ~0u : sv[i];
   }
-  // row_shr:1, 2, 4, 8 (a lane without a source keeps its own value)
-  m = min(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x111, 0xf,
-                                                   0xf, false));
-  m = min(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x112, 0xf,
-                                                   0xf, false));
-  m = min(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x114, 0xf,
-                                                   0xf, false));
-  m = min(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x118, 0xf,
-                                                   0xf, false));
-  // lane 15 of each row holds its row's minimum
-  const unsigned r0 = __builtin_amdgcn_readlane(m, 15);
-  const unsigned r1 = __builtin_amdgcn_readlane(m, 31);
-  const unsigned r2 = __builtin_amdgcn_readlane(m, 47);
-  const unsigned r3 = __builtin_amdgcn_readlane(m, 63);
-  return min(min(r0, r1), min(r2, r3));
+  return wave_min_dpp(v);
 }
 
 // the smallest arrival slot of the other blocks (slot w left out), read now
