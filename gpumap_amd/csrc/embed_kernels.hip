@@ -2889,6 +2889,25 @@ template <typename K> static int blocks_per_cu(K kernel, int threads) {
   return per_cu;
 }
 
+// Launch of a per-cycle grid-wide SGD kernel (k_sgd_pipe, k_sgd_waves,
+// k_sgd_waves_multi), whose blocks wait for one another.  Its callers keep
+// the grid within the occupancy the device holds at once and hold the device
+// exclusively while it runs (no other kernel of this process is in flight),
+// so every block is resident without the cooperative launch's guarantee; and
+// every wait in these kernels is bounded, reporting a timeout rather than
+// hanging.  The plain launch leaves the device idle for less time around the
+// kernel: 0.320 against 0.334 ms per k_sgd_pipe launch at 70,000 rows (HIP
+// events around the SGD's launch sequence, MI355X).  GPUMAP_AMD_SGD_COOP=1
+// restores the cooperative launch.
+static hipError_t grid_launch(const void *fn, unsigned blocks,
+                              unsigned threads, void **args, hipStream_t s) {
+  const char *v = std::getenv("GPUMAP_AMD_SGD_COOP");
+  if (v && v[0] == '1')
+    return hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(threads), args,
+                                      0, s);
+  return hipLaunchKernel(fn, dim3(blocks), dim3(threads), args, 0, s);
+}
+
 template <int NC, int EPT>
 static hipError_t try_fused(float *emb, const int32_t *head,
                             const int32_t *tail, const uint32_t *nnz_p,
@@ -2933,9 +2952,8 @@ static hipError_t try_waves(float *emb, const int32_t *head,
   void *args[] = {&emb,     &head,   &tail,  &nnz_p, &n,       &eps,
                   &n_epochs, &alpha0, &gamma, &a,     &b,       &nsr_inv,
                   &seed,    &rounding_p, &bar, &stats, &lag,    &fx};
-  return hipLaunchCooperativeKernel(
-      reinterpret_cast<const void *>(&k_sgd_waves<NC>), dim3((unsigned)need),
-      dim3(64), args, 0, s);
+  return grid_launch(reinterpret_cast<const void *>(&k_sgd_waves<NC>),
+                     (unsigned)need, 64, args, s);
 }
 
 template <int NC>
@@ -2965,9 +2983,8 @@ static hipError_t try_pipe(float *emb, const int32_t *head,
   void *args[] = {&emb,     &head,   &tail,  &nnz_p, &n,       &eps,
                   &n_epochs, &alpha0, &gamma, &a,     &b,       &nsr_inv,
                   &seed,    &rounding_p, &bar, &stats, &lag,    &fx};
-  return hipLaunchCooperativeKernel(
-      reinterpret_cast<const void *>(&k_sgd_pipe<NC>), dim3((unsigned)need),
-      dim3(256), args, 0, s);
+  return grid_launch(reinterpret_cast<const void *>(&k_sgd_pipe<NC>),
+                     (unsigned)need, 256, args, s);
 }
 
 template <int NC>
@@ -3066,9 +3083,8 @@ static hipError_t try_batch(const SgdBatch &batch, unsigned waves,
     return hipErrorInvalidConfiguration;
   SgdBatch bb = batch;
   void *args[] = {&bb, &bar, &lag};
-  return hipLaunchCooperativeKernel(
-      reinterpret_cast<const void *>(&k_sgd_waves_multi<NC>), dim3(waves),
-      dim3(64), args, 0, s);
+  return grid_launch(reinterpret_cast<const void *>(&k_sgd_waves_multi<NC>),
+                     waves, 64, args, s);
 }
 
 hipError_t launch_sgd_batch(int nc, const SgdBatch &batch, unsigned waves,

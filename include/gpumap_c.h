@@ -141,7 +141,7 @@ int gm_run(gm_handle *h, uint32_t n_insertion_ops, uint32_t n_update_ops,
  * session i runs ops[3i], ops[3i+1], ops[3i+2] insertion, update and
  * embedding ops exactly as gm_run would, with the same results, except that
  * the three phase times are the batch's.  The sessions' insertion kernels
- * overlap on the device and their per-cycle SGDs run as one cooperative
+ * overlap on the device and their per-cycle SGDs run as one grid-wide
  * launch, each paced only by its own edges.  All sessions must be on one
  * device and distinct; calls on one device are serialised. */
 int gm_run_batch(gm_handle *const *handles, int32_t count,
@@ -213,9 +213,9 @@ int gm_debug_fuzzy(gm_handle *h, const uint32_t *batch, uint64_t n_batch,
 
 /* SGD epochs of the engine on a caller-given embedding (n x c, host) and COO
  * (host), with the engine's kernels: per_epoch 0 runs all epochs in one
- * cooperative launch of the kernel that fits the edge set (one launch per
+ * grid-wide launch of the kernel that fits the edge set (one launch per
  * epoch when none can be launched), 1 one launch per epoch, 2 the streaming
- * cooperative kernel, 3 the row-owner kernel the engine uses for edge sets
+ * grid-wide kernel, 3 the row-owner kernel the engine uses for edge sets
  * beyond the resident grid, at any size.  A slot with head = tail = -1 is
  * empty (the
  * engine's own COO keeps each edge's reverse in the slot after it, or an
@@ -229,7 +229,7 @@ int gm_debug_sgd(float *emb, uint64_t n, int32_t c, const int32_t *head,
  * kernels and launch shapes, on the caller's device arrays (emb n x c float32,
  * updated in place; head, tail int32 and w float32 of nnz COO slots, all in
  * device memory), with the schedule, head degrees, rounding grid, the
- * cooperative kernels' barrier words and row records in one static device
+ * grid-wide kernels' barrier words and row records in one static device
  * array per device (about 48 MiB, held until the process ends).  No counterpart in the
  * reference; it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
@@ -254,11 +254,11 @@ int gm_sgd_noalloc(float *emb, uint64_t n, int32_t c, const int32_t *head,
  *   GM_KSTAT_JOIN        local join; work = {list members, distance pairs,
  *                        queued pool offers, pairs re-ranked in FP32 after
  *                        the BF16 screen (GM_KNN_NNDESCENT_BF16)}
- *   GM_KSTAT_SGD         SGD with a register-resident cooperative kernel
+ *   GM_KSTAT_SGD         SGD with a register-resident grid-wide kernel
  *                        (one launch per embedding op: for up to 512 groups
  *                        of 64 edges k_sgd_pipe, or k_sgd_waves with
  *                        GPUMAP_AMD_SGD_PIPE=0, else k_sgd_fused) or, when
- *                        cooperative launch is unavailable, one launch per
+ *                        a grid-wide launch is unavailable, one launch per
  *                        epoch; work = {positive samples, negative samples,
  *                        COO slot-epochs scanned (edges and the empty slots
  *                        of the twin-paired COO), launches that ran

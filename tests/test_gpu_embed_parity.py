@@ -122,8 +122,9 @@ def test_sgd_with_negative_samples_matches_oracle():
 
 
 @pytest.mark.parametrize("kernel,epochs", [
-    ("pipe", 30), ("pipe", 29), ("pipe", 1), ("pipe", 5), ("waves", 30),
-    ("stream", 30), ("pull", 30), ("pull", 29)])
+    ("pipe", 30), ("pipe", 29), ("pipe", 1), ("pipe", 5), ("pipe_coop", 30),
+    ("waves", 30), ("waves_coop", 30), ("stream", 30), ("pull", 30),
+    ("pull", 29)])
 def test_sgd_cooperative_kernels_match_oracle(kernel, epochs, monkeypatch):
     # the cooperative kernels (pipelined k_sgd_pipe, per-wave k_sgd_waves,
     # streaming k_sgd_stream, row-owner k_sgd_pull): k_sgd_pipe and
@@ -133,8 +134,12 @@ def test_sgd_cooperative_kernels_match_oracle(kernel, epochs, monkeypatch):
     # on the conflict-free set all follow the oracle's trajectory (29
     # epochs: k_sgd_pull's sums end in its second buffer; k_sgd_pipe: 1 and
     # 5 epochs are shorter than its pipeline, 29 and 30 end before its
-    # padding to a multiple of 4)
-    monkeypatch.setenv("GPUMAP_AMD_SGD_PIPE", "0" if kernel == "waves" else "1")
+    # padding to a multiple of 4); the per-cycle kernels with a plain launch
+    # (default) and the cooperative one (GPUMAP_AMD_SGD_COOP=1)
+    monkeypatch.setenv("GPUMAP_AMD_SGD_PIPE",
+                       "0" if kernel.startswith("waves") else "1")
+    monkeypatch.setenv("GPUMAP_AMD_SGD_COOP",
+                       "1" if kernel.endswith("_coop") else "0")
     emb, head, tail, w, seed, _ = _conflict_free_case()
     want = oracle.sgd(emb, head, tail, w, epochs, 5, 1.577, 0.895, seed=seed)
     got = capi.sgd(emb, head, tail, w, epochs, 5, 1.577, 0.895, seed=seed,
