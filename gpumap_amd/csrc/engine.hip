@@ -69,6 +69,22 @@ inline unsigned blocks_for(uint64_t n, unsigned tpb) {
   return (unsigned)((n + tpb - 1) / tpb);
 }
 
+// The end of a phase's device work: the stream is polled for up to 2 ms (a
+// cycle's phases end within that), then waited for by blocking.
+inline void stream_wait(hipStream_t s) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    const hipError_t e = hipStreamQuery(s);
+    if (e == hipSuccess)
+      return;
+    if (e != hipErrorNotReady)
+      HIPC(e);
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2))
+      break;
+  }
+  HIPC(hipStreamSynchronize(s));
+}
+
 // reference UpdateQueue (queue.cuh:46-80): FIFO without duplicates; the
 // membership test is a byte per row (rows are < the session's capacity)
 struct DedupQueue {
@@ -1693,7 +1709,7 @@ struct Engine {
       check_live();
       auto t0 = clk::now();
       r->n_insertion_ops = insert(ni);
-      HIPC(hipStreamSynchronize(s));
+      stream_wait(s);
       insert_finish();
       r->insertion_time = std::chrono::duration<float>(clk::now() - t0).count();
       insertion_completed = insertion_queue.size() == 0 && n == N;
@@ -1704,7 +1720,7 @@ struct Engine {
       uint64_t ch = 0;
       auto t0 = clk::now();
       r->n_update_ops = update(nu, ch);
-      HIPC(hipStreamSynchronize(s));
+      stream_wait(s);
       r->update_time = std::chrono::duration<float>(clk::now() - t0).count();
       r->n_update_changed_edges = ch;
     }
@@ -1715,7 +1731,7 @@ struct Engine {
       check_live();
       auto t0 = clk::now();
       r->n_embedding_ops = embed(ne);
-      HIPC(hipStreamSynchronize(s));
+      stream_wait(s);
       r->embedding_time = std::chrono::duration<float>(clk::now() - t0).count();
     }
     finish_result(r);
@@ -1985,7 +2001,7 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
   for (int i = 0; i < cnt; ++i) {
     Engine &E = *es[i];
     if (did[i]) {
-      HIPC(hipStreamSynchronize(E.s));
+      stream_wait(E.s);
       E.insert_finish();
       rs[i].insertion_time = secs(tb);
       E.insertion_completed = E.insertion_queue.size() == 0 && E.n == E.N;
@@ -1995,7 +2011,7 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
       uint64_t ch = 0;
       rs[i].n_update_ops = E.update(ops[3 * i + 1], ch);
       rs[i].n_update_changed_edges = ch;
-      HIPC(hipStreamSynchronize(E.s));
+      stream_wait(E.s);
       rs[i].update_time = secs(t0);
     }
     rs[i].embedding_pending_edges = E.n;
@@ -2023,7 +2039,7 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
     };
     rs[i].n_embedding_ops = E.embed_begin(ops[3 * i + 2], &jobs[i], quiesce);
     if (!jobs[i].nwaves) {
-      HIPC(hipStreamSynchronize(E.s));
+      stream_wait(E.s);
       continue;
     }
     size_t g = group_of(E.c);
@@ -2049,8 +2065,8 @@ void run_batch(Engine *const *es, int cnt, const uint32_t *ops,
       es[i]->embed_end();
   for (int i = 0; i < cnt; ++i)
     if (emb[i])
-      HIPC(hipStreamSynchronize(es[i]->s));
-  HIPC(hipStreamSynchronize(bc.s));
+      stream_wait(es[i]->s);
+  stream_wait(bc.s);
   for (int i = 0; i < cnt; ++i)
     if (emb[i])
       rs[i].embedding_time = secs(te[i]);
