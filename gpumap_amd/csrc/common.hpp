@@ -249,6 +249,31 @@ __device__ inline int pool_offer_half(uint64_t *__restrict__ row,
 // reader resolves (sigma_of), so the value stored here depends on the row's
 // public kNN list only.  Sums are xor-shuffle trees; the bisection's
 // decisions are uniform over the half-wave.  All 32 lanes must call this.
+
+// One lane's double from the lane `CTRL` names (a DPP row pattern).
+template <int CTRL> __device__ inline double dpp_f64(double x) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(
+      0, (int)(unsigned)u, CTRL, 0xf, 0xf, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(
+      0, (int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(
+      (long long)(((unsigned long long)hi << 32) | (unsigned long long)lo));
+}
+
+// Sum of a double over each 16-lane row, every lane of the row receiving
+// it: rotations within the row by 8, 4, 2 and 1 lanes.  After the rotation
+// by 8 the values repeat every 8 lanes (then every 4, 2), so each rotation
+// hands a lane the value its xor partner holds, and the sums are those of
+// the xor-shuffle tree over the row, operand for operand.
+__device__ inline double row16_sum(double v) {
+  v += dpp_f64<0x128>(v); // row_ror:8
+  v += dpp_f64<0x124>(v); // row_ror:4
+  v += dpp_f64<0x122>(v); // row_ror:2
+  v += dpp_f64<0x121>(v); // row_ror:1
+  return v;
+}
+
 __device__ inline void row_sigma_rho(double di, int c, int k, int half,
                                      double &sigma_out, double &rho_out) {
   double sum = di;
@@ -267,12 +292,23 @@ __device__ inline void row_sigma_rho(double di, int c, int k, int half,
   const double target = log2((double)(k - 1));
   const bool term = c >= 2 && c < k;
   const double x = di - r;
+  // At k <= 16 every term sits in lanes 2 .. 15, the tree's first step (xor
+  // 16) adds exact zeros, and what follows is a sum over 16 lanes: lanes c
+  // and c + 16 both take entry c, and each 16-lane row sums by DPP
+  // (row16_sum), with the same result as the 32-lane tree.
+  const bool narrow = k <= 16;
+  const double xn = narrow ? __shfl(x, c & 15, 32) : x;
+  const bool tn = narrow ? ((c & 15) >= 2 && (c & 15) < k) : term;
   double lo = 0.0, hi = INFINITY, mid = 1.0;
   for (int it = 0; it < 64; ++it) {
-    double psum = term ? ((x > 0.0) ? exp(-x / mid) : 1.0) : 0.0;
+    double psum = tn ? ((xn > 0.0) ? exp(-xn / mid) : 1.0) : 0.0;
+    if (narrow) {
+      psum = row16_sum(psum);
+    } else {
 #pragma unroll
-    for (int off = 16; off > 0; off >>= 1)
-      psum += __shfl_xor(psum, off, 32);
+      for (int off = 16; off > 0; off >>= 1)
+        psum += __shfl_xor(psum, off, 32);
+    }
     if (fabs(psum - target) < 1e-5)
       break;
     if (psum > target) {

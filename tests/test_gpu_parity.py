@@ -215,11 +215,13 @@ def _fp64_union(idx, dist, sig, rho, batch, k):
     return out, ed
 
 
-def test_fuzzy_set_matches_oracle_and_fp64():
+@pytest.mark.parametrize("k", [15, 16, 17])
+def test_fuzzy_set_matches_oracle_and_fp64(k):
+    # sigma's bisection sums over 16-lane rows by DPP at k <= 16 and over the
+    # 32-lane half-wave above
     x = ev.gaussian10(n=3000, d=32)
-    e, o = _lockstep(x, chunk=1000, check_every=10, mode=0)
+    e, o = _lockstep(x, chunk=1000, k=k, check_every=10, mode=0)
     idx, dist = e.knn()
-    k = 15
     for batch in (np.arange(3000, dtype=np.uint32),
                   np.arange(5, 3000, 7, dtype=np.uint32)):
         h1, t1, w1, s1, r1 = e.fuzzy(batch)
