@@ -525,11 +525,15 @@ __global__ __launch_bounds__(256) void k_bf16_lbscan(
 // tau of one batch row: the 32nd smallest of its nsplit * PARTW upper bounds
 // (+inf when it has fewer than 32 other rows), found by bisection over the
 // float bit patterns (all values are >= 0) with block-wide counts; the
-// values stay in registers and no bin is contended.
+// values stay in registers and no bin is contended.  A wave counts with
+// ballots (one per register, popcounts on the scalar unit), and the four
+// waves' counts meet in LDS behind one barrier per step (two alternating
+// count slots: a wave writes a slot again only after the next step's
+// barrier, which every wave reaches after reading it).
 __global__ __launch_bounds__(256) void k_tau_select(
     const float *__restrict__ part, int nsplit, float *__restrict__ tau,
     uint64_t nb) {
-  __shared__ unsigned s_cnt[4];
+  __shared__ unsigned s_cnt[2][4];
   const uint64_t r = blockIdx.x;
   if (r >= nb)
     return;
@@ -552,14 +556,12 @@ __global__ __launch_bounds__(256) void k_tau_select(
     unsigned c = 0;
 #pragma unroll
     for (int i = 0; i < PER; ++i)
-      c += x[i] <= mid ? 1u : 0u;
-    for (int off = 32; off > 0; off >>= 1)
-      c += __shfl_xor(c, off, 64);
+      c += (unsigned)__popcll(__ballot(x[i] <= mid));
+    unsigned *sc = s_cnt[it & 1];
     if ((threadIdx.x & 63) == 0)
-      s_cnt[threadIdx.x >> 6] = c;
+      sc[threadIdx.x >> 6] = c;
     __syncthreads();
-    const unsigned tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-    __syncthreads();
+    const unsigned tot = sc[0] + sc[1] + sc[2] + sc[3];
     if (tot >= (unsigned)W)
       hi = mid;
     else
