@@ -257,31 +257,33 @@ __global__ void k_coo_write(const uint32_t *__restrict__ pidx, int k,
                             int32_t *__restrict__ head,
                             int32_t *__restrict__ tail,
                             float *__restrict__ wout) {
-  if (threadIdx.x >= 64)
-    return;
+  // a 1024-thread block: wave w places entries [64 w, 64 w + 64) of the
+  // block's 1024, after the block's earlier waves' entries
+  __shared__ uint32_t s_cnt[16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const float thr = coo_thr(maxw_bits, n_epochs);
-  const uint64_t t0 = (uint64_t)blockIdx.x * 1024;
+  const uint64_t s = (uint64_t)blockIdx.x * 1024 + threadIdx.x;
+  const bool f = s < nd && coo_keep(w, flags, s, thr);
+  const uint64_t m = __ballot(f);
+  if (lane == 0)
+    s_cnt[wv] = (uint32_t)__popcll(m);
+  __syncthreads();
   uint32_t at = boff[blockIdx.x];
-  for (int i = 0; i < 1024; i += 64) {
-    const uint64_t s = t0 + i + threadIdx.x;
-    const bool f = s < nd && coo_keep(w, flags, s, thr);
-    const uint64_t m = __ballot(f);
-    if (f) {
-      const int32_t hi = (int32_t)batch[s / k];
-      const int c = (int)(s % k);
-      const int32_t hj = c == 0 ? hi : (int32_t)pidx[(uint64_t)hi * k + c - 1];
-      const bool rev = flags[s] & 2;
-      const float ws = w[s];
-      const uint32_t pos =
-          at + 2u * __popcll(m & ((1ull << threadIdx.x) - 1ull));
-      head[pos] = hi;
-      tail[pos] = hj;
-      wout[pos] = ws;
-      head[pos + 1] = rev ? hj : -1;
-      tail[pos + 1] = rev ? hi : -1;
-      wout[pos + 1] = rev ? ws : 0.0f;
-    }
-    at += 2u * __popcll(m);
+  for (int v = 0; v < wv; ++v)
+    at += 2u * s_cnt[v];
+  if (f) {
+    const int32_t hi = (int32_t)batch[s / k];
+    const int c = (int)(s % k);
+    const int32_t hj = c == 0 ? hi : (int32_t)pidx[(uint64_t)hi * k + c - 1];
+    const bool rev = flags[s] & 2;
+    const float ws = w[s];
+    const uint32_t pos = at + 2u * __popcll(m & ((1ull << lane) - 1ull));
+    head[pos] = hi;
+    tail[pos] = hj;
+    wout[pos] = ws;
+    head[pos + 1] = rev ? hj : -1;
+    tail[pos + 1] = rev ? hi : -1;
+    wout[pos + 1] = rev ? ws : 0.0f;
   }
 }
 

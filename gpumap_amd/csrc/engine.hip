@@ -1436,7 +1436,7 @@ struct Engine {
     LAUNCH_CHECK();
     k_scan_small<<<1, 1024, 0, s>>>(bcount, nblk, scal + S_NNZ);
     LAUNCH_CHECK();
-    k_coo_write<<<nblk, 64, 0, s>>>(pidx, k, dbatch, w, wflags, nd,
+    k_coo_write<<<nblk, 1024, 0, s>>>(pidx, k, dbatch, w, wflags, nd,
                                     scal + S_MAXW, (int)p.max_epoch, bcount,
                                     head, tail, wout);
     LAUNCH_CHECK();

@@ -771,39 +771,44 @@ __global__ __launch_bounds__(64) void k_init_embedding(float *__restrict__ emb, 
   for (int d = 0; d < 4; ++d)
     if (d < c)
       e[d] = rng.normal();
-  // the row's k entries, read four at a time as independent loads (entries
-  // past k read entry 0 and are skipped); pre-batch neighbours only
-  float dmin = 3.402823466e38f;
-  for (int j0 = 0; j0 < k; j0 += 4) {
+  // the row's k entries, read sixteen at a time as independent loads
+  // (entries past k read entry 0 and are skipped; at k <= 16 the first
+  // pass's loads serve the second); pre-batch neighbours only
+  constexpr int KC = 16;
+  uint32_t ids[KC];
+  float dv[KC];
+  auto load = [&](int j0) {
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < KC; ++t) {
       const uint64_t at = u * k + (j0 + t < k ? j0 + t : 0);
-      const uint32_t id = pidx[at];
-      const float dv = pdist[at];
-      if (j0 + t < k && id < b0)
-        dmin = fminf(dmin, dv);
+      ids[t] = pidx[at];
+      dv[t] = pdist[at];
     }
+  };
+  float dmin = 3.402823466e38f;
+  for (int j0 = 0; j0 < k; j0 += KC) {
+    load(j0);
+#pragma unroll
+    for (int t = 0; t < KC; ++t)
+      if (j0 + t < k && ids[t] < b0)
+        dmin = fminf(dmin, dv[t]);
   }
   if (dmin != 3.402823466e38f) {
     float wsum = 0.0f, acc[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int j0 = 0; j0 < k; j0 += 4) {
-      uint32_t ids[4];
-      float dv[4], ev[4][4];
+    for (int j0 = 0; j0 < k; j0 += KC) {
+      if (k > KC)
+        load(j0);
+      float ev[KC][4];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const uint64_t at = u * k + (j0 + t < k ? j0 + t : 0);
-        ids[t] = pidx[at];
-        dv[t] = pdist[at];
-      }
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
+      for (int t = 0; t < KC; ++t) {
         const uint64_t r = ids[t] < b0 ? ids[t] : 0;
 #pragma unroll
         for (int d = 0; d < 4; ++d)
           ev[t][d] = d < c ? emb[r * c + d] : 0.0f;
       }
+      // the weighted sums in entry order
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
+      for (int t = 0; t < KC; ++t) {
         if (j0 + t >= k || ids[t] >= b0)
           continue;
         const float wj = expf(-(dv[t] - dmin));
