@@ -921,7 +921,7 @@ struct Engine {
     LAUNCH_CHECK();
     k_scan_small<<<1, 1024, 0, s>>>(bcount, nblk, scal + S_TOTAL);
     LAUNCH_CHECK();
-    k_flag_write<<<nblk, 64, 0, s>>>(flag, 0, hi, bcount, list, f0, f1);
+    k_flag_write<<<nblk, 1024, 0, s>>>(flag, 0, hi, bcount, list, f0, f1);
     LAUNCH_CHECK();
   }
 

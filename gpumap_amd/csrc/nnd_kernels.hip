@@ -736,20 +736,22 @@ __global__ void k_flag_write(uint8_t *__restrict__ flag, uint64_t lo,
                              uint64_t hi, const uint32_t *__restrict__ boff,
                              uint32_t *__restrict__ out, uint64_t f0,
                              uint64_t f1) {
-  // one wave walks the block's 1024 rows in order, 64 at a time
-  const uint64_t r0 = lo + (uint64_t)blockIdx.x * 1024;
-  if (threadIdx.x >= 64)
-    return;
+  // a 1024-thread block: wave w lists rows [64 w, 64 w + 64) of the
+  // block's 1024, after the block's earlier waves' rows
+  __shared__ uint32_t s_cnt[16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint64_t r = lo + (uint64_t)blockIdx.x * 1024 + threadIdx.x;
+  const bool f = r < hi && (flag[r] || (r >= f0 && r < f1));
+  const uint64_t m = __ballot(f);
+  if (lane == 0)
+    s_cnt[wv] = (uint32_t)__popcll(m);
+  __syncthreads();
   uint32_t at = boff[blockIdx.x];
-  for (int i = 0; i < 1024; i += 64) {
-    const uint64_t r = r0 + i + threadIdx.x;
-    const bool f = r < hi && (flag[r] || (r >= f0 && r < f1));
-    const uint64_t m = __ballot(f);
-    if (f) {
-      out[at + __popcll(m & ((1ull << threadIdx.x) - 1ull))] = (uint32_t)r;
-      flag[r] = 0;
-    }
-    at += __popcll(m);
+  for (int v = 0; v < wv; ++v)
+    at += s_cnt[v];
+  if (f) {
+    out[at + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)r;
+    flag[r] = 0;
   }
 }
 
